@@ -1,0 +1,115 @@
+/*
+ * C-ABI of librsa_mpi_amd.so: the single-rank bias-free RSA hot path of
+ * MarcJos/RSA_MPI on one MI355X (gfx950).
+ *
+ * Every entry point takes plain pointers and sizes, returns 0 on success and a
+ * non-zero code on failure, never throws, and leaves a thread-local message in
+ * rsa_gpu_last_error().  Host arrays are caller-owned and copied.  One context
+ * owns all device buffers of one domain; it is not thread-safe.
+ *
+ * The C++ drop-in headers under include/rsa_mpi_amd/ forward the reference's
+ * operator API to these functions; the citation on each line names the
+ * reference interface it replaces.
+ */
+#ifndef RSA_MPI_AMD_H
+#define RSA_MPI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rsa_gpu_ctx rsa_gpu_ctx;
+
+/* Per-proceed statistics (the quantities proceed_voxel logs or asserts,
+ * operator_algorithm.ixx:104-187). */
+typedef struct rsa_gpu_stats {
+  int64_t loop_iterations;  /* draw-loop iterations, zero-shot ones included */
+  int64_t draws;            /* iterations that drew at least one shot */
+  int64_t shots;            /* total candidate centres drawn */
+  int64_t accepted;         /* spheres validated by this proceed */
+  int64_t refines;          /* remove_covered + subdivide_uncovered passes */
+  int64_t rounds;           /* add_to_sample rounds over all draws */
+  int64_t rounds_max;       /* largest number of rounds in one draw */
+  int64_t nvox_max;         /* largest voxel-list length */
+  int64_t nvox_final;       /* voxel-list length at exit (0 = fully packed) */
+  int64_t words;            /* MT19937 words consumed */
+  int64_t ghost_cell_drops; /* candidates whose centre rounded into a ghost cell */
+  int64_t image_cell_mismatch; /* periodic images whose computed cell is not the mirrored one */
+  int end_reason;           /* 1 fully packed, 2 lack of new radii, 3 draw limit */
+  double seconds;           /* wall time of rsa_gpu_proceed */
+  double voxel_seconds;     /* GPU time in refine passes (device events) */
+} rsa_gpu_stats;
+
+/* Last error message of the calling thread ("" if none). */
+const char* rsa_gpu_last_error(void);
+
+/* Device and build description, e.g. "gfx950 / 256 CUs". */
+const char* rsa_gpu_device_info(void);
+
+/* Periodic domain [inf, sup] of dimension dim (2, 3 or 4) with ghost_layer
+ * (>= 1) cell layers and implicit radius domain_rad.
+ * Replaces rsa_domain<DIM>::rsa_domain (rsa_domain.hxx:53-54, rsa_domain.ixx:6-28). */
+int rsa_gpu_create(int dim, const double* inf, const double* sup, int ghost_layer, double domain_rad,
+                   rsa_gpu_ctx** out);
+
+/* Releases the context and its device memory. */
+void rsa_gpu_destroy(rsa_gpu_ctx* ctx);
+
+/* rsa_algo<DIM> constructor (operator_algorithm.ixx:15-30): installs the radius
+ * classes (radius, count, phase) of a sac_de_billes::RadiusGenerator<DIM>
+ * (radius_generator.ixx:9-28: sorted nonincreasing, exclusion added) and
+ * rebuilds the cell grid with the largest radius, which discards the spheres
+ * already placed.  nbshots_singledraw is m_nbshots_singledraw. */
+int rsa_gpu_algo_init(rsa_gpu_ctx* ctx, int n, const double* radius, const uint64_t* count, const int* phase,
+                      double exclusion, int nbshots_singledraw);
+
+/* RadiusGenerator (radius, volume fraction, phase) form over volume
+ * (radius_generator.ixx:30-36, 71-86), followed by the rsa_algo constructor. */
+int rsa_gpu_algo_init_fractions(rsa_gpu_ctx* ctx, int n, const double* radius, const double* fraction,
+                                const int* phase, double volume, double exclusion, int nbshots_singledraw);
+
+/* rsa_algo<DIM>::reset_radius_generator (operator_algorithm.ixx:238-250): new
+ * classes, placed spheres kept. */
+int rsa_gpu_reset_classes(rsa_gpu_ctx* ctx, int n, const double* radius, const uint64_t* count,
+                          const int* phase, double exclusion);
+
+/* rsa_algo<DIM>::set_miss_rate (operator_algorithm.hxx:66). */
+int rsa_gpu_set_miss_rate(rsa_gpu_ctx* ctx, double miss_rate);
+
+/* rsa_algo<DIM>::proceed<1>(seed) (operator_algorithm.ixx:225-236, 85-188).
+ * max_draws < 0 runs to the end; otherwise stops after that many drawing
+ * iterations (test hook).  stats may be NULL. */
+int rsa_gpu_proceed(rsa_gpu_ctx* ctx, uint64_t seed, int64_t max_draws, rsa_gpu_stats* stats);
+
+/* rsa_domain<DIM>::add (rsa_domain.hxx:84): n spheres (pos is n*dim, row-major)
+ * inserted as validated.  Priorities are not kept, as in the reference. */
+int rsa_gpu_add_spheres(rsa_gpu_ctx* ctx, int64_t n, const double* pos, const double* radius,
+                        const uint64_t* phase);
+
+/* rsa_grid<DIM>::get_number_of_spheres_fast (rsa_grid.ixx:102-121). */
+int rsa_gpu_count(rsa_gpu_ctx* ctx, uint64_t* n);
+
+/* rsa_grid<DIM>::extract_spheres / extract_data<Real> order (rsa_grid.ixx:73-98,
+ * 139-160): real cells in index order, then in-cell order.  pos is cap*dim. */
+int rsa_gpu_extract(rsa_gpu_ctx* ctx, double* pos, double* radius, uint64_t* phase, uint64_t cap,
+                    uint64_t* n);
+
+/* Total volume of the placed spheres (rsa_domain<DIM>::compute_total_volume_of_spheres,
+ * rsa_domain.ixx:170-175, summed per real cell in index order). */
+int rsa_gpu_volume_of_spheres(rsa_gpu_ctx* ctx, double* volume);
+
+/* Current voxel list (list_of_voxels<DIM> state after the last proceed):
+ * coords is cap*dim int64 lattice coordinates, len the common voxel length. */
+int rsa_gpu_voxels(rsa_gpu_ctx* ctx, int64_t* coords, double* len, uint64_t cap, uint64_t* n);
+
+/* Raw MT19937 words of std::mt19937 seeded with seed_seq{rank, seed}
+ * (rsa_random.hxx:37-42), from the library's own generator. */
+int rsa_gpu_mt_words(uint64_t seed, uint64_t rank, uint64_t n, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSA_MPI_AMD_H */

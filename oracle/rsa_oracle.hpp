@@ -1,0 +1,1136 @@
+// TEST INFRASTRUCTURE -- NOT PART OF THE PRODUCT.
+//
+// CPU restatement of the single-rank bias-free RSA voxel path of MarcJos/RSA_MPI.
+// It is the checker for the HIP product in rsa_mpi_amd/: only tests/, the smoke()
+// entry and bench.py's cpu_baseline leg may load it.  It follows the reference's
+// data flow (per-cell storages with swap-with-last removal, ghost areas holding
+// shifted periodic images, round-based conflict resolution) so that the in-cell
+// order, and hence every order-dependent decision, is the reference's.
+//
+// Every function cites the reference file:line it restates.  Third-party
+// arithmetic (std::mt19937, std::seed_seq, std::uniform_int_distribution,
+// std::uniform_real_distribution, std::poisson_distribution, std::sort) is
+// libstdc++ 11 itself, exactly as the reference calls it.
+//
+// Deliberate, documented departures (none changes a result on the inputs the
+// reference handles without undefined behaviour):
+//  * find_solve_conflicts sorts Undecided spheres by (priority, cell, slot)
+//    instead of the reference's randomised tie comparator
+//    (operator_cells_and_data_storage.ixx:101-125).  That comparator is not a
+//    strict weak ordering (it can report a<a), which is undefined behaviour for
+//    std::sort.  Ties never influence the labels: a sphere only looks at
+//    partners with a strictly smaller priority (:39), and all of those are
+//    labelled before it under any priority-sorted order.
+//  * Out-of-grid cell indices (a centre far outside [inf, sup]) throw instead of
+//    writing out of bounds (rsa_grid.ixx:182-187).
+//  * get_list_of_spheres keeps every matching sphere instead of writing past a
+//    2^D*3^D-entry buffer (list_of_voxels.ixx:283-307).
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <numeric>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace rsa_oracle {
+
+template <int D>
+using Pt = std::array<double, D>;
+
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+// ---------------------------------------------------------------- RNG
+// One std::mt19937 per rank seeded with seed_seq{rank, seed}
+// (rsa_random.hxx:37-42).  The wrapper only counts the words drawn; it keeps
+// min()/max()/result_type so every libstdc++ distribution takes the same code
+// path as with a bare std::mt19937.
+struct CountingMT {
+  using result_type = std::mt19937::result_type;
+  std::mt19937 eng;
+  uint64_t used = 0;
+  static constexpr result_type min() { return std::mt19937::min(); }
+  static constexpr result_type max() { return std::mt19937::max(); }
+  result_type operator()() {
+    ++used;
+    return eng();
+  }
+};
+
+inline void seed_engine(CountingMT& g, uint64_t seed, uint64_t rank = 0) {
+  std::seed_seq seq{static_cast<size_t>(rank), static_cast<size_t>(seed)};
+  g.eng.seed(seq);
+  g.used = 0;
+}
+
+// ---------------------------------------------------------------- helpers
+// auxiliary/AuxiFunctions.ixx:34-44 (puissance<D>: split exponent recursion)
+template <int E>
+inline double puissance(double x) {
+  if constexpr (E == 0) {
+    return 1.;
+  } else if constexpr (E == 1) {
+    return x;
+  } else {
+    constexpr int e1 = E / 2;
+    return puissance<e1>(x) * puissance<E - e1>(x);
+  }
+}
+
+// auxiliary/Geometry/Sphere.ixx:31-44
+inline double volume_unit_sphere(int dim) {
+  const double pi = M_PI;
+  double mult = 1;
+  if (dim % 2 == 0) {
+    for (int k = 0; k <= dim / 2 - 1; k++) mult *= (2 * pi) / (dim - 2 * k);
+  } else {
+    mult = 1. / pi;
+    for (int k = 0; k <= (dim - 1) / 2; k++) mult *= (2 * pi) / (dim - 2 * k);
+  }
+  return mult;
+}
+
+// auxiliary/Geometry/Sphere.ixx:46-57
+template <int D>
+inline double volume_sphere(double r) {
+  return volume_unit_sphere(D) * puissance<D>(r);
+}
+
+// auxiliary/Geometry/BasicGeometricOperations.ixx:102-109 (distanceCarre)
+template <int D>
+inline double dist2(const Pt<D>& a, const Pt<D>& b) {
+  double s = 0.;
+  for (int d = 0; d < D; d++) {
+    const double t = a[d] - b[d];
+    s += t * t;
+  }
+  return s;
+}
+
+// operator_cells_and_data_storage.ixx:5-14 (is_there_collision)
+template <int D>
+inline bool collide(double r1, const Pt<D>& p1, double r2, const Pt<D>& p2) {
+  double s = 0;
+  for (int d = 0; d < D; d++) {
+    const double dr = p1[d] - p2[d];
+    s += dr * dr;
+  }
+  return s < (r1 + r2) * (r1 + r2);
+}
+
+// auxiliary/Path.ixx:85-116 (TabCorner).  For D>=4 the reference loop writes
+// tab[i][D] out of bounds and never sets tab[i][0]; the resulting table is
+// (0, bit1(i), ..., bit_{D-1}(i)).  compat=false gives the intended table.
+template <int D>
+inline std::vector<std::array<int, D>> tab_corner(bool compat = true) {
+  std::vector<std::array<int, D>> t(ipow(2, D));
+  if constexpr (D == 2) {
+    t = {{{0, 0}}, {{1, 1}}, {{0, 1}}, {{1, 0}}};
+  } else if constexpr (D == 3) {
+    t = {{{0, 0, 0}}, {{1, 1, 1}}, {{0, 0, 1}}, {{1, 1, 0}},
+         {{0, 1, 0}}, {{1, 0, 1}}, {{0, 1, 1}}, {{1, 0, 0}}};
+  } else {
+    for (int i = 0; i < ipow(2, D); i++)
+      for (int d = 0; d < D; d++) t[i][d] = (compat && d == 0) ? 0 : ((i >> d) & 1);
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------- storage
+enum : uint8_t { TAG_VALIDATED = 0, TAG_UNDECIDED = 1, TAG_CONFLICT = 2, TAG_SKIP = 3 };
+
+// rsa_data_storage<D> (rsa_data_storage.hxx:29-139): SoA of priority, phase, radius, centre.
+template <int D>
+struct Store {
+  std::vector<uint64_t> prio, phase;
+  std::vector<double> rad;
+  std::vector<Pt<D>> pos;
+  size_t size() const { return rad.size(); }
+  void add(uint64_t pr, uint64_t ph, double r, const Pt<D>& p) {
+    prio.push_back(pr);
+    phase.push_back(ph);
+    rad.push_back(r);
+    pos.push_back(p);
+  }
+  void clear() {
+    prio.clear();
+    phase.clear();
+    rad.clear();
+    pos.clear();
+  }
+};
+
+// rsa_data_storage_with_tags<D> (rsa_data_storage.hxx:155-181, .ixx:108-116, 305-322)
+template <int D>
+struct Cell : Store<D> {
+  std::vector<uint8_t> tag;
+  void add(uint64_t pr, uint64_t ph, double r, const Pt<D>& p, uint8_t t) {
+    Store<D>::add(pr, ph, r, p);
+    tag.push_back(t);
+  }
+  // swap-with-last removal (rsa_data_storage.ixx:108-116 + 318-322)
+  void remove_at(size_t i) {
+    const size_t last = this->size() - 1;
+    this->prio[i] = this->prio[last];
+    this->phase[i] = this->phase[last];
+    this->rad[i] = this->rad[last];
+    this->pos[i] = this->pos[last];
+    tag[i] = tag[last];
+    this->prio.pop_back();
+    this->phase.pop_back();
+    this->rad.pop_back();
+    this->pos.pop_back();
+    tag.pop_back();
+  }
+  // remove_tagged_spheres iterates from the back (rsa_data_storage.ixx:305-315)
+  template <class Pred>
+  void remove_tagged(Pred pred) {
+    for (int64_t i = (int64_t)this->size() - 1; i >= 0; i--)
+      if (pred(tag[i])) remove_at((size_t)i);
+  }
+};
+
+// ---------------------------------------------------------------- grid
+// rsa_grid_traversal<D> + rsa_grid<D> (rsa_grid_traversal.ixx, rsa_grid.ixx)
+template <int D>
+struct Grid {
+  int gl = 0;
+  std::array<int, D> n{};
+  Pt<D> shift{}, cl{}, inv{};
+  std::vector<int> nshift;  // 3^D neighbour offsets in the reference's sorted order
+  std::vector<int> edge_trav, ghost_trav;
+  std::vector<Cell<D>> cells;
+  uint64_t nb_fast = 0;
+
+  int size() const {
+    int r = 1;
+    for (int d = 0; d < D; d++) r *= n[d];
+    return r;
+  }
+  std::array<int, D> coords(int idx) const {  // rsa_grid_traversal.ixx:354-371
+    std::array<int, D> c{};
+    for (int d = D - 1; d >= 0; d--) {
+      int sz = 1;
+      for (int k = 0; k < d; k++) sz *= n[k];
+      c[d] = idx / sz;
+      idx -= c[d] * sz;
+    }
+    return c;
+  }
+  bool is_ghost_c(const std::array<int, D>& c) const {  // :261-270
+    for (int d = 0; d < D; d++) {
+      if (c[d] < 0 || c[d] >= n[d]) return false;
+    }
+    for (int d = 0; d < D; d++) {
+      if (c[d] < gl) return true;
+      if (c[d] > n[d] - 1 - gl) return true;
+    }
+    return false;
+  }
+  bool is_ghost(int idx) const { return is_ghost_c(coords(idx)); }
+  bool is_edge_c(const std::array<int, D>& c) const {  // :286-292 (also true on some ghost cells)
+    for (int d = 0; d < D; d++) {
+      if (c[d] == gl) return true;
+      if (c[d] == n[d] - 1 - gl) return true;
+    }
+    return false;
+  }
+  bool is_real(int idx) const {  // :294-301
+    auto c = coords(idx);
+    for (int d = 0; d < D; d++) {
+      if (c[d] < gl) return false;
+      if (c[d] > n[d] - 1 - gl) return false;
+    }
+    return true;
+  }
+
+  // rsa_grid_traversal.ixx:30-66: stencil entry i has offsets (digit_d(i) - 1)_d in
+  // base 3; raw[i] is its cell-index offset; the returned permutation is the
+  // order the reference visits them in (libstdc++ std::sort by squared norm,
+  // not stable, the same call as :58-62).
+  static std::vector<int> stencil_order(const std::array<int, D>& nn, std::vector<int>& raw) {
+    const int n3 = ipow(3, D);
+    std::vector<int> order(n3);
+    raw.assign(n3, 0);
+    std::vector<std::array<double, D>> vec(n3);
+    for (int i = 0; i < n3; i++) {
+      order[i] = i;
+      int tmp = i, val = 0;
+      for (int d = 0; d < D; d++) {
+        const int r = tmp % 3;
+        int s = r - 1;
+        for (int k = d - 1; k >= 0; k--) s *= nn[k];
+        val += s;
+        vec[i][d] = r - 1;
+        tmp = (tmp - r) / 3;
+      }
+      raw[i] = val;
+    }
+    auto norm2 = [](const std::array<double, D>& v) {
+      double s = 0.;
+      for (int d = 0; d < D; d++) s += v[d] * v[d];
+      return s;
+    };
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return norm2(vec[a]) < norm2(vec[b]); });
+    return order;
+  }
+
+  // rsa_grid_traversal.ixx:11-73 (define)
+  void define(double rad, int ghost_layer, const Pt<D>& inf, const Pt<D>& sup) {
+    gl = ghost_layer;
+    for (int d = 0; d < D; d++) {
+      const double length = rad * 2;
+      const double width = sup[d] - inf[d];
+      const int nin = static_cast<int>(width / length);
+      if (nin == 0) throw std::runtime_error("Too small domain wrt ghost layer");
+      cl[d] = width / nin;
+      inv[d] = 1. / cl[d];
+      shift[d] = inf[d] - gl * cl[d];
+      n[d] = nin + 2 * gl;
+    }
+    std::vector<int> raw;
+    const std::vector<int> order = stencil_order(n, raw);
+    nshift.resize(order.size());
+    for (size_t i = 0; i < order.size(); i++) nshift[i] = raw[order[i]];
+    const int nc = size();
+    edge_trav.clear();
+    ghost_trav.clear();
+    for (int idx = 0; idx < nc; idx++) {
+      auto c = coords(idx);
+      if (is_ghost_c(c)) ghost_trav.push_back(idx);
+      if (is_edge_c(c)) edge_trav.push_back(idx);
+    }
+    cells.assign(nc, Cell<D>());
+    nb_fast = 0;
+  }
+
+  // rsa_grid_traversal.ixx:373-382 (compute_cell_idx)
+  uint64_t cell_idx(const Pt<D>& p) const {
+    uint64_t ret = 0;
+    for (int d = D - 1; d >= 0; d--) {
+      const int64_t ijk = static_cast<int64_t>((p[d] - shift[d]) * inv[d]);
+      ret = ret * static_cast<uint64_t>(n[d]) + static_cast<uint64_t>(ijk);
+    }
+    return ret;
+  }
+  int checked_cell(const Pt<D>& p) const {
+    const uint64_t c = cell_idx(p);
+    if (c >= (uint64_t)size()) throw std::runtime_error("oracle: centre outside the cell grid");
+    return (int)c;
+  }
+};
+
+// ---------------------------------------------------------------- domain
+// rsa_domain<D> single rank (rsa_domain.ixx:6-175) with its 3^D-1 ghost areas
+// (rsa_ghost_area.ixx, operator_ghost.ixx).
+template <int D>
+struct Domain {
+  struct Area {
+    Pt<D> lo{}, hi{}, shift{};
+    Store<D> data;
+  };
+  Pt<D> ginf{}, gsup{}, inf{}, sup{};
+  double rad = 0.05;
+  int gl = 0;
+  std::vector<Area> areas;
+  Grid<D> grid;
+
+  Domain(const Pt<D>& global_inf, const Pt<D>& global_sup, int ghost_layer, double a_rad)
+      : ginf(global_inf), gsup(global_sup), rad(a_rad), gl(ghost_layer) {
+    Pt<D> dsize{};
+    for (int d = 0; d < D; d++) {  // rsa_domain.ixx:51-56 with one rank: coord 0, ndims 1
+      dsize[d] = gsup[d] - ginf[d];
+      const double size = dsize[d] / 1;
+      inf[d] = ginf[d] + 0 * size;
+      sup[d] = ginf[d] + (0 + 1) * size;
+    }
+    // relative positions in getAllIndices order (x fastest, Loops.ixx:44-74), origin skipped
+    const int n3 = ipow(3, D);
+    for (int i = 0; i < n3; i++) {
+      std::array<int, D> rel{};
+      int tmp = i;
+      for (int d = 0; d < D; d++) {
+        rel[d] = tmp % 3 - 1;
+        tmp /= 3;
+      }
+      bool zero = true;
+      for (int d = 0; d < D; d++)
+        if (rel[d] != 0) zero = false;
+      if (zero) continue;
+      Area a;
+      for (int d = 0; d < D; d++) {  // rsa_domain.ixx:117-133 and the periodic fill lambda :61-84
+        if (rel[d] == -1) {
+          a.lo[d] = inf[d];
+          a.hi[d] = inf[d] + 2 * a_rad;
+          a.shift[d] = dsize[d];
+        } else if (rel[d] == 0) {
+          a.lo[d] = inf[d];
+          a.hi[d] = sup[d];
+          a.shift[d] = 0;
+        } else {
+          a.lo[d] = sup[d] - 2 * a_rad;
+          a.hi[d] = sup[d];
+          a.shift[d] = -dsize[d];
+        }
+      }
+      areas.push_back(a);
+    }
+    grid.define(a_rad, ghost_layer, inf, sup);
+    for (int d = 1; d < D; d++) {  // rsa_domain.ixx:136-143 (starts at dim 1)
+      if (sup[d] - inf[d] < 2 * a_rad) throw std::runtime_error("The domain is not sufficiently large!");
+    }
+  }
+
+  double total_volume() const {  // rsa_domain.ixx:166-169
+    double v = 1.;
+    for (int d = 0; d < D; d++) v *= (gsup[d] - ginf[d]);
+    return v;
+  }
+
+  // helper_check_interval (helper.ixx:26-34): closed box test
+  static bool in_box(const Pt<D>& lo, const Pt<D>& hi, const Pt<D>& p) {
+    for (int d = 0; d < D; d++) {
+      if (p[d] < lo[d]) return false;
+      if (p[d] > hi[d]) return false;
+    }
+    return true;
+  }
+
+  // fill_ghost<D, Undecided> (operator_ghost.ixx:5-27): edge cells in index order
+  void fill_ghost_undecided() {
+    for (int cid : grid.edge_trav) {
+      const Cell<D>& c = grid.cells[cid];
+      for (size_t s = 0; s < c.size(); s++) {
+        if (c.tag[s] != TAG_UNDECIDED) continue;
+        for (Area& a : areas)
+          if (in_box(a.lo, a.hi, c.pos[s])) a.data.add(c.prio[s], c.phase[s], c.rad[s], c.pos[s]);
+      }
+    }
+  }
+
+  // put_ghost_into (operator_ghost.ixx:115-121): with one rank every area is sent
+  // to and received from itself; the per-(source, tag) FIFO order of MPI makes
+  // receive buffer g hold area g.  Images are shifted (rsa_ghost_area.ixx:22-30)
+  // and appended in area order to the cells their centres fall in.
+  void put_ghost_into(uint8_t tag) {
+    for (Area& a : areas) {
+      Store<D>& s = a.data;
+      for (int d = 0; d < D; d++) {
+        const double sh = a.shift[d];
+        if (std::abs(sh) < 1E-16) continue;
+        for (size_t i = 0; i < s.size(); i++) s.pos[i][d] += sh;
+      }
+    }
+    for (Area& a : areas) {
+      Store<D>& s = a.data;
+      for (size_t i = 0; i < s.size(); i++)
+        grid.cells[grid.checked_cell(s.pos[i])].add(s.prio[i], s.phase[i], s.rad[i], s.pos[i], tag);
+      s.clear();
+    }
+  }
+
+  // rsa_grid::extract_spheres (rsa_grid.ixx:73-85): real cells in index order, in-cell order
+  void extract(std::vector<Pt<D>>& pos, std::vector<double>& r, std::vector<uint64_t>& ph) const {
+    const int nc = grid.size();
+    for (int idx = 0; idx < nc; idx++) {
+      if (grid.is_ghost(idx)) continue;
+      const Cell<D>& c = grid.cells[idx];
+      for (size_t s = 0; s < c.size(); s++) {
+        pos.push_back(c.pos[s]);
+        r.push_back(c.rad[s]);
+        ph.push_back(c.phase[s]);
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------- radius law
+// sac_de_billes::RadiusGenerator<D> (radius_generator.hxx:89-149, .ixx:9-86)
+template <int D>
+struct RadiusGen {
+  std::vector<std::tuple<double, uint64_t, int>> classes;  // (radius, count, phase)
+  double exclusion = 0, max_radius = 0, min_radius = 0;
+  int current = 0;
+  // custom law (RandomRadiusGenerator, radius_generator.hxx:12-38): radius = transform(u)
+  std::function<double(double)> transform;
+  uint64_t custom_phase = 0;
+  bool custom = false;
+
+  RadiusGen() = default;
+  // (radius, count, phase) constructor, radius_generator.ixx:9-28
+  static RadiusGen from_counts(std::vector<std::tuple<double, uint64_t, int>> c, double excl = 0) {
+    RadiusGen g;
+    g.classes = c;
+    g.exclusion = excl;
+    if (g.classes.empty()) throw std::runtime_error("Empty radius generator!");
+    std::sort(g.classes.begin(), g.classes.end(),
+              [](auto& a, auto& b) { return std::get<0>(a) > std::get<0>(b); });
+    for (size_t i = 0; i < g.classes.size(); i++) std::get<0>(g.classes[0]) += 0.5 * excl;
+    g.max_radius = std::get<0>(g.classes[0]);
+    g.min_radius = std::get<0>(g.classes[g.classes.size() - 1]);
+    return g;
+  }
+  // (radius, volume fraction, phase) constructor, radius_generator.ixx:30-36 + 71-86
+  static RadiusGen from_fractions(const std::vector<std::tuple<double, double, int>>& c, double volume,
+                                  double excl = 0) {
+    std::vector<std::tuple<double, uint64_t, int>> r(c.size());
+    for (size_t i = 0; i < c.size(); i++) {
+      const double radius = std::get<0>(c[i]);
+      const double phi = std::get<1>(c[i]);
+      const uint64_t nb = static_cast<uint64_t>(volume * phi / volume_sphere<D>(radius));
+      r[i] = std::make_tuple(radius, nb, std::get<2>(c[i]));
+    }
+    return from_counts(r, excl);
+  }
+  // custom random law, radius_generator.ixx:38-48
+  static RadiusGen from_law(double rmin, double rmax, std::function<double(double)> f, uint64_t phase,
+                            uint64_t nb, double excl = 0) {
+    RadiusGen g;
+    g.classes = {std::make_tuple(0., nb, 0)};
+    g.exclusion = excl;
+    g.max_radius = rmax;
+    g.min_radius = rmin;
+    g.transform = f;
+    g.custom_phase = phase;
+    g.custom = true;
+    return g;
+  }
+  bool still() const { return uint64_t(current) < classes.size(); }
+  uint64_t current_number() const { return std::get<1>(classes[current]); }
+  // radius_generator.ixx:50-60
+  template <class URNG>
+  void draw(size_t n, URNG& g, std::vector<uint64_t>& ph, std::vector<double>& r) const {
+    if (!custom) {
+      ph.assign(n, (uint64_t)std::get<2>(classes[current]));
+      r.assign(n, std::get<0>(classes[current]));
+    } else {
+      std::uniform_real_distribution<> u(0., 1.);
+      ph.assign(n, custom_phase);
+      r.assign(n, 0.);
+      for (size_t i = 0; i < n; i++) r[i] = transform(u(g));
+    }
+  }
+  void update_placed(size_t nb) {  // radius_generator.ixx:62-68
+    std::get<1>(classes[current]) = current_number() - nb;
+    if (current_number() == 0) current++;
+  }
+};
+
+// ---------------------------------------------------------------- voxels
+// voxel_list::list_of_voxels<D> (list_of_voxels.hxx, .ixx)
+template <int D>
+struct Voxels {
+  std::vector<std::array<int64_t, D>> c;
+  Pt<D> len{}, origin{};
+  std::vector<std::array<int, D>> tab;
+  std::vector<Pt<D>> corners;
+
+  void set_len(const Pt<D>& l) {  // list_of_voxels.ixx:213-216 + 309-318
+    len = l;
+    corners.assign(tab.size(), Pt<D>{});
+    for (size_t i = 0; i < tab.size(); i++)
+      for (int d = 0; d < D; d++) corners[i][d] = tab[i][d] * len[d];
+  }
+  // list_of_voxels.ixx:11-28 + 192-211 + Loops.ixx:44-74
+  Voxels(const Pt<D>& o, const Pt<D>& width, double max_diag, bool compat_tab = true) : origin(o) {
+    tab = tab_corner<D>(compat_tab);
+    std::array<int, D> nv{};
+    Pt<D> l{};
+    for (int d = 0; d < D; d++) nv[d] = 1 + static_cast<int>(std::sqrt(D) * width[d] / max_diag);
+    for (int d = 0; d < D; d++) l[d] = width[d] / nv[d];
+    set_len(l);
+    size_t total = 1;
+    for (int d = 0; d < D; d++) total *= (size_t)nv[d];
+    c.resize(total);
+    std::array<int64_t, D> idx{};
+    for (size_t k = 0; k < total; k++) {  // x fastest
+      c[k] = idx;
+      for (int d = 0; d < D; d++) {
+        if (++idx[d] < nv[d]) break;
+        idx[d] = 0;
+      }
+    }
+  }
+  size_t size() const { return c.size(); }
+  Pt<D> orig_voxel(size_t i) const {  // :146-153
+    Pt<D> r = origin;
+    for (int d = 0; d < D; d++) r[d] += c[i][d] * len[d];
+    return r;
+  }
+  Pt<D> center(size_t i) const {  // :155-162
+    Pt<D> r = origin;
+    for (int d = 0; d < D; d++) r[d] += (c[i][d] + 0.5) * len[d];
+    return r;
+  }
+
+  // auxi::is_in_sphere (list_of_voxels.ixx:225-238)
+  static bool in_sphere(const std::vector<Pt<D>>& corners, const Pt<D>& delta, double r, double rmin) {
+    const double R2 = (r + rmin) * (r + rmin);
+    for (size_t i = 0; i < corners.size(); i++)
+      if (dist2<D>(corners[i], delta) > R2) return false;
+    return true;
+  }
+  // auxi::is_covered, stencil form (list_of_voxels.ixx:240-259)
+  static bool covered(const Pt<D>& ov, const Pt<D>& l, const std::vector<Pt<D>>& corners, const Grid<D>& g,
+                      double rmin) {
+    Pt<D> ctr{};
+    for (int d = 0; d < D; d++) ctr[d] = ov[d] + 0.5 * l[d];
+    const int64_t cell = (int64_t)g.cell_idx(ctr);
+    for (size_t nc = 0; nc < g.nshift.size(); nc++) {
+      const Cell<D>& s = g.cells[cell + g.nshift[nc]];
+      for (size_t k = 0; k < s.size(); k++) {
+        Pt<D> delta{};
+        for (int d = 0; d < D; d++) delta[d] = s.pos[k][d] - ov[d];
+        if (in_sphere(corners, delta, s.rad[k], rmin)) return true;
+      }
+    }
+    return false;
+  }
+
+  // remove_covered (list_of_voxels.ixx:30-54): flags, then swap-with-last compaction
+  std::vector<char> cover_flags(const Grid<D>& g, double rmin) const {
+    const int64_t n = (int64_t)size();
+    std::vector<char> f(n, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t i = 0; i < n; i++) f[i] = covered(orig_voxel(i), len, corners, g, rmin) ? 1 : 0;
+    return f;
+  }
+  void compact(std::vector<char>& f) {
+    int64_t old = (int64_t)size();
+    for (int64_t i = 0; i < old; i++) {
+      if (f[i]) {
+        c[i] = c[old - 1];
+        f[i] = f[old - 1];
+        i--;
+        old--;
+      }
+    }
+    c.resize(old);
+  }
+  void remove_covered(const Grid<D>& g, double rmin) {
+    auto f = cover_flags(g, rmin);
+    compact(f);
+  }
+
+  // subdivide_uncovered (list_of_voxels.ixx:56-100) with get_list_of_spheres
+  // (:283-307) and the list form of is_covered (:261-281).  Children of each
+  // parent are tested independently; the stable append order is kept.
+  void subdivide(const Grid<D>& g, double rmin) {
+    if (size() == 0) return;
+    Pt<D> nl{};
+    for (int d = 0; d < D; d++) nl[d] = 0.5 * len[d];
+    std::vector<Pt<D>> ncorners(tab.size());
+    for (size_t i = 0; i < tab.size(); i++)
+      for (int d = 0; d < D; d++) ncorners[i][d] = tab[i][d] * nl[d];
+    const int nch = (int)tab.size();
+    const int64_t np = (int64_t)size();
+    std::vector<uint32_t> keep(np, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < np; i++) {
+      const Pt<D> cv = center(i);
+      const int64_t cell = (int64_t)g.cell_idx(cv);
+      std::vector<std::pair<int64_t, size_t>> list;
+      for (size_t nc = 0; nc < g.nshift.size(); nc++) {
+        const int64_t cid = cell + g.nshift[nc];
+        const Cell<D>& s = g.cells[cid];
+        for (size_t k = 0; k < s.size(); k++) {
+          const double rr = s.rad[k] + rmin;
+          if (dist2<D>(s.pos[k], cv) < rr * rr) list.emplace_back(cid, k);
+        }
+      }
+      uint32_t m = 0;
+      for (int ic = 0; ic < nch; ic++) {
+        Pt<D> ov{};
+        for (int d = 0; d < D; d++) {
+          const int64_t nc2 = 2 * c[i][d] + tab[ic][d];
+          ov[d] = origin[d] + nc2 * nl[d];
+        }
+        bool cov = false;
+        for (auto& e : list) {
+          const Cell<D>& s = g.cells[e.first];
+          Pt<D> delta{};
+          for (int d = 0; d < D; d++) delta[d] = s.pos[e.second][d] - ov[d];
+          if (in_sphere(ncorners, delta, s.rad[e.second], rmin)) {
+            cov = true;
+            break;
+          }
+        }
+        if (!cov) m |= (1u << ic);
+      }
+      keep[i] = m;
+    }
+    std::vector<std::array<int64_t, D>> out;
+    size_t total = 0;
+    for (int64_t i = 0; i < np; i++) total += __builtin_popcount(keep[i]);
+    out.reserve(total);
+    for (int64_t i = 0; i < np; i++)
+      for (int ic = 0; ic < nch; ic++)
+        if (keep[i] & (1u << ic)) {
+          std::array<int64_t, D> cc{};
+          for (int d = 0; d < D; d++) cc[d] = 2 * c[i][d] + tab[ic][d];
+          out.push_back(cc);
+        }
+    set_len(nl);
+    c.swap(out);
+  }
+
+  // pick_points (list_of_voxels.ixx:107-129)
+  template <class URNG>
+  std::vector<Pt<D>> pick(int64_t n, URNG& g) const {
+    if (size() == 0 && n != 0) throw std::runtime_error("Impossible to draw centers in no voxels");
+    std::vector<Pt<D>> ret(n);
+    std::uniform_real_distribution<> coord(0, 1);
+    std::uniform_int_distribution<> vox(0, (int)(size() - 1));
+    for (int64_t i = 0; i < n; i++) {
+      const int64_t id = vox(g);
+      const Pt<D> o = orig_voxel(id);
+      for (int d = 0; d < D; d++) ret[i][d] = o[d] + len[d] * coord(g);
+    }
+    return ret;
+  }
+};
+
+// ---------------------------------------------------------------- pivot
+// median_of_medians::find_pivot_for_first_elements on one rank
+// (median_of_medians.ixx:8-178), order = std::less<uint64_t>.
+inline double mom_find_pivot_pair(const std::vector<uint64_t>& v, double pct, double& second) {
+  double first = 0., secnd = 0.;
+  if (v.empty()) {
+    first = std::nan("0");
+    secnd = std::nan("0");
+  } else {
+    int64_t piv = static_cast<int64_t>(v.size() * pct) - 1;
+    int64_t mn = piv, mx = piv + 1;
+    if (std::abs(v.size() * pct - 1 - piv) < 1e-8) mn -= 1;
+    if (mn < 0) mn = 0;
+    if (std::abs(v.size() * pct - 1 - piv - 1) < 1e-8) mx += 1;
+    if (mx > int64_t(v.size()) - 1) mx = v.size() - 1;
+    first = v[mn] - std::numeric_limits<double>::epsilon();
+    secnd = v[mx] + std::numeric_limits<double>::epsilon();
+  }
+  second = secnd;
+  return first;
+}
+inline int64_t mom_count_le(const std::vector<uint64_t>& sorted, double pivot) {
+  if (sorted.empty()) return 0;
+  // std::upper_bound with std::less<uint64_t>: the double pivot converts to uint64_t
+  auto it = std::upper_bound(sorted.begin(), sorted.end(), pivot, std::less<uint64_t>{});
+  return (int)(it - sorted.begin());
+}
+inline double mom_pivot(const std::vector<uint64_t>& values, int64_t nb_first) {
+  const int64_t gsize = (int64_t)values.size();
+  if (nb_first > gsize)
+    throw std::runtime_error(
+        "Impossible to find the pivot for N first elements if N is smaller than the size of the array");
+  if (nb_first <= 0) throw std::runtime_error("Impossible to find the pivot for N first elements if N  <= 0");
+  if (gsize == 0) throw std::runtime_error("Impossible to find the pivot for N first elements if no element");
+  const double pct = nb_first / (1. * gsize);
+  std::vector<uint64_t> sorted = values;
+  std::sort(sorted.begin(), sorted.end(), std::less<uint64_t>{});
+  double c1 = 0;
+  const double c0 = mom_find_pivot_pair(sorted, pct, c1);
+  std::vector<double> cand = {c0, c1};
+  std::sort(cand.begin(), cand.end(), std::less<uint64_t>{});
+  cand.erase(std::remove_if(cand.begin(), cand.end(), [](double e) { return std::isnan(e); }), cand.end());
+  auto cmp = [&](int64_t nb, double p) { return nb <= mom_count_le(sorted, p); };
+  auto it = std::upper_bound(cand.begin(), cand.end(), nb_first, cmp);
+  if (it == cand.end()) throw std::runtime_error("Unexpected - 0");
+  const double pmax = *it;
+  double pivot = pmax;
+  int64_t below = mom_count_le(sorted, pivot);
+  if (below != nb_first) {
+    if (it == cand.begin()) throw std::runtime_error("Unexpected - 1");
+    const double pmin = *(it - 1);
+    const double step = pmax - pmin;
+    double add = 0.5 * step;
+    for (int j = 0; j < 20; j++) {
+      pivot = pmin + add;
+      below = mom_count_le(sorted, pivot);
+      if (below == nb_first) break;
+      if (below > nb_first) add = 0.5 * std::abs(add);
+      if (below < nb_first) add = -0.5 * std::abs(add);
+    }
+  }
+  return pivot;
+}
+
+// ---------------------------------------------------------------- trace
+struct DrawTrace {
+  int64_t draw = 0;        // loop index of the draw (zero-shot iterations included)
+  int64_t nvox_before = 0; // voxel count used for the Poisson intensity
+  int64_t shots = 0;       // Poisson shot count
+  int64_t accepted = 0;    // spheres validated by this draw
+  int64_t rounds = 0;      // add_to_sample calls
+  double miss_rate = 0;
+  int refined = 0;         // remove_covered + subdivide_uncovered ran
+  int64_t nvox_after = 0;
+  uint64_t words_before = 0, words_after_poisson = 0, words_after_draw = 0;
+};
+
+// Candidates of one draw (shot order) and which of them the draw validated;
+// recorded for the first draws when requested, for brute-force checks.
+template <int D>
+struct DrawDump {
+  std::vector<Pt<D>> pos;
+  std::vector<uint64_t> prio;
+  std::vector<double> rad;
+  std::vector<uint8_t> accepted;
+  int outreach = 0;
+};
+
+// ---------------------------------------------------------------- algorithm
+// algorithm::rsa_algo<D> (operator_algorithm.hxx:18-109, operator_algorithm.ixx)
+template <int D>
+struct Algo {
+  Domain<D>& dom;
+  RadiusGen<D> gen;
+  int nbshots;
+  double r_max;
+  double aimed;
+  bool compat_tab = true;
+  std::vector<DrawTrace> trace;
+  uint64_t words_used = 0;
+  int64_t max_draws = -1;  // test hook: stop after this many draws (-1: run to the end)
+  std::string end_message;
+  int dump_limit = 0;  // test hook: record the candidates of the first draws
+  std::vector<DrawDump<D>> dumps;
+
+  static double default_miss_rate() {  // operator_algorithm.ixx:344-351
+    if (D <= 3) return 0.95;
+    return 1 - 0.05 * puissance<D - 3>(0.1);
+  }
+
+  // ctor: operator_algorithm.ixx:15-30 (the grid is rebuilt with r_max)
+  Algo(Domain<D>& d, const RadiusGen<D>& g, int nbshots_singledraw)
+      : dom(d), gen(g), nbshots(nbshots_singledraw), r_max(g.max_radius), aimed(default_miss_rate()) {
+    dom.grid.define(g.max_radius, dom.gl, dom.inf, dom.sup);
+    if (g.max_radius > dom.rad)
+      throw std::runtime_error(
+          "Impossible to have the maximal radius of radius generator larger than domain implicit radius");
+  }
+
+  // reset_radius_generator (operator_algorithm.ixx:238-250)
+  void reset_radius_generator(const RadiusGen<D>& g) {
+    if (gen.max_radius < g.max_radius) throw std::runtime_error("Incorrect maximal radius");
+    r_max = std::max(r_max, g.max_radius);
+    gen = g;
+  }
+
+  // operator_cells_and_data_storage.ixx:16-52 with ONLY_VALIDATED, no priority, no skip
+  bool hits_validated(int cell, double r, const Pt<D>& p) const {
+    const Grid<D>& g = dom.grid;
+    for (size_t nc = 0; nc < g.nshift.size(); nc++) {
+      const Cell<D>& s = g.cells[cell + g.nshift[nc]];
+      for (size_t k = 0; k < s.size(); k++) {
+        if (s.tag[k] != TAG_VALIDATED) continue;
+        if (collide<D>(r, p, s.rad[k], s.pos[k])) return true;
+      }
+    }
+    return false;
+  }
+  // detection<D, PRIORITY, SKIP, !ONLY_VALIDATED>: first hit in stencil then in-cell order
+  bool first_partner(int cell, uint64_t id, double r, const Pt<D>& p, int& hc, int& hs) const {
+    const Grid<D>& g = dom.grid;
+    for (size_t nc = 0; nc < g.nshift.size(); nc++) {
+      const int cid = cell + g.nshift[nc];
+      const Cell<D>& s = g.cells[cid];
+      for (size_t k = 0; k < s.size(); k++) {
+        if (s.tag[k] == TAG_SKIP) continue;
+        if (s.prio[k] < id && collide<D>(r, p, s.rad[k], s.pos[k])) {
+          hc = cid;
+          hs = (int)k;
+          return true;
+        }
+      }
+    }
+    return false;
+  }
+
+  // remove_collisions_with_validated_spheres (operator_cells_and_data_storage.ixx:54-76)
+  void remove_collisions_with_validated(const std::vector<uint64_t>& cells) {
+    Grid<D>& g = dom.grid;
+    const int64_t nc = (int64_t)cells.size();
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < nc; i++) {
+      const int cid = (int)cells[i];
+      if (!g.is_real(cid)) continue;
+      Cell<D>& c = g.cells[cid];
+      for (int64_t s = (int64_t)c.size() - 1; s >= 0; s--) {
+        if (c.tag[s] == TAG_UNDECIDED && hits_validated(cid, c.rad[s], c.pos[s])) c.tag[s] = TAG_SKIP;
+      }
+    }
+    remove_tagged_real(cells, [](uint8_t t) { return t == TAG_SKIP; });
+  }
+  template <class Pred>
+  void remove_tagged_real(const std::vector<uint64_t>& cells, Pred pred) {
+    for (uint64_t cid : cells)
+      if (dom.grid.is_real((int)cid)) dom.grid.cells[cid].remove_tagged(pred);
+  }
+
+  // recompute_conflict_cells (operator_algorithm.ixx:353-371)
+  void recompute_conflict_cells(std::vector<uint64_t>& cells) const {
+    int cur = (int)cells.size();
+    for (int64_t i = (int64_t)cells.size() - 1; i >= 0; i--) {
+      bool conflict = false;
+      const Cell<D>& c = dom.grid.cells[cells[i]];
+      for (size_t s = 0; s < c.size(); s++)
+        if (c.tag[s] != TAG_VALIDATED) {
+          conflict = true;
+          break;
+        }
+      if (!conflict) {
+        cells[i] = cells[cur - 1];
+        cur--;
+      }
+    }
+    cells.resize(cur);
+  }
+
+  // find_solve_conflicts (operator_cells_and_data_storage.ixx:79-143)
+  void find_solve_conflicts(const std::vector<uint64_t>& cells) {
+    Grid<D>& g = dom.grid;
+    struct Ref {
+      uint64_t prio;
+      int cell, s;
+    };
+    std::vector<Ref> ind;
+    for (uint64_t cid : cells) {
+      const Cell<D>& c = g.cells[cid];
+      for (size_t s = 0; s < c.size(); s++)
+        if (c.tag[s] == TAG_UNDECIDED) ind.push_back({c.prio[s], (int)cid, (int)s});
+    }
+    std::sort(ind.begin(), ind.end(), [](const Ref& a, const Ref& b) {
+      if ((int)a.prio != (int)b.prio) return (int)a.prio < (int)b.prio;
+      if (a.cell != b.cell) return a.cell < b.cell;
+      return a.s < b.s;
+    });
+    for (const Ref& r : ind) {
+      if (g.is_ghost(r.cell)) continue;
+      Cell<D>& c = g.cells[r.cell];
+      int hc = -1, hs = -1;
+      if (first_partner(r.cell, c.prio[r.s], c.rad[r.s], c.pos[r.s], hc, hs)) {
+        if (g.is_ghost(hc) || g.cells[hc].tag[hs] == TAG_CONFLICT)
+          c.tag[r.s] = TAG_CONFLICT;
+        else
+          c.tag[r.s] = TAG_SKIP;
+      }
+    }
+  }
+
+  int64_t how_many_real(const std::vector<uint64_t>& cells, bool not_validated) const {
+    int64_t n = 0;
+    for (uint64_t cid : cells) {
+      if (!dom.grid.is_real((int)cid)) continue;
+      const Cell<D>& c = dom.grid.cells[cid];
+      for (size_t s = 0; s < c.size(); s++) {
+        if (not_validated ? (c.tag[s] != TAG_VALIDATED) : (c.tag[s] == TAG_UNDECIDED)) n++;
+      }
+    }
+    return n;
+  }
+
+  // update_sphere_tags (operator_algorithm.ixx:446-473)
+  void update_sphere_tags(const std::vector<uint64_t>& cells, uint64_t nb_undecided) {
+    Grid<D>& g = dom.grid;
+    dom.fill_ghost_undecided();
+    dom.put_ghost_into(TAG_VALIDATED);
+    for (int cid : g.ghost_trav) g.cells[cid].remove_tagged([](uint8_t t) { return t != TAG_VALIDATED; });
+    remove_tagged_real(cells, [](uint8_t t) { return t == TAG_SKIP; });
+    g.nb_fast = g.nb_fast + nb_undecided;
+    for (uint64_t cid : cells) {
+      if (!g.is_real((int)cid)) continue;
+      Cell<D>& c = g.cells[cid];
+      for (size_t s = 0; s < c.size(); s++) {
+        if (c.tag[s] == TAG_UNDECIDED)
+          c.tag[s] = TAG_VALIDATED;
+        else if (c.tag[s] == TAG_CONFLICT)
+          c.tag[s] = TAG_UNDECIDED;
+      }
+    }
+  }
+
+  // retain_only_first_elements (operator_algorithm.ixx:475-519)
+  int64_t retain_only_first(const std::vector<uint64_t>& cells, uint64_t maximal) {
+    Grid<D>& g = dom.grid;
+    std::vector<uint64_t> prio;
+    std::vector<std::pair<int, int>> ids;
+    for (uint64_t cid : cells) {
+      if (!g.is_real((int)cid)) continue;
+      const Cell<D>& c = g.cells[cid];
+      for (size_t s = 0; s < c.size(); s++)
+        if (c.tag[s] == TAG_UNDECIDED) {
+          prio.push_back(c.prio[s]);
+          ids.emplace_back((int)cid, (int)s);
+        }
+    }
+    const double pivot = mom_pivot(prio, (int64_t)maximal);
+    int64_t nb = 0;
+    for (size_t i = 0; i < ids.size(); i++) {
+      if (prio[i] > pivot)
+        g.cells[ids[i].first].tag[ids[i].second] = TAG_SKIP;
+      else
+        nb++;
+    }
+    for (uint64_t cid : cells) {
+      if (!g.is_real((int)cid)) continue;
+      Cell<D>& c = g.cells[cid];
+      for (size_t s = 0; s < c.size(); s++)
+        if (c.tag[s] != TAG_UNDECIDED && c.tag[s] != TAG_VALIDATED) c.tag[s] = TAG_SKIP;
+    }
+    return nb;
+  }
+
+  // add_to_sample (operator_algorithm.ixx:383-443).  a_maximal_nb_spheres is a
+  // by-value parameter there, so every round compares with the draw-start value.
+  bool add_to_sample(std::vector<uint64_t>& cells, uint64_t maximal, int64_t& local_new, bool may_outreach) {
+    remove_collisions_with_validated(cells);
+    recompute_conflict_cells(cells);
+    dom.fill_ghost_undecided();
+    dom.put_ghost_into(TAG_UNDECIDED);
+    find_solve_conflicts(cells);
+    const uint64_t nb_undecided = (uint64_t)how_many_real(cells, false);
+    bool outreached = false;
+    uint64_t total_undecided = 0;
+    if (may_outreach) {
+      total_undecided = nb_undecided;
+      outreached = (total_undecided > maximal);
+    }
+    if (outreached) {
+      const int64_t fin = retain_only_first(cells, maximal);
+      update_sphere_tags(cells, (uint64_t)fin);
+      local_new += fin;
+      return false;
+    }
+    update_sphere_tags(cells, nb_undecided);
+    local_new += nb_undecided;
+    maximal -= total_undecided;
+    if (maximal == 0) {
+      for (uint64_t cid : cells) dom.grid.cells[cid].remove_tagged([](uint8_t t) { return t != TAG_VALIDATED; });
+      return false;
+    }
+    if (how_many_real(cells, true) != 0) {
+      recompute_conflict_cells(cells);
+      return true;
+    }
+    return false;
+  }
+
+  // generate_spheres + generate_candidates (operator_algorithm.ixx:534-582)
+  template <class URNG>
+  int64_t single_draw(const Voxels<D>& vox, URNG& rng, int nb_shots, uint64_t maximal, bool may_outreach,
+                      int64_t& rounds) {
+    const std::vector<Pt<D>> pos = vox.pick(nb_shots, rng);
+    std::vector<uint64_t> prio(nb_shots);
+    std::uniform_int_distribution<> pl(0, std::numeric_limits<int>::max());  // generate_priority<int> :373-381
+    for (int i = 0; i < nb_shots; i++) prio[i] = pl(rng);
+    std::vector<uint64_t> ph;
+    std::vector<double> rad;
+    gen.draw((size_t)nb_shots, rng, ph, rad);
+    Grid<D>& g = dom.grid;
+    std::vector<uint64_t> cells;
+    for (int i = 0; i < nb_shots; i++) {  // build_grid (operator_cells_and_data_storage.ixx:153-159)
+      const int cid = g.checked_cell(pos[i]);
+      g.cells[cid].add(prio[i], ph[i], rad[i], pos[i], TAG_UNDECIDED);
+    }
+    for (int i = 0; i < nb_shots; i++) cells.push_back((uint64_t)(int)g.cell_idx(pos[i]));  // :331-342
+    std::sort(cells.begin(), cells.end());
+    cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+    int64_t local_new = 0;
+    rounds = 0;
+    while (true) {
+      rounds++;
+      if (!add_to_sample(cells, maximal, local_new, may_outreach)) break;
+    }
+    if ((int)dumps.size() < dump_limit) {
+      DrawDump<D> dd;
+      dd.pos = pos;
+      dd.prio = prio;
+      dd.rad = rad;
+      dd.outreach = may_outreach ? 1 : 0;
+      dd.accepted.assign(nb_shots, 0);
+      for (int i = 0; i < nb_shots; i++) {
+        const uint64_t c = g.cell_idx(pos[i]);
+        if (c >= (uint64_t)g.size()) continue;
+        const Cell<D>& cl = g.cells[c];
+        for (size_t k = 0; k < cl.size(); k++)
+          if (cl.tag[k] == TAG_VALIDATED && cl.pos[k] == pos[i]) dd.accepted[i] = 1;
+      }
+      dumps.push_back(dd);
+    }
+    return local_new;
+  }
+
+  // proceed<1> -> proceed_voxel (operator_algorithm.ixx:85-188, 190-236)
+  void proceed(uint64_t seed) {
+    CountingMT rng;
+    seed_engine(rng, seed);
+    Pt<D> width{};
+    for (int d = 0; d < D; d++) width[d] = dom.sup[d] - dom.inf[d];
+    Voxels<D> vox(dom.inf, width, r_max + gen.min_radius, compat_tab);
+    uint64_t old_nb = 0;
+    for (int64_t draw = 0; true; draw++) {
+      if (max_draws >= 0 && (int64_t)trace.size() >= max_draws) {
+        end_message = "stopped";
+        break;
+      }
+      const int nv = (int)vox.size();
+      const double intensity = 0.01 * nv;  // compute_intensity_poisson :221-223
+      const double max_intensity = intensity;
+      const uint64_t total_vox = (uint64_t)(int64_t)vox.size();
+      if (total_vox == 0) {
+        end_message = "[log rsa_mpi] End of algorithm : fully packed";
+        break;
+      }
+      DrawTrace tr;
+      tr.draw = draw;
+      tr.nvox_before = nv;
+      tr.words_before = rng.used;
+      // compute_nb_shots_voxel :208-218
+      double mult = nbshots / max_intensity;
+      if (mult > 1) mult = 1.;
+      std::poisson_distribution<> pd(mult * intensity);
+      const int nb_shots_i = pd(rng);
+      const uint64_t nb_shots = (uint64_t)nb_shots_i;
+      tr.words_after_poisson = rng.used;
+      const uint64_t total_shots = nb_shots;
+      if (total_shots == 0) continue;
+      const bool may_outreach = (total_shots >= gen.current_number());
+      const uint64_t maximal = gen.current_number();
+      int64_t rounds = 0;
+      const uint64_t nb_new = (uint64_t)single_draw(vox, rng, (int)nb_shots, maximal, may_outreach, rounds);
+      tr.words_after_draw = rng.used;
+      gen.update_placed(nb_new);
+      const bool cont = gen.still();
+      const int new_nb = (int)dom.grid.nb_fast;
+      const uint64_t nb_miss = nb_shots - (uint64_t)(new_nb - old_nb);
+      old_nb = (uint64_t)new_nb;
+      const double miss_rate = nb_miss / (0.000001 + total_shots);
+      tr.shots = (int64_t)nb_shots;
+      tr.accepted = (int64_t)nb_new;
+      tr.rounds = rounds;
+      tr.miss_rate = miss_rate;
+      if (miss_rate > aimed) {  // update_covered_voxels :190-206
+        vox.remove_covered(dom.grid, gen.min_radius);
+        vox.subdivide(dom.grid, gen.min_radius);
+        tr.refined = 1;
+      }
+      tr.nvox_after = (int64_t)vox.size();
+      trace.push_back(tr);
+      if (!cont) {
+        end_message = "End of generation, due to lack of new radii";
+        break;
+      }
+    }
+    words_used = rng.used;
+  }
+};
+
+}  // namespace rsa_oracle

@@ -1,0 +1,224 @@
+"""MI355X-native single-rank RSA (random sequential adsorption) with the
+operator API of MarcJos/RSA_MPI.
+
+This module mirrors the reference's Python module ``rsa_mpi_py``
+(Interface_python/Interface.cxx:40-120): per dimension a ``sphere``,
+``rsa_domain``, ``radius_generator``, ``rsa_algo`` class and a ``throw_spheres``
+function (``rsa_domain_3D``, ``rsa_algo_3D``, ...).  All work runs in
+librsa_mpi_amd.so (hand-written gfx950 HIP kernels behind a C-ABI); this file
+only marshals arguments.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import Stats, check, dptr, f64, lib, mt_words  # noqa: F401
+
+__all__ = ["Domain", "RadiusGenerator", "Algo", "Stats", "mt_words", "uniform_generate", "throw_spheres",
+           "set_nb_threads"]
+
+
+def set_nb_threads(n):
+    """rsa_mpi_py.set_nb_threads (Interface.cxx:24-35): the GPU path has no OpenMP pool."""
+    return None
+
+
+class Sphere:
+    """Sphere<DIM> (auxiliary/Geometry/Sphere.hxx:18-32): center, radius, phase."""
+
+    def __init__(self, center=None, radius=0.0, phase=0):
+        self.center = list(center) if center is not None else []
+        self.radius = float(radius)
+        self.phase = int(phase)
+
+    def __repr__(self):
+        return f"Sphere(center={self.center}, radius={self.radius}, phase={self.phase})"
+
+
+class Domain:
+    """rsa_domain<DIM> on one GPU (rsa_domain.hxx:53-93): periodic box [inf, sup]."""
+
+    def __init__(self, inf, sup, rad, ghost_layer=1):
+        self.inf = [float(v) for v in inf]
+        self.sup = [float(v) for v in sup]
+        self.dim = len(self.inf)
+        if len(self.sup) != self.dim:
+            raise ValueError("Incoherent dimensions!")
+        self.rad = float(rad)
+        self._ctx = ctypes.c_void_p()
+        a, b = f64(self.inf), f64(self.sup)
+        check(lib().rsa_gpu_create(self.dim, dptr(a), dptr(b), int(ghost_layer), self.rad, ctypes.byref(self._ctx)))
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx is not None and ctx.value:
+            lib().rsa_gpu_destroy(ctx)
+            self._ctx = None
+
+    def get_total_volume(self):  # rsa_domain.ixx:166-169
+        v = 1.0
+        for a, b in zip(self.inf, self.sup):
+            v *= (b - a)
+        return v
+
+    def domain_log(self):
+        print(" ==== log: domain [0] ==== ")
+        print(" == m_inf : " + " ".join(str(v) for v in self.inf))
+        print(" == m_sup : " + " ".join(str(v) for v in self.sup))
+
+    def number_of_spheres(self):
+        n = ctypes.c_uint64()
+        check(lib().rsa_gpu_count(self._ctx, ctypes.byref(n)))
+        return n.value
+
+    def extract_arrays(self):
+        """(centres [n, dim], radii [n], phases [n]) in rsa_grid::extract_spheres order."""
+        n = ctypes.c_uint64()
+        check(lib().rsa_gpu_extract(self._ctx, None, None, None, 0, ctypes.byref(n)))
+        cnt = n.value
+        pos = np.zeros(cnt * self.dim)
+        rad = np.zeros(cnt)
+        ph = np.zeros(cnt, dtype=np.uint64)
+        check(lib().rsa_gpu_extract(self._ctx, dptr(pos), dptr(rad), ph.ctypes.data_as(_lib._U64), cnt,
+                                    ctypes.byref(n)))
+        return pos.reshape(cnt, self.dim), rad, ph
+
+    def extract_spheres(self):
+        pos, rad, ph = self.extract_arrays()
+        return [Sphere(p.tolist(), r, int(f)) for p, r, f in zip(pos, rad, ph)]
+
+    def compute_total_volume_of_spheres(self):
+        v = ctypes.c_double()
+        check(lib().rsa_gpu_volume_of_spheres(self._ctx, ctypes.byref(v)))
+        return v.value
+
+    def add(self, centers, radii, phases=None):
+        """rsa_domain::add (rsa_domain.hxx:84): insert validated spheres."""
+        c = f64(centers).reshape(-1)
+        r = f64(radii)
+        ph = np.ascontiguousarray(phases if phases is not None else np.zeros(len(r)), dtype=np.uint64)
+        check(lib().rsa_gpu_add_spheres(self._ctx, len(r), dptr(c), dptr(r), ph.ctypes.data_as(_lib._U64)))
+
+    def voxels(self):
+        """Current voxel list (int64 lattice coordinates) and the common voxel length."""
+        n = ctypes.c_uint64()
+        ln = np.zeros(self.dim)
+        check(lib().rsa_gpu_voxels(self._ctx, None, dptr(ln), 0, ctypes.byref(n)))
+        co = np.zeros(n.value * self.dim, dtype=np.int64)
+        check(lib().rsa_gpu_voxels(self._ctx, co.ctypes.data_as(_lib._I64), dptr(ln), n.value, ctypes.byref(n)))
+        return co.reshape(-1, self.dim), ln
+
+
+class RadiusGenerator:
+    """sac_de_billes::RadiusGenerator<DIM> (radius_generator.hxx:89-149).
+
+    ``RadiusGenerator([(r, N, phase), ...], exclusion)`` or
+    ``RadiusGenerator([(r, phi, phase), ...], volume, exclusion)``.
+    """
+
+    def __init__(self, classes, volume_or_exclusion=0.0, exclusion=None):
+        if len(classes) == 0:
+            raise RuntimeError("Empty radius generator!")
+        self.radius = [float(c[0]) for c in classes]
+        self.phase = [int(c[2]) for c in classes]
+        if exclusion is None:  # (radius, count, phase) form
+            self.count = [int(c[1]) for c in classes]
+            self.fraction = None
+            self.volume = None
+            self.exclusion = float(volume_or_exclusion)
+        else:
+            self.fraction = [float(c[1]) for c in classes]
+            self.volume = float(volume_or_exclusion)
+            self.exclusion = float(exclusion)
+            self.count = None
+
+    def counts(self, dim):
+        """compute_desired_radius_nb_phase (radius_generator.ixx:71-86)."""
+        if self.count is not None:
+            return list(self.count)
+        import math
+        mult = 1.0
+        if dim % 2 == 0:
+            for k in range(dim // 2):
+                mult *= (2 * math.pi) / (dim - 2 * k)
+        else:
+            mult = 1.0 / math.pi
+            for k in range((dim - 1) // 2 + 1):
+                mult *= (2 * math.pi) / (dim - 2 * k)
+        out = []
+        for r, phi in zip(self.radius, self.fraction):
+            p = {2: r * r, 3: r * (r * r), 4: (r * r) * (r * r)}[dim]
+            out.append(int(self.volume * phi / (mult * p)))
+        return out
+
+    def _arrays(self, dim):
+        return (f64(self.radius), np.ascontiguousarray(self.counts(dim), dtype=np.uint64),
+                np.ascontiguousarray(self.phase, dtype=np.int32))
+
+
+class Algo:
+    """algorithm::rsa_algo<DIM> (operator_algorithm.hxx:18-109), voxel method."""
+
+    def __init__(self, domain, radius_generator, nbshots_singledraw=6000, n_draw=10):
+        self.domain = domain
+        r, c, p = radius_generator._arrays(domain.dim)
+        check(lib().rsa_gpu_algo_init(domain._ctx, len(r), dptr(r), c.ctypes.data_as(_lib._U64),
+                                      p.ctypes.data_as(_lib._I), radius_generator.exclusion, int(nbshots_singledraw)))
+        self.last_stats = None
+
+    def proceed(self, seed, max_draws=-1):
+        st = Stats()
+        check(lib().rsa_gpu_proceed(self.domain._ctx, int(seed), int(max_draws), ctypes.byref(st)))
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def reset_radius_generator(self, radius_generator):
+        r, c, p = radius_generator._arrays(self.domain.dim)
+        check(lib().rsa_gpu_reset_classes(self.domain._ctx, len(r), dptr(r), c.ctypes.data_as(_lib._U64),
+                                          p.ctypes.data_as(_lib._I), radius_generator.exclusion))
+
+    def set_miss_rate(self, m):
+        check(lib().rsa_gpu_set_miss_rate(self.domain._ctx, float(m)))
+
+
+def uniform_generate(domain, rad_or_generator, size=6000, n_draw=1, seed=0):
+    """algorithm::uniform_generate<DIM, 1> (operator_algorithm.ixx:275-292)."""
+    if isinstance(rad_or_generator, RadiusGenerator):
+        gen = rad_or_generator
+    else:
+        gen = RadiusGenerator([(float(rad_or_generator), 1.0, 0)], domain.get_total_volume(), 0.0)
+    algo = Algo(domain, gen, size, n_draw)
+    algo.proceed(seed)
+    return algo
+
+
+def throw_spheres(L, desired_radius_volumeFraction_phase, seed, write_paraview=False):
+    """user_interface::throw_spheres (test.hxx:13-32): box [0, L]; the reference
+    passes volume 1 to the radius generator (kept)."""
+    dim = len(L)
+    gen = RadiusGenerator(desired_radius_volumeFraction_phase, 1.0, 0.0)
+    domain = Domain([0.0] * dim, [float(v) for v in L], max(gen.radius))
+    uniform_generate(domain, gen, 6000, 10, seed)
+    return domain.extract_spheres()
+
+
+def _make_dim_aliases():
+    g = globals()
+    for d in (2, 3, 4):
+        def mk_domain(d=d):
+            class _Dom(Domain):
+                def __init__(self, global_inf, global_sup, rad):
+                    if len(global_inf) != d:
+                        raise ValueError("Incoherent dimensions!")
+                    super().__init__(global_inf, global_sup, rad)
+            _Dom.__name__ = f"rsa_domain_{d}D"
+            return _Dom
+        g[f"rsa_domain_{d}D"] = mk_domain()
+        g[f"radius_generator_{d}D"] = RadiusGenerator
+        g[f"rsa_algo_{d}D"] = Algo
+        g[f"sphere{d}D"] = Sphere
+        g[f"throw_spheres_{d}D"] = throw_spheres
+
+
+_make_dim_aliases()

@@ -1,0 +1,90 @@
+"""ctypes binding of librsa_mpi_amd.so (the C-ABI declared in include/rsa_mpi_amd.h).
+
+The library is built in-tree by ``rsa_mpi_amd/build.sh``; loading fails loudly when
+it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "librsa_mpi_amd.so")
+
+# every symbol include/rsa_mpi_amd.h declares
+EXPORTED = (
+    "rsa_gpu_last_error", "rsa_gpu_device_info", "rsa_gpu_create", "rsa_gpu_destroy",
+    "rsa_gpu_algo_init", "rsa_gpu_algo_init_fractions", "rsa_gpu_reset_classes",
+    "rsa_gpu_set_miss_rate", "rsa_gpu_proceed", "rsa_gpu_add_spheres", "rsa_gpu_count",
+    "rsa_gpu_extract", "rsa_gpu_volume_of_spheres", "rsa_gpu_voxels", "rsa_gpu_mt_words",
+)
+
+
+class Stats(ctypes.Structure):
+    """rsa_gpu_stats"""
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "loop_iterations", "draws", "shots", "accepted", "refines", "rounds", "rounds_max",
+        "nvox_max", "nvox_final", "words", "ghost_cell_drops", "image_cell_mismatch")] + [
+        ("end_reason", ctypes.c_int), ("seconds", ctypes.c_double), ("voxel_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_U64 = ctypes.POINTER(ctypes.c_uint64)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_I = ctypes.POINTER(ctypes.c_int)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+
+
+def lib():
+    """Load (once) and return the library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run rsa_mpi_amd/build.sh (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.rsa_gpu_last_error.restype = ctypes.c_char_p
+    L.rsa_gpu_device_info.restype = ctypes.c_char_p
+    L.rsa_gpu_create.argtypes = [ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_P)]
+    L.rsa_gpu_destroy.argtypes = [_P]
+    L.rsa_gpu_destroy.restype = None
+    L.rsa_gpu_algo_init.argtypes = [_P, ctypes.c_int, _D, _U64, _I, ctypes.c_double, ctypes.c_int]
+    L.rsa_gpu_algo_init_fractions.argtypes = [_P, ctypes.c_int, _D, _D, _I, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_int]
+    L.rsa_gpu_reset_classes.argtypes = [_P, ctypes.c_int, _D, _U64, _I, ctypes.c_double]
+    L.rsa_gpu_set_miss_rate.argtypes = [_P, ctypes.c_double]
+    L.rsa_gpu_proceed.argtypes = [_P, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(Stats)]
+    L.rsa_gpu_add_spheres.argtypes = [_P, ctypes.c_int64, _D, _D, _U64]
+    L.rsa_gpu_count.argtypes = [_P, _U64]
+    L.rsa_gpu_extract.argtypes = [_P, _D, _D, _U64, ctypes.c_uint64, _U64]
+    L.rsa_gpu_volume_of_spheres.argtypes = [_P, _D]
+    L.rsa_gpu_voxels.argtypes = [_P, _I64, _D, ctypes.c_uint64, _U64]
+    L.rsa_gpu_mt_words.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _U32]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().rsa_gpu_last_error().decode())
+
+
+def dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def mt_words(seed, n, rank=0):
+    """Raw MT19937 words of seed_seq{rank, seed} from the library's host generator."""
+    out = np.zeros(n, dtype=np.uint32)
+    check(lib().rsa_gpu_mt_words(seed, rank, n, out.ctypes.data_as(_U32)))
+    return out

@@ -1,0 +1,180 @@
+// extern "C" entry points of librsa_mpi_amd.so (declared in include/rsa_mpi_amd.h).
+// No C++ exception crosses this boundary: failures return non-zero and leave
+// the message in a thread-local string.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/rsa_mpi_amd.h"
+#include "engine.hpp"
+#include "mt19937.hpp"
+
+struct rsa_gpu_ctx {
+  std::unique_ptr<rsa_amd::IEngine> eng;
+};
+
+namespace {
+thread_local std::string g_err;
+std::string g_info;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  } catch (...) {
+    g_err = "unknown error";
+    return 1;
+  }
+}
+
+std::vector<rsa_amd::RadiusClass> make_classes(int n, const double* radius, const uint64_t* count, const int* phase) {
+  std::vector<rsa_amd::RadiusClass> c;
+  for (int i = 0; i < n; i++) c.push_back({radius[i], count[i], phase ? phase[i] : 0});
+  return c;
+}
+
+int need_ctx(rsa_gpu_ctx* ctx) {
+  if (!ctx || !ctx->eng) {
+    g_err = "null context";
+    return 1;
+  }
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+const char* rsa_gpu_last_error(void) { return g_err.c_str(); }
+
+const char* rsa_gpu_device_info(void) {
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
+    g_info = "no HIP device";
+  } else {
+    g_info = std::string(p.gcnArchName) + " / " + std::to_string(p.multiProcessorCount) + " CUs / " +
+             std::to_string(p.totalGlobalMem >> 30) + " GiB";
+  }
+  return g_info.c_str();
+}
+
+int rsa_gpu_create(int dim, const double* inf, const double* sup, int ghost_layer, double domain_rad,
+                   rsa_gpu_ctx** out) {
+  return guarded([&] {
+    if (!out) throw rsa_amd::Error("null output pointer");
+    auto* c = new rsa_gpu_ctx();
+    try {
+      c->eng.reset(rsa_amd::make_engine(dim, inf, sup, ghost_layer, domain_rad));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+void rsa_gpu_destroy(rsa_gpu_ctx* ctx) { delete ctx; }
+
+int rsa_gpu_algo_init(rsa_gpu_ctx* ctx, int n, const double* radius, const uint64_t* count, const int* phase,
+                      double exclusion, int nbshots_singledraw) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { ctx->eng->algo_init(make_classes(n, radius, count, phase), exclusion, nbshots_singledraw); });
+}
+
+int rsa_gpu_algo_init_fractions(rsa_gpu_ctx* ctx, int n, const double* radius, const double* fraction,
+                                const int* phase, double volume, double exclusion, int nbshots_singledraw) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] {
+    // compute_desired_radius_nb_phase (radius_generator.ixx:71-86)
+    std::vector<uint64_t> cnt(n);
+    const int dim = ctx->eng->dim();
+    double mult = 1;  // volumeUnitSphere (auxiliary/Geometry/Sphere.ixx:31-44)
+    const double pi = M_PI;
+    if (dim % 2 == 0) {
+      for (int k = 0; k <= dim / 2 - 1; k++) mult *= (2 * pi) / (dim - 2 * k);
+    } else {
+      mult = 1. / pi;
+      for (int k = 0; k <= (dim - 1) / 2; k++) mult *= (2 * pi) / (dim - 2 * k);
+    }
+    for (int i = 0; i < n; i++) {
+      const double r = radius[i];
+      double p = 0;  // puissance<D> (AuxiFunctions.ixx:34-44)
+      if (dim == 2) p = r * r;
+      if (dim == 3) p = r * (r * r);
+      if (dim == 4) p = (r * r) * (r * r);
+      cnt[i] = static_cast<uint64_t>(volume * fraction[i] / (mult * p));
+    }
+    ctx->eng->algo_init(make_classes(n, radius, cnt.data(), phase), exclusion, nbshots_singledraw);
+  });
+}
+
+int rsa_gpu_reset_classes(rsa_gpu_ctx* ctx, int n, const double* radius, const uint64_t* count, const int* phase,
+                          double exclusion) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { ctx->eng->reset_classes(make_classes(n, radius, count, phase), exclusion); });
+}
+
+int rsa_gpu_set_miss_rate(rsa_gpu_ctx* ctx, double miss_rate) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { ctx->eng->set_miss_rate(miss_rate); });
+}
+
+int rsa_gpu_proceed(rsa_gpu_ctx* ctx, uint64_t seed, int64_t max_draws, rsa_gpu_stats* stats) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] {
+    rsa_gpu_stats local;
+    ctx->eng->proceed(seed, max_draws, stats ? stats : &local);
+  });
+}
+
+int rsa_gpu_add_spheres(rsa_gpu_ctx* ctx, int64_t n, const double* pos, const double* radius,
+                        const uint64_t* phase) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] {
+    std::vector<uint64_t> ph;
+    if (!phase) {
+      ph.assign(n, 0);
+      phase = ph.data();
+    }
+    ctx->eng->add_spheres(n, pos, radius, phase);
+  });
+}
+
+int rsa_gpu_count(rsa_gpu_ctx* ctx, uint64_t* n) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { *n = ctx->eng->count(); });
+}
+
+int rsa_gpu_extract(rsa_gpu_ctx* ctx, double* pos, double* radius, uint64_t* phase, uint64_t cap, uint64_t* n) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { *n = ctx->eng->extract(pos, radius, phase, cap); });
+}
+
+int rsa_gpu_volume_of_spheres(rsa_gpu_ctx* ctx, double* volume) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { *volume = ctx->eng->volume_of_spheres(); });
+}
+
+int rsa_gpu_voxels(rsa_gpu_ctx* ctx, int64_t* coords, double* len, uint64_t cap, uint64_t* n) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { *n = ctx->eng->voxels(coords, len, cap); });
+}
+
+int rsa_gpu_mt_words(uint64_t seed, uint64_t rank, uint64_t n, uint32_t* out) {
+  return guarded([&] {
+    rsa_amd::MTStream s;
+    s.reset(seed, rank);
+    s.ensure(n);
+    std::memcpy(out, s.buf.data(), n * sizeof(uint32_t));
+  });
+}
+
+}  // extern "C"

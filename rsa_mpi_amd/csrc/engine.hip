@@ -1,0 +1,1645 @@
+// Single-rank bias-free RSA (MarcJos/RSA_MPI, method 1 = voxel) on one MI355X.
+//
+// Device state: validated spheres in per-cell slots (cell grid of
+// rsa_grid_traversal with its ghost layers kept as index space only: a ghost
+// neighbour is read from the real cell it mirrors, filtered by the ghost-area
+// band and shifted, which is what the reference's periodic self-exchange puts
+// there), the voxel list as int32 SoA lattice coordinates plus one common
+// length, the MT19937 word stream in an HBM ring, and per-draw candidate SoA.
+//
+// One draw (operator_algorithm.ixx:117-173) = decode candidates from the word
+// stream, bin them into cells in shot order, run the reference's conflict
+// rounds exactly (validated filter, priority sweep labels, tag update with the
+// same swap-with-last list edits), commit survivors to their cells in the
+// reference's in-cell order; then, when the miss rate exceeds the aimed rate,
+// one fused voxel pass computes coverage (remove_covered) and the children
+// masks (subdivide_uncovered), followed by the swap-with-last compaction in
+// closed form and a stable children scatter.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <random>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "device_common.hpp"
+#include "engine.hpp"
+#include "mt19937.hpp"
+
+namespace rsa_amd {
+
+#define HIP_OK(expr)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      throw Error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + #expr);           \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void reserve(size_t n) {  // contents are not preserved
+    if (n <= cap && p) return;
+    if (p) HIP_OK(hipFree(p));
+    p = nullptr;
+    size_t c = std::max<size_t>(n, 1);
+    HIP_OK(hipMalloc(&p, c * sizeof(T)));
+    cap = c;
+  }
+  void release() {
+    if (p) HIP_OK(hipFree(p));
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+constexpr int WG = 256;
+static inline unsigned nblocks(int64_t n, int per = WG) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
+
+// Counters written by kernels and read by the host after a draw step.
+enum Ctr : int {
+  C_NTOUCHED = 0,
+  C_CURSOR,       // partner-list allocation cursor
+  C_ACCEPTED,     // Undecided after a sweep (how_many<Real, Undecided>)
+  C_CONFLICT,     // Conflict after a sweep
+  C_REMAIN,       // Not_Validated after the tag update
+  C_OVERFLOW,     // cell capacity exceeded
+  C_CHAIN_OVF,    // word window too short for the shots
+  C_LABEL_STUCK,  // label passes made no progress
+  C_DROPPED,      // centres in ghost cells
+  C_IMG_MISMATCH, // periodic image whose computed cell is not the mirrored ghost cell
+  C_END_REL,      // word offset after the last shot's coordinates
+  C_KEEP,         // uncovered voxels
+  C_NCHILD,       // children kept by subdivide
+  C_NUM
+};
+
+// ============================================================== kernels: word stream
+
+__device__ inline uint32_t ring_at(const uint32_t* ring, uint64_t mask, uint64_t pos) { return ring[pos & mask]; }
+
+// Voxel-index words drawn by uniform_int_distribution<int>(0, N-1) use Lemire's
+// multiply-shift with rejection of low 32 bits below (2^32 - N) mod N
+// (uniform_int_dist.h:246-268, 313-317).  One workgroup builds the rejection
+// bitmap of the word window in LDS; wave 0 then walks the shots: a shot takes
+// the first accepted word at or after its start, then 2*D coordinate words.
+template <int D>
+__global__ void __launch_bounds__(1024) k_chain(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
+                                                int nwin, int nshots, uint32_t N, uint32_t thr,
+                                                uint32_t* __restrict__ vpos, int* __restrict__ ctr) {
+  extern __shared__ uint32_t bits[];  // nwin/32 + 1 words
+  const int nb = (nwin + 31) / 32;
+  for (int w = threadIdx.x; w < nb; w += blockDim.x) {
+    uint32_t b = 0;
+    for (int k = 0; k < 32; k++) {
+      const int pos = w * 32 + k;
+      if (pos < nwin) {
+        const uint64_t prod = (uint64_t)ring_at(ring, mask, w0 + pos) * (uint64_t)N;
+        if ((uint32_t)prod < thr) b |= (1u << k);
+      }
+    }
+    bits[w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  const int stride = 2 * D + 1;
+  auto rej = [&](int p) -> bool { return (bits[p >> 5] >> (p & 31)) & 1u; };
+  int cur = 0, base = 0;
+  bool ovf = false;
+  while (base < nshots) {
+    const int i = base + lane;
+    const int s = cur + lane * stride;
+    const bool valid = i < nshots;
+    const bool out = valid && (s + stride > nwin);
+    const bool r = valid && !out && rej(s);
+    const uint64_t bo = __ballot(out);
+    const uint64_t br = __ballot(r);
+    if (bo) {  // window too short: let the host retry with a larger one
+      ovf = true;
+      break;
+    }
+    if (br == 0) {
+      if (valid) vpos[i] = (uint32_t)s;
+      const int cnt = min(64, nshots - base);
+      cur += cnt * stride;
+      base += cnt;
+    } else {
+      const int f = __ffsll((unsigned long long)br) - 1;
+      if (lane < f) vpos[i] = (uint32_t)s;
+      int p = cur + f * stride;
+      while (p < nwin && rej(p)) p++;
+      if (p + stride > nwin) {
+        ovf = true;
+        break;
+      }
+      if (lane == f) vpos[i] = (uint32_t)p;
+      cur = p + stride;
+      base += f + 1;
+    }
+  }
+  if (lane == 0) {
+    if (ovf)
+      ctr[C_CHAIN_OVF] = 1;
+    else
+      ctr[C_END_REL] = cur;
+  }
+}
+
+template <int D>
+struct VoxView {
+  const int32_t* c[D];
+  double origin[D], len[D];
+};
+
+struct ClassDev {
+  double radius;
+  uint32_t phase;
+};
+
+// pick_points + generate_priority + constant-radius class (list_of_voxels.ixx:107-129,
+// operator_algorithm.ixx:373-381, radius_generator.ixx:50-60), then the cell of
+// each centre (build_grid, rsa_grid.ixx:182-187).
+template <int D>
+__global__ void k_decode(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0, int nshots, uint32_t N,
+                         const uint32_t* __restrict__ vpos, VoxView<D> vx, ClassDev cls,
+                         Geo<D> g, double* __restrict__ cpos, double* __restrict__ crad,
+                         uint32_t* __restrict__ cprio, uint32_t* __restrict__ cphase, int* __restrict__ ccell,
+                         uint8_t* __restrict__ ctag, int* ctr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots) return;
+  const uint32_t v = vpos[i];
+  const uint32_t gw = ring_at(ring, mask, w0 + v);
+  const uint32_t id = (uint32_t)(((uint64_t)gw * (uint64_t)N) >> 32);
+  double p[D];
+  for (int d = 0; d < D; d++) {
+    const double o = vx.origin[d] + (double)vx.c[d][id] * vx.len[d];
+    const double u = canonical(ring_at(ring, mask, w0 + v + 1 + 2 * d), ring_at(ring, mask, w0 + v + 2 + 2 * d));
+    p[d] = o + vx.len[d] * u;
+    cpos[(size_t)d * nshots + i] = p[d];
+  }
+  const int end_rel = ctr[C_END_REL];
+  cprio[i] = ring_at(ring, mask, w0 + (uint64_t)end_rel + i) >> 1;
+  crad[i] = cls.radius;
+  cphase[i] = cls.phase;
+  const int64_t cell = cell_index<D>(g, p);
+  bool ok = cell >= 0 && cell < g.ncells;
+  if (ok) {
+    int cc[D];
+    cell_coords<D>(g, cell, cc);
+    ok = !coords_ghost<D>(g, cc);
+  }
+  if (ok) {
+    ccell[i] = (int)cell;
+    ctag[i] = T_UNDECIDED;
+  } else {
+    ccell[i] = -1;
+    ctag[i] = T_DROPPED;
+    atomicAdd(&ctr[C_DROPPED], 1);
+  }
+}
+
+// ============================================================== kernels: binning in shot order
+
+__global__ void k_bin_count(int nshots, const int* __restrict__ ccell, int* __restrict__ cellcc,
+                            int* __restrict__ carr, int* __restrict__ cell2t, int* __restrict__ touched,
+                            int* __restrict__ ctr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots) return;
+  const int c = ccell[i];
+  if (c < 0) return;
+  const int a = atomicAdd(&cellcc[c], 1);
+  carr[i] = a;
+  if (a == 0) {
+    const int t = atomicAdd(&ctr[C_NTOUCHED], 1);
+    touched[t] = c;
+    cell2t[c] = t;
+  }
+}
+
+// single workgroup: exclusive scan of the touched cells' candidate counts
+__global__ void __launch_bounds__(1024) k_bin_scan(const int* __restrict__ touched, const int* __restrict__ cellcc,
+                                                   int* __restrict__ seg, int* __restrict__ len,
+                                                   const int* __restrict__ ctr) {
+  __shared__ int sh[1024];
+  const int nt = ctr[C_NTOUCHED];
+  int carry = 0;
+  for (int base = 0; base < nt; base += 1024) {
+    const int t = base + threadIdx.x;
+    const int v = t < nt ? cellcc[touched[t]] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (t < nt) {
+      seg[t] = carry + sh[threadIdx.x] - v;
+      len[t] = v;
+    }
+    const int tot = sh[1023];
+    __syncthreads();
+    carry += tot;
+  }
+}
+
+__global__ void k_bin_place(int nshots, const int* __restrict__ ccell, const int* __restrict__ carr,
+                            const int* __restrict__ cell2t, const int* __restrict__ seg, int* __restrict__ ord) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots) return;
+  const int c = ccell[i];
+  if (c < 0) return;
+  ord[seg[cell2t[c]] + carr[i]] = i;
+}
+
+// build_grid appends in candidate order: sort each cell's arrivals by shot index
+__global__ void k_bin_sort(const int* __restrict__ seg, const int* __restrict__ len, int* __restrict__ ord,
+                           const int* __restrict__ ctr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ctr[C_NTOUCHED]) return;
+  int* a = ord + seg[t];
+  const int n = len[t];
+  for (int k = 1; k < n; k++) {
+    const int v = a[k];
+    int j = k - 1;
+    while (j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      j--;
+    }
+    a[j + 1] = v;
+  }
+}
+
+// ============================================================== kernels: conflict rounds
+
+template <int D>
+__device__ inline void cand_pos(const double* cpos, int n, int i, double* p) {
+  for (int d = 0; d < D; d++) p[d] = cpos[(size_t)d * n + i];
+}
+
+// remove_collisions_with_validated_spheres (operator_cells_and_data_storage.ixx:54-76):
+// an Undecided candidate colliding with any Validated sphere of its 3^D stencil
+// (placed spheres in slots, spheres validated earlier in this draw, and the
+// periodic images of both) becomes Skip.
+template <int D>
+__global__ void k_filter(Geo<D> g, CellStore<D> cs, int nshots, const double* __restrict__ cpos,
+                         const double* __restrict__ crad, const int* __restrict__ ccell, uint8_t* __restrict__ ctag,
+                         const int* __restrict__ cell2t, const int* __restrict__ seg, const int* __restrict__ len,
+                         const int* __restrict__ ord) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots) return;
+  if (ctag[i] != T_UNDECIDED) return;
+  double p[D];
+  cand_pos<D>(cpos, nshots, i, p);
+  const double r = crad[i];
+  int cc[D];
+  cell_coords<D>(g, ccell[i], cc);
+  for (int s = 0; s < g.nst; s++) {
+    Nb<D> nb;
+    if (!neighbour<D>(g, cc, s, nb)) continue;
+    const int nv = cs.cnt[nb.src];
+    const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
+    for (int k = 0; k < nv; k++) {
+      const Slot<D> q = sl[k];
+      double qp[D];
+      if (nb.ghost) {
+        if (!image_of<D>(g, nb, q.c, qp)) continue;
+      } else {
+        for (int d = 0; d < D; d++) qp[d] = q.c[d];
+      }
+      if (collide<D>(r, p, q.r, qp)) {
+        ctag[i] = T_SKIP;
+        return;
+      }
+    }
+    const int t = cell2t[nb.src];
+    if (t < 0) continue;
+    const int* o = ord + seg[t];
+    const int ln = len[t];
+    for (int k = 0; k < ln; k++) {
+      const int j = o[k];
+      if (ctag[j] != T_VALIDATED) continue;
+      double jp[D], qp[D];
+      cand_pos<D>(cpos, nshots, j, jp);
+      if (nb.ghost) {
+        if (!image_of<D>(g, nb, jp, qp)) continue;
+      } else {
+        for (int d = 0; d < D; d++) qp[d] = jp[d];
+      }
+      if (collide<D>(r, p, crad[j], qp)) {
+        ctag[i] = T_SKIP;
+        return;
+      }
+    }
+  }
+}
+
+// remove_tagged_spheres on one cell's candidate tail: from the back, swap with
+// last (rsa_data_storage.ixx:108-116, 305-315).  mode 0: Skip, 1: Not_Validated.
+__global__ void k_list_remove(int mode, const int* __restrict__ seg, int* __restrict__ len, int* __restrict__ ord,
+                              const uint8_t* __restrict__ ctag, const int* __restrict__ ctr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ctr[C_NTOUCHED]) return;
+  int* a = ord + seg[t];
+  int n = len[t];
+  for (int k = n - 1; k >= 0; k--) {
+    const uint8_t tg = ctag[a[k]];
+    const bool rm = mode == 0 ? (tg == T_SKIP) : (tg != T_VALIDATED);
+    if (rm) {
+      a[k] = a[n - 1];
+      n--;
+    }
+  }
+  len[t] = n;
+}
+
+// Ordered partner lists for the sweep (find_solve_conflicts,
+// operator_cells_and_data_storage.ixx:127-142 with detection :16-52): for each
+// Undecided candidate, every Undecided candidate with a strictly smaller
+// priority that collides with it, in stencil order then in-cell order; a
+// partner seen through a ghost neighbour is its periodic image (bit 31 set).
+template <int D>
+__device__ inline int scan_partners(const Geo<D>& g, int nshots, int i, const double* cpos, const double* crad,
+                                    const uint32_t* cprio, const int* ccell, const uint8_t* ctag,
+                                    const int* cell2t, const int* seg, const int* len, const int* ord, int* out,
+                                    int* ctr) {
+  double p[D];
+  cand_pos<D>(cpos, nshots, i, p);
+  const double r = crad[i];
+  const uint32_t pi = cprio[i];
+  int cc[D];
+  cell_coords<D>(g, ccell[i], cc);
+  int n = 0;
+  for (int s = 0; s < g.nst; s++) {
+    Nb<D> nb;
+    if (!neighbour<D>(g, cc, s, nb)) continue;
+    const int t = cell2t[nb.src];
+    if (t < 0) continue;
+    const int* o = ord + seg[t];
+    const int ln = len[t];
+    for (int k = 0; k < ln; k++) {
+      const int j = o[k];
+      if (ctag[j] != T_UNDECIDED) continue;
+      if (!(cprio[j] < pi)) continue;
+      double jp[D], qp[D];
+      cand_pos<D>(cpos, nshots, j, jp);
+      if (nb.ghost) {
+        if (!image_of<D>(g, nb, jp, qp)) continue;
+        // the reference files the image under compute_cell_idx(image); it must be this ghost cell
+        if (out == nullptr) {
+          int64_t ic = cell_index<D>(g, qp);
+          int64_t want = 0;
+          for (int d = D - 1; d >= 0; d--) want = want * g.n[d] + (cc[d] + g.st[s][d]);
+          if (ic != want) atomicAdd(&ctr[C_IMG_MISMATCH], 1);
+        }
+      } else {
+        for (int d = 0; d < D; d++) qp[d] = jp[d];
+      }
+      if (collide<D>(r, p, crad[j], qp)) {
+        if (out) out[n] = j | (nb.ghost ? (int)0x80000000 : 0);
+        n++;
+      }
+    }
+  }
+  return n;
+}
+
+template <int D>
+__global__ void k_partners(Geo<D> g, int nshots, const double* __restrict__ cpos, const double* __restrict__ crad,
+                           const uint32_t* __restrict__ cprio, const int* __restrict__ ccell,
+                           const uint8_t* __restrict__ ctag, const int* __restrict__ cell2t,
+                           const int* __restrict__ seg, const int* __restrict__ len, const int* __restrict__ ord,
+                           int* __restrict__ poff, int* __restrict__ pcnt, int* __restrict__ plist, int pcap,
+                           int* __restrict__ ctr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots) return;
+  if (ctag[i] != T_UNDECIDED) {
+    pcnt[i] = 0;
+    return;
+  }
+  const int n = scan_partners<D>(g, nshots, i, cpos, crad, cprio, ccell, ctag, cell2t, seg, len, ord, nullptr, ctr);
+  pcnt[i] = n;
+  if (n == 0) return;
+  const int off = atomicAdd(&ctr[C_CURSOR], n);
+  poff[i] = off;
+  if (off + n > pcap) {
+    atomicMax(&ctr[C_OVERFLOW], 2);
+    return;
+  }
+  scan_partners<D>(g, nshots, i, cpos, crad, cprio, ccell, ctag, cell2t, seg, len, ord, plist + off, ctr);
+}
+
+// The sequential priority sweep as a dependency evaluation: a candidate's
+// label is fixed by its first partner (in detection order) that is not Skip:
+// an image or a Conflict partner gives Conflict, an Undecided real partner
+// gives Skip, none leaves it Undecided.  Partners have strictly smaller
+// priorities, so they are labelled first in the reference's sorted sweep and
+// the passes below reach the same labels.  One workgroup.
+__global__ void __launch_bounds__(1024) k_label(int nshots, uint8_t* __restrict__ ctag, uint8_t* __restrict__ lab,
+                                                const int* __restrict__ poff, const int* __restrict__ pcnt,
+                                                const int* __restrict__ plist, int* __restrict__ ctr) {
+  __shared__ int s_prog, s_left;
+  for (int i = threadIdx.x; i < nshots; i += blockDim.x) lab[i] = ctag[i] == T_UNDECIDED ? T_UNRESOLVED : ctag[i];
+  __syncthreads();
+  for (int pass = 0; pass < nshots + 2; pass++) {
+    if (threadIdx.x == 0) {
+      s_prog = 0;
+      s_left = 0;
+    }
+    __syncthreads();
+    int prog = 0, left = 0;
+    for (int i = threadIdx.x; i < nshots; i += blockDim.x) {
+      if (__atomic_load_n(&lab[i], __ATOMIC_RELAXED) != T_UNRESOLVED) continue;
+      const int n = pcnt[i];
+      const int* pl = plist + (n ? poff[i] : 0);
+      uint8_t res = T_UNDECIDED;
+      bool wait = false;
+      for (int k = 0; k < n; k++) {
+        const int e = pl[k];
+        if (e < 0) {
+          res = T_CONFLICT;
+          break;
+        }
+        const uint8_t lj = __atomic_load_n(&lab[e], __ATOMIC_RELAXED);
+        if (lj == T_UNRESOLVED) {
+          wait = true;
+          break;
+        }
+        if (lj == T_SKIP) continue;
+        res = (lj == T_CONFLICT) ? T_CONFLICT : T_SKIP;
+        break;
+      }
+      if (wait) {
+        left++;
+      } else {
+        __atomic_store_n(&lab[i], res, __ATOMIC_RELAXED);
+        prog++;
+      }
+    }
+    if (prog) atomicAdd(&s_prog, prog);
+    if (left) atomicAdd(&s_left, left);
+    __syncthreads();
+    const int sp = s_prog, sl = s_left;
+    __syncthreads();
+    if (sl == 0) break;
+    if (sp == 0) {
+      if (threadIdx.x == 0) ctr[C_LABEL_STUCK] = 1;
+      break;
+    }
+  }
+  __syncthreads();
+  int acc = 0, con = 0;
+  for (int i = threadIdx.x; i < nshots; i += blockDim.x) {
+    if (ctag[i] != T_UNDECIDED) continue;
+    const uint8_t l = lab[i];
+    ctag[i] = l;
+    acc += l == T_UNDECIDED;
+    con += l == T_CONFLICT;
+  }
+  if (acc) atomicAdd(&ctr[C_ACCEPTED], acc);
+  if (con) atomicAdd(&ctr[C_CONFLICT], con);
+}
+
+// retain_only_first_elements (operator_algorithm.ixx:475-519) after the host
+// found the pivot: Undecided above it and every other non-validated candidate
+// become Skip.
+__global__ void k_retain(int nshots, double pivot, const uint32_t* __restrict__ cprio, uint8_t* __restrict__ ctag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots) return;
+  const uint8_t t = ctag[i];
+  if (t == T_DROPPED || t == T_SKIP || t == T_VALIDATED) return;
+  if (t == T_UNDECIDED) {
+    if ((double)cprio[i] > pivot) ctag[i] = T_SKIP;
+  } else {
+    ctag[i] = T_SKIP;
+  }
+}
+
+__global__ void k_gather_undecided(int nshots, const uint32_t* __restrict__ cprio, const uint8_t* __restrict__ ctag,
+                                   uint32_t* __restrict__ out, int* __restrict__ ctr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots) return;
+  if (ctag[i] != T_UNDECIDED) return;
+  out[atomicAdd(&ctr[C_CURSOR], 1)] = cprio[i];
+}
+
+// update_sphere_tags on real cells (operator_algorithm.ixx:446-473): Skip
+// removed from the tail (swap with last, from the back), Undecided ->
+// Validated, Conflict -> Undecided.
+__global__ void k_update(const int* __restrict__ seg, int* __restrict__ len, int* __restrict__ ord,
+                         uint8_t* __restrict__ ctag, int* __restrict__ ctr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ctr[C_NTOUCHED]) return;
+  int* a = ord + seg[t];
+  int n = len[t];
+  for (int k = n - 1; k >= 0; k--) {
+    if (ctag[a[k]] == T_SKIP) {
+      a[k] = a[n - 1];
+      n--;
+    }
+  }
+  len[t] = n;
+  int rem = 0;
+  for (int k = 0; k < n; k++) {
+    const int j = a[k];
+    if (ctag[j] == T_UNDECIDED)
+      ctag[j] = T_VALIDATED;
+    else if (ctag[j] == T_CONFLICT) {
+      ctag[j] = T_UNDECIDED;
+      rem++;
+    }
+  }
+  if (rem) atomicAdd(&ctr[C_REMAIN], rem);
+}
+
+// End of draw: the tail of each touched cell holds the spheres validated by
+// this draw in the reference's in-cell order; append them after the cell's
+// earlier spheres and clear the per-draw cell maps.
+template <int D>
+__global__ void k_commit(CellStore<D> cs, int nshots, const double* __restrict__ cpos, const double* __restrict__ crad,
+                         const uint32_t* __restrict__ cphase, const uint8_t* __restrict__ ctag,
+                         const int* __restrict__ touched, const int* __restrict__ seg, const int* __restrict__ len,
+                         const int* __restrict__ ord, int* __restrict__ cell2t, int* __restrict__ cellcc,
+                         int* __restrict__ ctr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ctr[C_NTOUCHED]) return;
+  const int c = touched[t];
+  const int* a = ord + seg[t];
+  const int n = len[t];
+  int k0 = cs.cnt[c];
+  for (int k = 0; k < n; k++) {
+    const int j = a[k];
+    if (ctag[j] != T_VALIDATED) continue;
+    if (k0 >= cs.cap) {
+      atomicMax(&ctr[C_OVERFLOW], 1);
+      break;
+    }
+    Slot<D> s;
+    for (int d = 0; d < D; d++) s.c[d] = cpos[(size_t)d * nshots + j];
+    s.r = crad[j];
+    cs.sph[(size_t)c * cs.cap + k0] = s;
+    cs.phase[(size_t)c * cs.cap + k0] = cphase[j];
+    k0++;
+  }
+  cs.cnt[c] = (uint8_t)k0;
+  cell2t[c] = -1;
+  cellcc[c] = 0;
+}
+
+// ============================================================== kernels: voxel refine
+
+template <int D>
+struct VoxParams {
+  static constexpr int NCH = ipow_c(2, D);
+  double origin[D], len[D], nl[D];
+  double corners[NCH][D];   // parent corners: tab[i][d] * len[d]
+  double ncorners[NCH][D];  // child corners: tab[i][d] * nl[d]
+  signed char tab[NCH][D];
+  double rmin;
+};
+
+// auxi::is_in_sphere (list_of_voxels.ixx:225-238)
+template <int D>
+__device__ inline bool in_sphere(const double (*corners)[D], const double* delta, double r, double rmin) {
+  constexpr int NCH = ipow_c(2, D);
+  const double R2 = (r + rmin) * (r + rmin);
+#pragma unroll
+  for (int i = 0; i < NCH; i++) {
+    double s = 0.;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const double t = corners[i][d] - delta[d];
+      s += t * t;
+    }
+    if (s > R2) return false;
+  }
+  return true;
+}
+
+// remove_covered's coverage flag (list_of_voxels.ixx:30-40, 240-259) and
+// subdivide_uncovered's children test (:56-100, 261-307) in one pass:
+// out[i] = 0 if voxel i is covered, else 0x10000 | (bit k set when child k in
+// TabCorner order stays uncovered).  blk[b] counts uncovered voxels per block.
+template <int D>
+__global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView<D> vx, int64_t nvox,
+                                              VoxParams<D> vp, uint32_t* __restrict__ out, int* __restrict__ blk) {
+  constexpr int NCH = ipow_c(2, D);
+  constexpr uint32_t FULL = (NCH == 32) ? 0xffffffffu : ((1u << NCH) - 1u);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t code = 0;
+  if (i < nvox) {
+    int64_t c[D];
+    double ov[D], ca[D], cb[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      c[d] = vx.c[d][i];
+      ov[d] = vp.origin[d] + (double)c[d] * vp.len[d];  // orig_voxel (list_of_voxels.ixx:146-153)
+      ca[d] = ov[d] + 0.5 * vp.len[d];                  // is_covered's centre (:245)
+      cb[d] = vp.origin[d] + ((double)c[d] + 0.5) * vp.len[d];  // center() used by subdivide (:155-162)
+    }
+    const int64_t cella = cell_index<D>(g, ca);
+    const int64_t cellb = cell_index<D>(g, cb);
+    double ovk[NCH][D];  // children origins (list_of_voxels.ixx:82-86)
+#pragma unroll
+    for (int k = 0; k < NCH; k++)
+#pragma unroll
+      for (int d = 0; d < D; d++) ovk[k][d] = vp.origin[d] + (double)(2 * c[d] + vp.tab[k][d]) * vp.nl[d];
+    uint32_t ccov = 0;
+    bool covered = false;
+    // pass 0: coverage around cella (and children when cellb == cella); pass 1: children around cellb
+    for (int pass = 0; pass < 2 && !covered; pass++) {
+      const int64_t cell = pass == 0 ? cella : cellb;
+      const bool do_cov = pass == 0;
+      const bool do_child = (pass == 0) ? (cellb == cella) : (cellb != cella);
+      if (!do_cov && !do_child) break;
+      int cc[D];
+      cell_coords<D>(g, cell, cc);
+      for (int s = 0; s < g.nst && !covered; s++) {
+        Nb<D> nb;
+        if (!neighbour<D>(g, cc, s, nb)) continue;
+        const int nv = cs.cnt[nb.src];
+        const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
+        for (int k = 0; k < nv; k++) {
+          const Slot<D> q = sl[k];
+          double qp[D];
+          if (nb.ghost) {
+            if (!image_of<D>(g, nb, q.c, qp)) continue;
+          } else {
+#pragma unroll
+            for (int d = 0; d < D; d++) qp[d] = q.c[d];
+          }
+          if (do_cov) {
+            double delta[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) delta[d] = qp[d] - ov[d];
+            if (in_sphere<D>(vp.corners, delta, q.r, vp.rmin)) {
+              covered = true;
+              break;
+            }
+          }
+          if (do_child && ccov != FULL) {
+            double s2 = 0.;  // get_list_of_spheres pre-filter (:297-298)
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+              const double t = qp[d] - cb[d];
+              s2 += t * t;
+            }
+            const double rr = q.r + vp.rmin;
+            if (s2 < rr * rr) {
+#pragma unroll
+              for (int ch = 0; ch < NCH; ch++) {
+                if (ccov & (1u << ch)) continue;
+                double delta[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) delta[d] = qp[d] - ovk[ch][d];
+                if (in_sphere<D>(vp.ncorners, delta, q.r, vp.rmin)) ccov |= (1u << ch);
+              }
+            }
+          }
+        }
+      }
+    }
+    code = covered ? 0u : (0x10000u | (~ccov & FULL & 0xffffu));
+    out[i] = code;
+  }
+  const int u = __syncthreads_count(code != 0);
+  if (threadIdx.x == 0) blk[blockIdx.x] = u;
+}
+
+// single workgroup exclusive scan over block counts; writes the total to *total
+__global__ void __launch_bounds__(1024) k_scan_blocks(int* __restrict__ blk, int nb, int* __restrict__ total) {
+  __shared__ int sh[1024];
+  int carry = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    const int t = base + threadIdx.x;
+    const int v = t < nb ? blk[t] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (t < nb) blk[t] = carry + sh[threadIdx.x] - v;
+    const int tot = sh[1023];
+    __syncthreads();
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// exclusive rank of element i among flagged elements: block offset + in-block prefix
+__device__ inline int block_rank(bool f, int blkoff) {
+  __shared__ int wsum[WG / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t b = __ballot(f);
+  const int inwave = __popcll(b & ((1ull << lane) - 1ull));
+  if (lane == 0) wsum[w] = __popcll(b);
+  __syncthreads();
+  int before = 0;
+  for (int k = 0; k < w; k++) before += wsum[k];
+  __syncthreads();
+  return blkoff + before + inwave;
+}
+
+// swap-with-last compaction of remove_covered (list_of_voxels.ixx:42-53) in
+// closed form: with keep uncovered voxels, the covered positions below keep,
+// in increasing order, receive the uncovered voxels at or above keep in
+// decreasing order.  Step 1 lists the holes.
+__global__ void __launch_bounds__(WG) k_holes(int64_t nvox, const uint32_t* __restrict__ code,
+                                              const int* __restrict__ blk, const int* __restrict__ ctr,
+                                              int* __restrict__ holes) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < nvox;
+  const bool unc = valid && code[i] != 0;
+  const int U = block_rank(unc, blk[blockIdx.x]);
+  const int keep = ctr[C_KEEP];
+  if (valid && !unc && i < keep) holes[i - U] = (int)i;
+}
+
+template <int D>
+__global__ void __launch_bounds__(WG) k_move(int64_t nvox, uint32_t* __restrict__ code, const int* __restrict__ blk,
+                                             const int* __restrict__ ctr, const int* __restrict__ holes,
+                                             VoxView<D> vx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < nvox;
+  const bool unc = valid && code[i] != 0;
+  const int U = block_rank(unc, blk[blockIdx.x]);
+  const int keep = ctr[C_KEEP];
+  if (valid && unc && i >= keep) {
+    const int dst = holes[keep - U - 1];
+    for (int d = 0; d < D; d++) const_cast<int32_t*>(vx.c[d])[dst] = vx.c[d][i];
+    code[dst] = code[i];
+  }
+}
+
+__global__ void __launch_bounds__(WG) k_child_count(int64_t n, const uint32_t* __restrict__ code, int* __restrict__ blk) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = i < n ? __popc(code[i] & 0xffffu) : 0;
+  __shared__ int sh[WG];
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int off = WG / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blk[blockIdx.x] = sh[0];
+}
+
+// stable append of the kept children, parents in list order, children in
+// TabCorner order (list_of_voxels.ixx:76-99): coordinates 2c + tab[k]
+template <int D>
+__global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __restrict__ code,
+                                                const int* __restrict__ blk, VoxView<D> src, int32_t* dst0,
+                                                int32_t* dst1, int32_t* dst2, int32_t* dst3, VoxParams<D> vp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t m = i < n ? (code[i] & 0xffffu) : 0u;
+  const int c = __popc(m);
+  __shared__ int sh[WG];
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int off = 1; off < WG; off <<= 1) {
+    const int add = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  if (i >= n || m == 0) return;
+  int o = blk[blockIdx.x] + sh[threadIdx.x] - c;
+  int32_t* dst[4] = {dst0, dst1, dst2, dst3};
+  int32_t pc[D];
+  for (int d = 0; d < D; d++) pc[d] = src.c[d][i];
+  constexpr int NCH = ipow_c(2, D);
+#pragma unroll
+  for (int k = 0; k < NCH; k++) {
+    if (!(m & (1u << k))) continue;
+    for (int d = 0; d < D; d++) dst[d][o] = 2 * pc[d] + vp.tab[k][d];
+    o++;
+  }
+}
+
+// initial lattice, x fastest (list_of_voxels.ixx:11-28, Loops.ixx:44-74)
+template <int D>
+__global__ void k_lattice(int64_t total, int n0, int n1, int n2, int n3, int32_t* c0, int32_t* c1, int32_t* c2,
+                          int32_t* c3) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int nn[4] = {n0, n1, n2, n3};
+  int32_t* cc[4] = {c0, c1, c2, c3};
+  int64_t r = i;
+  for (int d = 0; d < D; d++) {
+    cc[d][i] = (int32_t)(r % nn[d]);
+    r /= nn[d];
+  }
+}
+
+// ============================================================== kernels: extraction
+
+template <int D>
+__global__ void k_cell_counts(Geo<D> g, const uint8_t* __restrict__ cnt, int* __restrict__ blk) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int v = 0;
+  if (c < g.ncells) v = cnt[c];
+  __shared__ int sh[WG];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = WG / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blk[blockIdx.x] = sh[0];
+}
+
+template <int D>
+__global__ void k_extract(Geo<D> g, CellStore<D> cs, const int* __restrict__ blk, double* __restrict__ pos,
+                          double* __restrict__ rad, uint32_t* __restrict__ ph) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = c < g.ncells ? cs.cnt[c] : 0;
+  __shared__ int sh[WG];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < WG; off <<= 1) {
+    const int add = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  if (v == 0) return;
+  int o = blk[blockIdx.x] + sh[threadIdx.x] - v;
+  for (int k = 0; k < v; k++, o++) {
+    const Slot<D> s = cs.sph[(size_t)c * cs.cap + k];
+    for (int d = 0; d < D; d++) pos[(size_t)o * D + d] = s.c[d];
+    rad[o] = s.r;
+    ph[o] = cs.phase[(size_t)c * cs.cap + k];
+  }
+}
+
+template <int D>
+__global__ void k_insert(CellStore<D> cs, int64_t n, const int* __restrict__ cell, const int* __restrict__ slot,
+                         const double* __restrict__ pos, const double* __restrict__ rad,
+                         const uint32_t* __restrict__ ph) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Slot<D> s;
+  for (int d = 0; d < D; d++) s.c[d] = pos[i * D + d];
+  s.r = rad[i];
+  cs.sph[(size_t)cell[i] * cs.cap + slot[i]] = s;
+  cs.phase[(size_t)cell[i] * cs.cap + slot[i]] = ph[i];
+}
+
+// ============================================================== host engine
+
+static double volume_unit_sphere(int dim) {  // auxiliary/Geometry/Sphere.ixx:31-44
+  const double pi = M_PI;
+  double mult = 1;
+  if (dim % 2 == 0) {
+    for (int k = 0; k <= dim / 2 - 1; k++) mult *= (2 * pi) / (dim - 2 * k);
+  } else {
+    mult = 1. / pi;
+    for (int k = 0; k <= (dim - 1) / 2; k++) mult *= (2 * pi) / (dim - 2 * k);
+  }
+  return mult;
+}
+
+template <int D>
+static double puissance(double x) {  // AuxiFunctions.ixx:34-44
+  if constexpr (D == 0) return 1.;
+  if constexpr (D == 1) return x;
+  if constexpr (D == 2) return x * x;
+  if constexpr (D == 3) return x * (x * x);
+  if constexpr (D == 4) return (x * x) * (x * x);
+  return 0;
+}
+
+template <int D>
+static std::vector<std::array<int, D>> tab_corner_host() {  // auxiliary/Path.ixx:85-116
+  std::vector<std::array<int, D>> t(ipow_c(2, D));
+  if constexpr (D == 2) {
+    t = {{{0, 0}}, {{1, 1}}, {{0, 1}}, {{1, 0}}};
+  } else if constexpr (D == 3) {
+    t = {{{0, 0, 0}}, {{1, 1, 1}}, {{0, 0, 1}}, {{1, 1, 0}}, {{0, 1, 0}}, {{1, 0, 1}}, {{0, 1, 1}}, {{1, 0, 0}}};
+  } else {
+    // the reference's D >= 4 table never sets component 0 (see Path.ixx:105-115)
+    for (int i = 0; i < ipow_c(2, D); i++)
+      for (int d = 0; d < D; d++) t[i][d] = d == 0 ? 0 : ((i >> d) & 1);
+  }
+  return t;
+}
+
+template <int D>
+class Engine final : public IEngine {
+ public:
+  Engine(const double* ginf_, const double* gsup_, int ghost_layer, double domain_rad) {
+    if (ghost_layer < 1) throw Error("ghost_layer must be >= 1 for a periodic domain");
+    gl_ = ghost_layer;
+    drad_ = domain_rad;
+    double dsize[D];
+    for (int d = 0; d < D; d++) {  // rsa_domain.ixx:51-56 on one rank
+      ginf_a[d] = ginf_[d];
+      gsup_a[d] = gsup_[d];
+      dsize[d] = gsup_a[d] - ginf_a[d];
+      const double size = dsize[d] / 1;
+      inf_[d] = ginf_a[d] + 0 * size;
+      sup_[d] = ginf_a[d] + (0 + 1) * size;
+      L_[d] = dsize[d];
+    }
+    define_grid(domain_rad);  // the domain's own grid (rsa_domain.ixx:26); replaced by algo_init
+    for (int d = 1; d < D; d++)
+      if (sup_[d] - inf_[d] < 2 * domain_rad) throw Error("The domain is not sufficiently large!");
+    HIP_OK(hipStreamCreate(&st_));
+    HIP_OK(hipHostMalloc((void**)&hctr_, sizeof(int) * C_NUM, hipHostMallocDefault));
+    dctr_.reserve(C_NUM);
+    ring_.reserve(kRing);
+    aimed_ = D <= 3 ? 0.95 : 1 - 0.05 * puissance<(D > 3 ? D - 3 : 0)>(0.1);
+  }
+  ~Engine() override {
+    if (st_) (void)hipStreamDestroy(st_);
+    if (hctr_) (void)hipHostFree(hctr_);
+  }
+
+  int dim() const override { return D; }
+  double total_volume() const override {  // rsa_domain.ixx:166-169
+    double v = 1.;
+    for (int d = 0; d < D; d++) v *= (gsup_a[d] - ginf_a[d]);
+    return v;
+  }
+
+  // RadiusGenerator (radius_generator.ixx:9-28) then rsa_algo ctor (operator_algorithm.ixx:15-30)
+  void algo_init(const std::vector<RadiusClass>& cls, double exclusion, int nbshots) override {
+    set_classes(cls, exclusion);
+    nbshots_ = nbshots;
+    r_max_ = gmax_;
+    aimed_ = D <= 3 ? 0.95 : 1 - 0.05 * puissance<(D > 3 ? D - 3 : 0)>(0.1);
+    define_grid(gmax_);
+    if (gmax_ > drad_)
+      throw Error("Impossible to have the maximal radius of radius generator larger than domain implicit radius");
+    // cells beyond the ghost-band width would need images further than one layer
+    for (int d = 0; d < D; d++)
+      if (2 * drad_ > cl_[d])
+        throw Error("domain radius too large for the cell grid: the ghost band must fit in one cell layer");
+    alloc_cells();
+    algo_ready_ = true;
+  }
+
+  void reset_classes(const std::vector<RadiusClass>& cls, double exclusion) override {  // :238-250
+    if (!algo_ready_) throw Error("rsa_algo not initialised");
+    const auto saved = classes_;
+    const double old_max = gmax_, old_min = gmin_;
+    const int old_cur = cur_;
+    set_classes(cls, exclusion);  // the new generator (its constructor may throw)
+    if (old_max < gmax_) {
+      classes_ = saved;
+      gmax_ = old_max;
+      gmin_ = old_min;
+      cur_ = old_cur;
+      throw Error("Incorrect maximal radius");
+    }
+    r_max_ = std::max(r_max_, gmax_);
+  }
+
+  void set_miss_rate(double m) override { aimed_ = m; }
+
+  void add_spheres(int64_t n, const double* pos, const double* rad, const uint64_t* phase) override {
+    if (!algo_ready_) throw Error("rsa_algo not initialised");
+    if (n <= 0) return;
+    std::vector<uint8_t> cnt(geo_.ncells);
+    HIP_OK(hipMemcpy(cnt.data(), cells_cnt_.p, geo_.ncells, hipMemcpyDeviceToHost));
+    std::vector<int> cell(n), slot(n);
+    for (int64_t i = 0; i < n; i++) {  // build_grid(..., Validated) appends in input order
+      const int64_t c = cell_index<D>(geo_, pos + i * D);
+      if (c < 0 || c >= geo_.ncells) throw Error("sphere centre outside the cell grid");
+      if (cnt[c] >= cap_) throw Error("cell capacity exceeded");
+      cell[i] = (int)c;
+      slot[i] = cnt[c]++;
+    }
+    DevBuf<int> dcell, dslot;
+    DevBuf<double> dpos, drd;
+    DevBuf<uint32_t> dph;
+    dcell.reserve(n);
+    dslot.reserve(n);
+    dpos.reserve(n * D);
+    drd.reserve(n);
+    dph.reserve(n);
+    std::vector<uint32_t> ph(n);
+    for (int64_t i = 0; i < n; i++) ph[i] = (uint32_t)phase[i];
+    HIP_OK(hipMemcpy(dcell.p, cell.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dslot.p, slot.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dpos.p, pos, n * D * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(drd.p, rad, n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dph.p, ph.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(cells_cnt_.p, cnt.data(), geo_.ncells, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_insert<D>, dim3(nblocks(n)), dim3(WG), 0, st_, cs(), n, dcell.p, dslot.p, dpos.p, drd.p, dph.p);
+    HIP_OK(hipStreamSynchronize(st_));
+  }
+
+  uint64_t count() const override { return nb_fast_; }
+
+  uint64_t extract(double* pos, double* rad, uint64_t* phase, uint64_t cap) override {
+    if (!algo_ready_) return 0;
+    const unsigned nb = nblocks(geo_.ncells);
+    DevBuf<int> blk;
+    blk.reserve(nb);
+    hipLaunchKernelGGL(k_cell_counts<D>, dim3(nb), dim3(WG), 0, st_, geo_, cells_cnt_.p, blk.p);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, blk.p, (int)nb, dctr_.p + C_CURSOR);
+    int total = 0;
+    HIP_OK(hipMemcpyAsync(&total, dctr_.p + C_CURSOR, sizeof(int), hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipStreamSynchronize(st_));
+    if (pos == nullptr) return (uint64_t)total;
+    if ((uint64_t)total > cap) throw Error("extract: output capacity too small");
+    DevBuf<double> dp, dr;
+    DevBuf<uint32_t> dph;
+    dp.reserve((size_t)total * D);
+    dr.reserve(total);
+    dph.reserve(total);
+    hipLaunchKernelGGL(k_extract<D>, dim3(nb), dim3(WG), 0, st_, geo_, cs(), blk.p, dp.p, dr.p, dph.p);
+    HIP_OK(hipStreamSynchronize(st_));
+    HIP_OK(hipMemcpy(pos, dp.p, (size_t)total * D * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(rad, dr.p, (size_t)total * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> ph(total);
+    HIP_OK(hipMemcpy(ph.data(), dph.p, (size_t)total * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int i = 0; i < total; i++) phase[i] = ph[i];
+    return (uint64_t)total;
+  }
+
+  double volume_of_spheres() override {  // rsa_grid.ixx:61-71: per real cell, cells summed in index order
+    const uint64_t n = extract(nullptr, nullptr, nullptr, 0);
+    std::vector<double> p(n * D), r(n);
+    std::vector<uint64_t> ph(n);
+    extract(p.data(), r.data(), ph.data(), n);
+    const double mult = volume_unit_sphere(D);
+    double res = 0.;
+    // the reference sums per cell then adds the cell sums; redo that grouping
+    std::vector<uint8_t> cnt(geo_.ncells);
+    HIP_OK(hipMemcpy(cnt.data(), cells_cnt_.p, geo_.ncells, hipMemcpyDeviceToHost));
+    uint64_t k = 0;
+    for (int c = 0; c < geo_.ncells; c++) {
+      if (!cnt[c]) continue;
+      double loc = 0.;
+      for (int j = 0; j < cnt[c]; j++, k++) loc += mult * puissance<D>(r[k]);
+      res += loc;
+    }
+    return res;
+  }
+
+  uint64_t voxels(int64_t* coords, double* len, uint64_t cap) override {
+    if (len)
+      for (int d = 0; d < D; d++) len[d] = vlen_[d];
+    if (coords == nullptr) return (uint64_t)nvox_;
+    if ((uint64_t)nvox_ > cap) throw Error("voxels: output capacity too small");
+    std::vector<int32_t> tmp(nvox_);
+    for (int d = 0; d < D; d++) {
+      if (nvox_) HIP_OK(hipMemcpy(tmp.data(), vc_[vcur_][d].p, nvox_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < nvox_; i++) coords[i * D + d] = tmp[i];
+    }
+    return (uint64_t)nvox_;
+  }
+
+  void proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* out) override;
+
+ private:
+  // ---------------------------------------------------------------- state
+  static constexpr uint64_t kRing = 1ull << 23;  // words in the device ring (32 MiB)
+  double ginf_a[D], gsup_a[D], inf_[D], sup_[D], L_[D], cl_[D];
+  double drad_ = 0.05;
+  int gl_ = 1;
+  Geo<D> geo_{};
+  hipStream_t st_ = nullptr;
+  int* hctr_ = nullptr;
+  DevBuf<int> dctr_;
+
+  // radius generator (radius_generator.hxx:131-137)
+  std::vector<std::tuple<double, uint64_t, int>> classes_;
+  double gmax_ = 0, gmin_ = 0;
+  int cur_ = 0;
+  double r_max_ = 0;
+  int nbshots_ = 6000;
+  double aimed_ = 0.95;
+  bool algo_ready_ = false;
+
+  // cells
+  int cap_ = 8;
+  DevBuf<uint8_t> cells_cnt_;
+  DevBuf<Slot<D>> cells_sph_;
+  DevBuf<uint32_t> cells_ph_;
+  DevBuf<int> cell2t_, cellcc_;
+  uint64_t nb_fast_ = 0;
+
+  // voxels
+  DevBuf<int32_t> vc_[2][D];
+  int vcur_ = 0;
+  int64_t nvox_ = 0;
+  double vlen_[D] = {}, vorigin_[D] = {};
+  std::vector<std::array<int, D>> tab_;
+
+  // candidates
+  DevBuf<double> cpos_, crad_;
+  DevBuf<uint32_t> cprio_, cphase_, vpos_, undec_;
+  DevBuf<int> ccell_, carr_, touched_, seg_, len_, ord_, poff_, pcnt_, plist_;
+  DevBuf<uint8_t> ctag_, lab_;
+
+  // refine scratch
+  DevBuf<uint32_t> code_;
+  DevBuf<int> blk_, holes_;
+
+  // word stream
+  MTStream hs_;
+  DevBuf<uint32_t> ring_;
+  uint64_t dev_hi_ = 0;  // words [0, dev_hi_) have been written to the ring
+
+  CellStore<D> cs() { return CellStore<D>{cells_cnt_.p, cells_sph_.p, cells_ph_.p, cap_}; }
+
+  void define_grid(double rad) {  // rsa_grid_traversal.ixx:11-73
+    for (int d = 0; d < D; d++) {
+      const double length = rad * 2;
+      const double width = sup_[d] - inf_[d];
+      const int nin = static_cast<int>(width / length);
+      if (nin == 0) throw Error("Too small domain wrt ghost layer");
+      cl_[d] = width / nin;
+      geo_.cinv[d] = 1. / cl_[d];
+      geo_.cshift[d] = inf_[d] - gl_ * cl_[d];
+      geo_.n[d] = nin + 2 * gl_;
+      geo_.inf[d] = inf_[d];
+      geo_.sup[d] = sup_[d];
+      geo_.lo_band_hi[d] = inf_[d] + 2 * drad_;  // rsa_domain.ixx:117-133
+      geo_.hi_band_lo[d] = sup_[d] - 2 * drad_;
+      geo_.L[d] = L_[d];
+    }
+    geo_.gl = gl_;
+    int64_t nc = 1;
+    for (int d = 0; d < D; d++) nc *= geo_.n[d];
+    if (nc > (int64_t)1 << 30) throw Error("cell grid too large");
+    geo_.ncells = (int)nc;
+    // stencil order (rsa_grid_traversal.ixx:30-66): libstdc++ std::sort by squared norm
+    const int n3 = ipow_c(3, D);
+    std::vector<int> order(n3);
+    std::vector<std::array<double, D>> vec(n3);
+    for (int i = 0; i < n3; i++) {
+      order[i] = i;
+      int tmp = i;
+      for (int d = 0; d < D; d++) {
+        const int r = tmp % 3;
+        vec[i][d] = r - 1;
+        tmp = (tmp - r) / 3;
+      }
+    }
+    auto norm2 = [](const std::array<double, D>& v) {
+      double s = 0.;
+      for (int d = 0; d < D; d++) s += v[d] * v[d];
+      return s;
+    };
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return norm2(vec[a]) < norm2(vec[b]); });
+    geo_.nst = n3;
+    for (int i = 0; i < n3; i++)
+      for (int d = 0; d < 4; d++) geo_.st[i][d] = d < D ? (signed char)vec[order[i]][d] : 0;
+  }
+
+  void set_classes(const std::vector<RadiusClass>& cls, double exclusion) {  // radius_generator.ixx:9-28
+    if (cls.empty()) throw Error("Empty radius generator!");
+    classes_.clear();
+    for (const auto& c : cls) classes_.emplace_back(c.radius, c.count, c.phase);
+    std::sort(classes_.begin(), classes_.end(), [](auto& a, auto& b) { return std::get<0>(a) > std::get<0>(b); });
+    for (size_t i = 0; i < classes_.size(); i++) std::get<0>(classes_[0]) += 0.5 * exclusion;
+    gmax_ = std::get<0>(classes_[0]);
+    gmin_ = std::get<0>(classes_[classes_.size() - 1]);
+    cur_ = 0;
+  }
+
+  void alloc_cells() {
+    // Spheres per cell.  Disjoint balls of radius r_min centred in the cell lie in
+    // the cell grown by r_min per side (volume bound).  With equal radii, if
+    // the half-cell diagonal is below 2r no two centres share a half-cell, so
+    // at most 2^D (a margin keeps rounding of the collision test harmless).
+    double vol = 1.;
+    for (int d = 0; d < D; d++) vol *= (cl_[d] + 2 * gmin_);
+    const double vb = volume_unit_sphere(D) * puissance<D>(gmin_);
+    int cap = (int)std::ceil(vol / vb);
+    if (gmax_ == gmin_) {
+      double diag2 = 0.;
+      for (int d = 0; d < D; d++) diag2 += 0.25 * cl_[d] * cl_[d];
+      if (std::sqrt(diag2) < 2 * gmin_ * (1 - 1e-9)) cap = std::min(cap, ipow_c(2, D));
+    }
+    cap = std::max(cap, 2);
+    if (cap > 255) throw Error("radius ratio too large for the per-cell capacity");
+    cap_ = cap;
+    cells_cnt_.reserve(geo_.ncells);
+    cells_sph_.reserve((size_t)geo_.ncells * cap_);
+    cells_ph_.reserve((size_t)geo_.ncells * cap_);
+    cell2t_.reserve(geo_.ncells);
+    cellcc_.reserve(geo_.ncells);
+    HIP_OK(hipMemset(cells_cnt_.p, 0, geo_.ncells));
+    HIP_OK(hipMemset(cell2t_.p, 0xff, (size_t)geo_.ncells * sizeof(int)));
+    HIP_OK(hipMemset(cellcc_.p, 0, (size_t)geo_.ncells * sizeof(int)));
+    nb_fast_ = 0;
+  }
+
+  uint64_t current_number() const { return std::get<1>(classes_[cur_]); }
+  bool still() const { return (uint64_t)cur_ < classes_.size(); }
+  void update_placed(uint64_t nb) {  // radius_generator.ixx:62-68
+    std::get<1>(classes_[cur_]) = current_number() - nb;
+    if (current_number() == 0) cur_++;
+  }
+
+  void reset_ctr() { HIP_OK(hipMemsetAsync(dctr_.p, 0, sizeof(int) * C_NUM, st_)); }
+  void read_ctr() {
+    HIP_OK(hipMemcpyAsync(hctr_, dctr_.p, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipStreamSynchronize(st_));
+  }
+
+  // copy words [dev_hi_, upto) of the host stream into the ring
+  void upload_words(uint64_t upto) {
+    hs_.ensure(upto);
+    while (dev_hi_ < upto) {
+      const uint64_t off = dev_hi_ & (kRing - 1);
+      const uint64_t n = std::min<uint64_t>(upto - dev_hi_, kRing - off);
+      HIP_OK(hipMemcpyAsync(ring_.p + off, hs_.buf.data() + (dev_hi_ - hs_.base), n * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, st_));
+      dev_hi_ += n;
+    }
+    HIP_OK(hipStreamSynchronize(st_));
+  }
+
+  void ensure_cand(int n) {
+    if ((size_t)n <= ccell_.cap && ccell_.p) return;
+    const size_t c = (size_t)std::max(n, 1024) * 5 / 4;
+    cpos_.reserve(c * D);
+    crad_.reserve(c);
+    cprio_.reserve(c);
+    cphase_.reserve(c);
+    vpos_.reserve(c);
+    undec_.reserve(c);
+    ccell_.reserve(c);
+    carr_.reserve(c);
+    touched_.reserve(c);
+    seg_.reserve(c);
+    len_.reserve(c);
+    ord_.reserve(c);
+    poff_.reserve(c);
+    pcnt_.reserve(c);
+    ctag_.reserve(c);
+    lab_.reserve(c);
+    if (plist_.cap < 8 * c) plist_.reserve(8 * c);
+  }
+
+  void init_voxels(double max_diag) {  // list_of_voxels.ixx:11-28, 192-211
+    std::array<int, D> nv{};
+    tab_ = tab_corner_host<D>();
+    int64_t total = 1;
+    for (int d = 0; d < D; d++) {
+      const double width = sup_[d] - inf_[d];
+      nv[d] = 1 + static_cast<int>(std::sqrt(D) * width / max_diag);
+      vlen_[d] = width / nv[d];
+      vorigin_[d] = inf_[d];
+      total *= nv[d];
+    }
+    if (total > ((int64_t)1 << 31) - 1) throw Error("initial voxel lattice too large");
+    nvox_ = total;
+    vcur_ = 0;
+    for (int d = 0; d < D; d++) vc_[0][d].reserve(total);
+    int32_t* c[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int d = 0; d < D; d++) c[d] = vc_[0][d].p;
+    hipLaunchKernelGGL(k_lattice<D>, dim3(nblocks(total)), dim3(WG), 0, st_, total, nv[0], D > 1 ? nv[1] : 1,
+                       D > 2 ? nv[2] : 1, D > 3 ? nv[3] : 1, c[0], c[1], c[2], c[3]);
+    HIP_OK(hipGetLastError());
+    max_coord_ = 0;
+    for (int d = 0; d < D; d++) max_coord_ = std::max<int64_t>(max_coord_, nv[d]);
+  }
+  int64_t max_coord_ = 0;
+
+  VoxView<D> vview() const {
+    VoxView<D> v;
+    for (int d = 0; d < D; d++) {
+      v.c[d] = vc_[vcur_][d].p;
+      v.origin[d] = vorigin_[d];
+      v.len[d] = vlen_[d];
+    }
+    return v;
+  }
+
+  VoxParams<D> vparams(double rmin) const {
+    VoxParams<D> vp{};
+    for (int d = 0; d < D; d++) {
+      vp.origin[d] = vorigin_[d];
+      vp.len[d] = vlen_[d];
+      vp.nl[d] = 0.5 * vlen_[d];  // new_voxel_lengths (list_of_voxels.ixx:62)
+    }
+    for (int k = 0; k < VoxParams<D>::NCH; k++)
+      for (int d = 0; d < D; d++) {
+        vp.tab[k][d] = (signed char)tab_[k][d];
+        vp.corners[k][d] = tab_[k][d] * vp.len[d];  // create_corners_voxel_inplace (:309-318)
+        vp.ncorners[k][d] = tab_[k][d] * vp.nl[d];
+      }
+    vp.rmin = rmin;
+    return vp;
+  }
+
+  // remove_covered + subdivide_uncovered (operator_algorithm.ixx:190-197)
+  void refine(double rmin) {
+    if (nvox_ == 0) return;
+    const unsigned nb = nblocks(nvox_);
+    code_.reserve(nvox_);
+    blk_.reserve(nb);
+    holes_.reserve(nvox_);
+    const VoxParams<D> vp = vparams(rmin);
+    VoxView<D> vx = vview();
+    hipLaunchKernelGGL(k_cover<D>, dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, code_.p, blk_.p);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, blk_.p, (int)nb, dctr_.p + C_KEEP);
+    hipLaunchKernelGGL(k_holes, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
+    hipLaunchKernelGGL(k_move<D>, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p, vx);
+    read_ctr();
+    const int64_t keep = hctr_[C_KEEP];
+    if (keep == 0) {  // subdivide_uncovered returns early on an empty list (list_of_voxels.ixx:60)
+      nvox_ = 0;
+      return;
+    }
+    if (keep * (int64_t)VoxParams<D>::NCH > ((int64_t)1 << 31) - 1) throw Error("voxel list would exceed 2^31 entries");
+    const unsigned nb2 = nblocks(keep);
+    hipLaunchKernelGGL(k_child_count, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, blk_.p, (int)nb2, dctr_.p + C_NCHILD);
+    read_ctr();
+    const int64_t nchild = hctr_[C_NCHILD];
+    if (2 * max_coord_ > ((int64_t)1 << 31)) throw Error("voxel lattice coordinates exceed 31 bits");
+    const int nxt = 1 - vcur_;
+    for (int d = 0; d < D; d++) vc_[nxt][d].reserve(std::max<int64_t>(nchild, 1));
+    int32_t* dst[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int d = 0; d < D; d++) dst[d] = vc_[nxt][d].p;
+    hipLaunchKernelGGL(k_scatter<D>, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p, vx, dst[0], dst[1], dst[2],
+                       dst[3], vp);
+    HIP_OK(hipGetLastError());
+    vcur_ = nxt;
+    nvox_ = nchild;
+    max_coord_ = 2 * max_coord_;
+    for (int d = 0; d < D; d++) vlen_[d] = 0.5 * vlen_[d];
+  }
+
+  double mom_pivot(std::vector<uint64_t> values, int64_t nb_first);
+  int64_t single_draw(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
+                      uint64_t& w_next, rsa_gpu_stats* st);
+};
+
+// median_of_medians::find_pivot_for_first_elements on one rank (median_of_medians.ixx:8-178)
+template <int D>
+double Engine<D>::mom_pivot(std::vector<uint64_t> values, int64_t nb_first) {
+  const int64_t gsize = (int64_t)values.size();
+  if (nb_first > gsize)
+    throw Error("Impossible to find the pivot for N first elements if N is smaller than the size of the array");
+  if (nb_first <= 0) throw Error("Impossible to find the pivot for N first elements if N  <= 0");
+  if (gsize == 0) throw Error("Impossible to find the pivot for N first elements if no element");
+  const double pct = nb_first / (1. * gsize);
+  std::sort(values.begin(), values.end(), std::less<uint64_t>{});
+  auto count_le = [&](double pivot) -> int64_t {  // upper_bound with less<uint64_t>
+    if (values.empty()) return 0;
+    auto it = std::upper_bound(values.begin(), values.end(), pivot, std::less<uint64_t>{});
+    return (int)(it - values.begin());
+  };
+  double first, secnd;
+  {
+    int64_t piv = static_cast<int64_t>(values.size() * pct) - 1;
+    int64_t mn = piv, mx = piv + 1;
+    if (std::abs(values.size() * pct - 1 - piv) < 1e-8) mn -= 1;
+    if (mn < 0) mn = 0;
+    if (std::abs(values.size() * pct - 1 - piv - 1) < 1e-8) mx += 1;
+    if (mx > int64_t(values.size()) - 1) mx = values.size() - 1;
+    first = values[mn] - std::numeric_limits<double>::epsilon();
+    secnd = values[mx] + std::numeric_limits<double>::epsilon();
+  }
+  std::vector<double> cand = {first, secnd};
+  std::sort(cand.begin(), cand.end(), std::less<uint64_t>{});
+  auto cmp = [&](int64_t nb, double p) { return nb <= count_le(p); };
+  auto it = std::upper_bound(cand.begin(), cand.end(), nb_first, cmp);
+  if (it == cand.end()) throw Error("Unexpected - 0");
+  const double pmax = *it;
+  double pivot = pmax;
+  int64_t below = count_le(pivot);
+  if (below != nb_first) {
+    if (it == cand.begin()) throw Error("Unexpected - 1");
+    const double pmin = *(it - 1);
+    const double step = pmax - pmin;
+    double add = 0.5 * step;
+    for (int j = 0; j < 20; j++) {
+      pivot = pmin + add;
+      below = count_le(pivot);
+      if (below == nb_first) break;
+      if (below > nb_first) add = 0.5 * std::abs(add);
+      if (below < nb_first) add = -0.5 * std::abs(add);
+    }
+  }
+  return pivot;
+}
+
+// generate_spheres + add_to_sample loop (operator_algorithm.ixx:534-582, 383-443)
+template <int D>
+int64_t Engine<D>::single_draw(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
+                               uint64_t& w_next, rsa_gpu_stats* st) {
+  ensure_cand(nshots);
+  const uint32_t N = (uint32_t)nvox_;
+  const uint32_t thr = (uint32_t)(0u - N) % N;
+  int nwin = nshots * (2 * D + 1) + std::max(1024, nshots / 4);
+  const unsigned nbs = nblocks(nshots);
+  for (;;) {  // word window: grows only if the rejections overflow it
+    upload_words(w0 + (uint64_t)nwin + (uint64_t)nshots + 64);
+    reset_ctr();
+    const size_t shmem = sizeof(uint32_t) * ((nwin + 31) / 32 + 1);
+    if (shmem > 60000) throw Error("word window too large for one workgroup");
+    hipLaunchKernelGGL(k_chain<D>, dim3(1), dim3(1024), shmem, st_, ring_.p, kRing - 1, w0, nwin, nshots, N, thr,
+                       vpos_.p, dctr_.p);
+    read_ctr();
+    if (!hctr_[C_CHAIN_OVF]) break;
+    nwin *= 2;
+  }
+  const int end_rel = hctr_[C_END_REL];
+  upload_words(w0 + (uint64_t)end_rel + (uint64_t)nshots);
+  ClassDev cls{std::get<0>(classes_[cur_]), (uint32_t)std::get<2>(classes_[cur_])};
+  hipLaunchKernelGGL(k_decode<D>, dim3(nbs), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N, vpos_.p,
+                     vview(), cls, geo_, cpos_.p, crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p, dctr_.p);
+  hipLaunchKernelGGL(k_bin_count, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, cellcc_.p, carr_.p, cell2t_.p,
+                     touched_.p, dctr_.p);
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st_, touched_.p, cellcc_.p, seg_.p, len_.p, dctr_.p);
+  hipLaunchKernelGGL(k_bin_place, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, carr_.p, cell2t_.p, seg_.p, ord_.p);
+  hipLaunchKernelGGL(k_bin_sort, dim3(nbs), dim3(WG), 0, st_, seg_.p, len_.p, ord_.p, dctr_.p);
+  HIP_OK(hipGetLastError());
+  w_next = w0 + (uint64_t)end_rel + (uint64_t)nshots;
+
+  int64_t local_new = 0;
+  rounds = 0;
+  for (;;) {
+    rounds++;
+    // per-round counters
+    HIP_OK(hipMemsetAsync(dctr_.p + C_CURSOR, 0, sizeof(int) * 4, st_));
+    hipLaunchKernelGGL(k_filter<D>, dim3(nbs), dim3(WG), 0, st_, geo_, cs(), nshots, cpos_.p, crad_.p, ccell_.p,
+                       ctag_.p, cell2t_.p, seg_.p, len_.p, ord_.p);
+    hipLaunchKernelGGL(k_list_remove, dim3(nbs), dim3(WG), 0, st_, 0, seg_.p, len_.p, ord_.p, ctag_.p, dctr_.p);
+    for (;;) {
+      HIP_OK(hipMemsetAsync(dctr_.p + C_CURSOR, 0, sizeof(int), st_));
+      hipLaunchKernelGGL(k_partners<D>, dim3(nbs), dim3(WG), 0, st_, geo_, nshots, cpos_.p, crad_.p, cprio_.p,
+                         ccell_.p, ctag_.p, cell2t_.p, seg_.p, len_.p, ord_.p, poff_.p, pcnt_.p, plist_.p,
+                         (int)plist_.cap, dctr_.p);
+      read_ctr();
+      if (hctr_[C_OVERFLOW] == 2) {
+        plist_.reserve((size_t)hctr_[C_CURSOR] * 2);
+        HIP_OK(hipMemsetAsync(dctr_.p + C_OVERFLOW, 0, sizeof(int), st_));
+        continue;
+      }
+      break;
+    }
+    hipLaunchKernelGGL(k_label, dim3(1), dim3(1024), 0, st_, nshots, ctag_.p, lab_.p, poff_.p, pcnt_.p, plist_.p,
+                       dctr_.p);
+    read_ctr();
+    if (hctr_[C_LABEL_STUCK]) throw Error("internal: conflict labelling made no progress");
+    const uint64_t nb_undecided = (uint64_t)hctr_[C_ACCEPTED];
+    bool outreached = false;
+    uint64_t total_undecided = 0;
+    if (may_outreach) {
+      total_undecided = nb_undecided;
+      outreached = total_undecided > maximal;
+    }
+    if (outreached) {  // keep the first `maximal` by priority (operator_algorithm.ixx:416-421)
+      HIP_OK(hipMemsetAsync(dctr_.p + C_CURSOR, 0, sizeof(int), st_));
+      hipLaunchKernelGGL(k_gather_undecided, dim3(nbs), dim3(WG), 0, st_, nshots, cprio_.p, ctag_.p, undec_.p,
+                         dctr_.p);
+      read_ctr();
+      std::vector<uint32_t> pr(hctr_[C_CURSOR]);
+      if (!pr.empty())
+        HIP_OK(hipMemcpy(pr.data(), undec_.p, pr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      std::vector<uint64_t> prio(pr.begin(), pr.end());
+      const double pivot = mom_pivot(prio, (int64_t)maximal);
+      int64_t fin = 0;
+      for (uint64_t p : prio)
+        if (!((double)p > pivot)) fin++;
+      hipLaunchKernelGGL(k_retain, dim3(nbs), dim3(WG), 0, st_, nshots, pivot, cprio_.p, ctag_.p);
+      hipLaunchKernelGGL(k_update, dim3(nbs), dim3(WG), 0, st_, seg_.p, len_.p, ord_.p, ctag_.p, dctr_.p);
+      local_new += fin;
+      break;
+    }
+    HIP_OK(hipMemsetAsync(dctr_.p + C_REMAIN, 0, sizeof(int), st_));
+    hipLaunchKernelGGL(k_update, dim3(nbs), dim3(WG), 0, st_, seg_.p, len_.p, ord_.p, ctag_.p, dctr_.p);
+    local_new += (int64_t)nb_undecided;
+    const uint64_t left = maximal - total_undecided;  // by-value parameter: draw-start value each round
+    if (left == 0) {
+      hipLaunchKernelGGL(k_list_remove, dim3(nbs), dim3(WG), 0, st_, 1, seg_.p, len_.p, ord_.p, ctag_.p, dctr_.p);
+      break;
+    }
+    read_ctr();
+    if (hctr_[C_REMAIN] == 0) break;
+  }
+  hipLaunchKernelGGL(k_commit<D>, dim3(nbs), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p, ctag_.p,
+                     touched_.p, seg_.p, len_.p, ord_.p, cell2t_.p, cellcc_.p, dctr_.p);
+  read_ctr();
+  if (hctr_[C_OVERFLOW] == 1) throw Error("cell capacity exceeded");
+  st->ghost_cell_drops += hctr_[C_DROPPED];
+  st->image_cell_mismatch += hctr_[C_IMG_MISMATCH];
+  return local_new;
+}
+
+// proceed<1> -> proceed_voxel (operator_algorithm.ixx:85-188, 208-236)
+template <int D>
+void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
+  if (!algo_ready_) throw Error("rsa_algo not initialised");
+  const auto t0 = std::chrono::steady_clock::now();
+  std::memset(st, 0, sizeof(*st));
+  hs_.reset(seed);
+  dev_hi_ = 0;
+  uint64_t w = 0;
+  init_voxels(r_max_ + gmin_);
+  st->nvox_max = nvox_;
+  uint64_t old_nb = 0;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  double vox_ms = 0;
+  for (;;) {
+    st->loop_iterations++;
+    if (max_draws >= 0 && st->draws >= max_draws) {
+      st->end_reason = 3;
+      break;
+    }
+    const int nv = (int)nvox_;
+    const double intensity = 0.01 * nv;
+    const double max_intensity = intensity;
+    const uint64_t total_vox = (uint64_t)nvox_;
+    if (total_vox == 0) {
+      st->end_reason = 1;
+      break;
+    }
+    double mult = nbshots_ / max_intensity;
+    if (mult > 1) mult = 1.;
+    StreamURNG g{&hs_, w};
+    std::poisson_distribution<> pd(mult * intensity);
+    const int nshots_i = pd(g);
+    w = g.pos;
+    const uint64_t nshots = (uint64_t)nshots_i;
+    if (nshots == 0) continue;
+    if (nshots > (uint64_t)INT32_MAX / 16) throw Error("too many shots in one draw");
+    const bool may_outreach = nshots >= current_number();
+    const uint64_t maximal = current_number();
+    int64_t rounds = 0;
+    uint64_t w_next = 0;
+    const uint64_t nb_new = (uint64_t)single_draw((int)nshots, maximal, may_outreach, w, rounds, w_next, st);
+    w = w_next;
+    hs_.drop_below(w);
+    st->draws++;
+    st->shots += (int64_t)nshots;
+    st->accepted += (int64_t)nb_new;
+    st->rounds += rounds;
+    st->rounds_max = std::max(st->rounds_max, rounds);
+    update_placed(nb_new);
+    const bool cont = still();
+    nb_fast_ += nb_new;
+    const int new_nb = (int)nb_fast_;
+    const uint64_t nb_miss = nshots - (uint64_t)(new_nb - old_nb);
+    old_nb = (uint64_t)new_nb;
+    const double miss_rate = nb_miss / (0.000001 + nshots);
+    if (miss_rate > aimed_) {
+      HIP_OK(hipEventRecord(e0, st_));
+      refine(gmin_);
+      HIP_OK(hipEventRecord(e1, st_));
+      HIP_OK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+      vox_ms += ms;
+      st->refines++;
+      st->nvox_max = std::max<int64_t>(st->nvox_max, nvox_);
+    }
+    if (!cont) {
+      st->end_reason = 2;
+      break;
+    }
+  }
+  HIP_OK(hipStreamSynchronize(st_));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  st->nvox_final = nvox_;
+  st->words = (int64_t)w;
+  st->voxel_seconds = vox_ms * 1e-3;
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+IEngine* make_engine(int dim, const double* inf, const double* sup, int gl, double rad) {
+  switch (dim) {
+    case 2:
+      return new Engine<2>(inf, sup, gl, rad);
+    case 3:
+      return new Engine<3>(inf, sup, gl, rad);
+    case 4:
+      return new Engine<4>(inf, sup, gl, rad);
+    default:
+      throw Error("dimension must be 2, 3 or 4");
+  }
+}
+
+}  // namespace rsa_amd

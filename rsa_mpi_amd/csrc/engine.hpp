@@ -1,0 +1,45 @@
+// Host-side interface of one single-rank RSA context (one periodic domain on
+// one GPU).  Implemented per dimension in engine.hip; the C-ABI in capi.cpp
+// only sees this interface.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rsa_mpi_amd.h"
+
+namespace rsa_amd {
+
+struct RadiusClass {
+  double radius;
+  uint64_t count;
+  int phase;
+};
+
+class IEngine {
+ public:
+  virtual ~IEngine() = default;
+  virtual int dim() const = 0;
+  virtual void algo_init(const std::vector<RadiusClass>& classes, double exclusion, int nbshots) = 0;
+  virtual void reset_classes(const std::vector<RadiusClass>& classes, double exclusion) = 0;
+  virtual void set_miss_rate(double m) = 0;
+  virtual void proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) = 0;
+  virtual void add_spheres(int64_t n, const double* pos, const double* rad, const uint64_t* phase) = 0;
+  virtual uint64_t count() const = 0;
+  virtual uint64_t extract(double* pos, double* rad, uint64_t* phase, uint64_t cap) = 0;
+  virtual double volume_of_spheres() = 0;
+  virtual uint64_t voxels(int64_t* coords, double* len, uint64_t cap) = 0;
+  virtual double total_volume() const = 0;
+};
+
+IEngine* make_engine(int dim, const double* inf, const double* sup, int ghost_layer, double domain_rad);
+
+// reference error messages (rsa_grid_traversal.ixx:22, operator_algorithm.ixx:28,246,
+// radius_generator.ixx:18, list_of_voxels.ixx:113, rsa_domain.ixx:140)
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+}  // namespace rsa_amd
