@@ -1,0 +1,120 @@
+#!/usr/bin/env python
+"""Headline benchmark: accepted spheres/s to maximal packing on one MI355X.
+
+A step is one proceed(seed) of the reference's voxel method (method 1) from an
+empty periodic box to a maximal packing: 3D monodisperse r=0.005 in the unit
+cube, seed 0, nbshots_singledraw 6000 (~734k spheres).  Inputs are synthetic
+and generated on the device from the seed; the timed region is proceed() only
+(domain and buffers are set up before it).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "accepted spheres/sec to maximal packing (3D, 1 MI355X) learned vs plain; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = dict(dim=3, L=1.0, r=0.005, seed=0, nbshots=6000)
+
+
+def run_gpu_step(R, profile_level=1):
+    w = WORKLOAD
+    dom = R.Domain([0.0] * w["dim"], [w["L"]] * w["dim"], w["r"])
+    gen = R.RadiusGenerator([(w["r"], 1.0, 0)], dom.get_total_volume(), 0.0)
+    algo = R.Algo(dom, gen, w["nbshots"])
+    if profile_level:
+        dom.profile(profile_level)
+    t0 = time.perf_counter()
+    st = algo.proceed(w["seed"])
+    dt = time.perf_counter() - t0
+    return dom, st, dt
+
+
+def cpu_baseline(side=0.8):
+    """The CPU oracle (restatement of the reference's single-rank path) on all
+    host cores, on a bounded sample of the same workload: the same radius and
+    seed in a periodic cube of side `side`, run to maximal packing."""
+    import _oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    O.build()
+    t0 = time.perf_counter()
+    o = O.run(3, [0.0] * 3, [side] * 3, [(WORKLOAD["r"], 1.0, 0)], seed=WORKLOAD["seed"])
+    dt = time.perf_counter() - t0
+    return dict(value=o["n"] / dt, unit="spheres/s", cores=threads, kind="port",
+                sample=f"3D r={WORKLOAD['r']} seed {WORKLOAD['seed']} in the periodic cube of side {side} "
+                       f"({side ** 3:.0%} of the workload volume) to maximal packing: {o['n']} spheres in {dt:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-side", type=float, default=0.8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py measures one GPU only: this build runs one rank's whole domain on a single MI355X "
+                 "(multi-GPU sharding is a design in DESIGN.md, not a measured path).")
+
+    import rsa_mpi_amd as R
+    R.lib()
+    for _ in range(args.warmup):
+        run_gpu_step(R, profile_level=0)
+
+    total_n, total_t = 0, 0.0
+    prof = {}
+    stats = None
+    for _ in range(args.steps):
+        dom, st, dt = run_gpu_step(R, profile_level=1)
+        total_n += st["accepted"]
+        total_t += dt
+        stats = st
+        for k, v in dom.kernel_profile().items():
+            a = prof.setdefault(k, dict(launches=0, seconds=0.0, bytes=0.0))
+            for f in a:
+                a[f] += v[f]
+    value = total_n / total_t
+
+    parity = None
+    gold = os.path.join(ROOT, "tests", "golden", "cfg2_3d_r0.005_seed0.json")
+    if os.path.exists(gold):
+        from _common import set_hash
+        g = json.load(open(gold))
+        pos, rad, _ = dom.extract_arrays()
+        parity = "sha256 match" if set_hash(pos, rad) == g["sha256"] and len(rad) == g["n"] else "MISMATCH"
+
+    k = prof.get("k_cover")
+    roofline = None
+    if k and k["seconds"] > 0:
+        achieved = k["bytes"] / k["seconds"] / 1e9
+        roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                        kernel="k_cover", launches=k["launches"] // args.steps,
+                        avg_launch_ms=round(1e3 * k["seconds"] / k["launches"], 3))
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "spheres/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * total_t / args.steps, 2), "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3D monodisperse RSA, r=0.005, periodic unit cube, seed 0, voxel method to "
+                               "maximal packing (plain GPU form)", "spheres": stats["accepted"],
+                   "draws": stats["draws"], "refines": stats["refines"], "nvox_max": stats["nvox_max"],
+                   "mt_words": stats["words"], "parity_vs_oracle": parity},
+        "roofline": roofline,
+        "kernels": {kk: dict(launches=v["launches"] // args.steps, ms=round(1e3 * v["seconds"] / args.steps, 3))
+                    for kk, v in prof.items()},
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_side),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

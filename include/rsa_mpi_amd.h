@@ -104,6 +104,17 @@ int rsa_gpu_volume_of_spheres(rsa_gpu_ctx* ctx, double* volume);
  * coords is cap*dim int64 lattice coordinates, len the common voxel length. */
 int rsa_gpu_voxels(rsa_gpu_ctx* ctx, int64_t* coords, double* len, uint64_t cap, uint64_t* n);
 
+/* Per-kernel device timing, off by default (level 0).  Level 1 brackets the
+ * voxel-refine kernels, level 2 also the per-draw kernels, with HIP events on
+ * the context's stream, and accumulates their compulsory ("algorithmic") bytes.  No counterpart in the reference (its
+ * only timers are wall clocks, rsa_test_perf.hxx:21-25). */
+int rsa_gpu_profile_enable(rsa_gpu_ctx* ctx, int level);
+
+/* Entry k of the kernel profile: name, launches, summed seconds, summed
+ * algorithmic bytes.  Returns 2 when k is past the last entry. */
+int rsa_gpu_kernel_profile(rsa_gpu_ctx* ctx, int k, const char** name, int64_t* launches, double* seconds,
+                           double* bytes);
+
 /* Raw MT19937 words of std::mt19937 seeded with seed_seq{rank, seed}
  * (rsa_random.hxx:37-42), from the library's own generator. */
 int rsa_gpu_mt_words(uint64_t seed, uint64_t rank, uint64_t n, uint32_t* out);

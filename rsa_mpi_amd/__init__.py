@@ -100,6 +100,28 @@ class Domain:
         ph = np.ascontiguousarray(phases if phases is not None else np.zeros(len(r)), dtype=np.uint64)
         check(lib().rsa_gpu_add_spheres(self._ctx, len(r), dptr(c), dptr(r), ph.ctypes.data_as(_lib._U64)))
 
+    def profile(self, level=1):
+        """Enable (level 1: refine kernels, 2: all) and reset the per-kernel event timing; 0 disables."""
+        check(lib().rsa_gpu_profile_enable(self._ctx, int(level)))
+
+    def kernel_profile(self):
+        """{kernel: dict(launches, seconds, bytes)} accumulated since profile(True)."""
+        out = {}
+        k = 0
+        while True:
+            name = ctypes.c_char_p()
+            n = ctypes.c_int64()
+            sec = ctypes.c_double()
+            by = ctypes.c_double()
+            rc = lib().rsa_gpu_kernel_profile(self._ctx, k, ctypes.byref(name), ctypes.byref(n), ctypes.byref(sec),
+                                              ctypes.byref(by))
+            if rc == 2:
+                break
+            check(rc)
+            out[name.value.decode()] = dict(launches=n.value, seconds=sec.value, bytes=by.value)
+            k += 1
+        return out
+
     def voxels(self):
         """Current voxel list (int64 lattice coordinates) and the common voxel length."""
         n = ctypes.c_uint64()

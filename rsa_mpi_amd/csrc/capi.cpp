@@ -168,6 +168,24 @@ int rsa_gpu_voxels(rsa_gpu_ctx* ctx, int64_t* coords, double* len, uint64_t cap,
   return guarded([&] { *n = ctx->eng->voxels(coords, len, cap); });
 }
 
+int rsa_gpu_profile_enable(rsa_gpu_ctx* ctx, int on) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { ctx->eng->profile_enable(on); });
+}
+
+int rsa_gpu_kernel_profile(rsa_gpu_ctx* ctx, int k, const char** name, int64_t* launches, double* seconds,
+                           double* bytes) {
+  if (need_ctx(ctx)) return 1;
+  int found = 0;
+  int rc = guarded([&] { found = ctx->eng->profile_get(k, name, launches, seconds, bytes) ? 1 : 0; });
+  if (rc) return rc;
+  if (!found) {
+    g_err = "no such profile entry";
+    return 2;
+  }
+  return 0;
+}
+
 int rsa_gpu_mt_words(uint64_t seed, uint64_t rank, uint64_t n, uint32_t* out) {
   return guarded([&] {
     rsa_amd::MTStream s;

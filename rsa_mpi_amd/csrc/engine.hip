@@ -969,6 +969,8 @@ class Engine final : public IEngine {
     aimed_ = D <= 3 ? 0.95 : 1 - 0.05 * puissance<(D > 3 ? D - 3 : 0)>(0.1);
   }
   ~Engine() override {
+    if (pe0_) (void)hipEventDestroy(pe0_);
+    if (pe1_) (void)hipEventDestroy(pe1_);
     if (st_) (void)hipStreamDestroy(st_);
     if (hctr_) (void)hipHostFree(hctr_);
   }
@@ -1112,6 +1114,25 @@ class Engine final : public IEngine {
 
   void proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* out) override;
 
+  void profile_enable(int level) override {
+    const bool on = level > 0;
+    prof_ = on;
+    prof_level_ = level;
+    kprof_.clear();
+    if (on && !pe0_) {
+      HIP_OK(hipEventCreate(&pe0_));
+      HIP_OK(hipEventCreate(&pe1_));
+    }
+  }
+  bool profile_get(int k, const char** name, int64_t* launches, double* seconds, double* bytes) override {
+    if (k < 0 || k >= (int)kprof_.size()) return false;
+    *name = kprof_[k].name;
+    *launches = kprof_[k].launches;
+    *seconds = kprof_[k].seconds;
+    *bytes = kprof_[k].bytes;
+    return true;
+  }
+
  private:
   // ---------------------------------------------------------------- state
   static constexpr uint64_t kRing = 1ull << 23;  // words in the device ring (32 MiB)
@@ -1163,6 +1184,42 @@ class Engine final : public IEngine {
   uint64_t dev_hi_ = 0;  // words [0, dev_hi_) have been written to the ring
 
   CellStore<D> cs() { return CellStore<D>{cells_cnt_.p, cells_sph_.p, cells_ph_.p, cap_}; }
+
+  // ---------------------------------------------------------------- kernel profile
+  struct KProf {
+    const char* name;
+    int64_t launches = 0;
+    double seconds = 0, bytes = 0;
+  };
+  bool prof_ = false;
+  std::vector<KProf> kprof_;
+  hipEvent_t pe0_ = nullptr, pe1_ = nullptr;
+  // launch() runs the kernel launch; with profiling on it is bracketed by events
+  // on the context's stream and its algorithmic bytes are accumulated
+  int prof_level_ = 0;  // 1: refine kernels only, 2: every bracketed kernel
+  template <class F>
+  void timed(const char* name, double bytes, F&& launch, int level = 2) {
+    if (!prof_ || level > prof_level_) {
+      launch();
+      return;
+    }
+    HIP_OK(hipEventRecord(pe0_, st_));
+    launch();
+    HIP_OK(hipEventRecord(pe1_, st_));
+    HIP_OK(hipEventSynchronize(pe1_));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, pe0_, pe1_));
+    KProf* k = nullptr;
+    for (auto& e : kprof_)
+      if (std::strcmp(e.name, name) == 0) k = &e;
+    if (!k) {
+      kprof_.push_back(KProf{name});
+      k = &kprof_.back();
+    }
+    k->launches++;
+    k->seconds += ms * 1e-3;
+    k->bytes += bytes;
+  }
 
   void define_grid(double rad) {  // rsa_grid_traversal.ixx:11-73
     for (int d = 0; d < D; d++) {
@@ -1357,7 +1414,11 @@ class Engine final : public IEngine {
     holes_.reserve(nvox_);
     const VoxParams<D> vp = vparams(rmin);
     VoxView<D> vx = vview();
-    hipLaunchKernelGGL(k_cover<D>, dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, code_.p, blk_.p);
+    // compulsory traffic: lattice coordinates in, code out, every placed sphere and cell count once
+    const double cover_bytes = (double)nvox_ * (4.0 * D + 4.0) + (double)nb_fast_ * sizeof(Slot<D>) + geo_.ncells;
+    timed("k_cover", cover_bytes, [&] {
+      hipLaunchKernelGGL(k_cover<D>, dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, code_.p, blk_.p);
+    }, 1);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, blk_.p, (int)nb, dctr_.p + C_KEEP);
     hipLaunchKernelGGL(k_holes, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
     hipLaunchKernelGGL(k_move<D>, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p, vx);
@@ -1378,8 +1439,10 @@ class Engine final : public IEngine {
     for (int d = 0; d < D; d++) vc_[nxt][d].reserve(std::max<int64_t>(nchild, 1));
     int32_t* dst[4] = {nullptr, nullptr, nullptr, nullptr};
     for (int d = 0; d < D; d++) dst[d] = vc_[nxt][d].p;
-    hipLaunchKernelGGL(k_scatter<D>, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p, vx, dst[0], dst[1], dst[2],
-                       dst[3], vp);
+    timed("k_scatter", (double)keep * (4.0 * D + 4.0) + (double)nchild * 4.0 * D, [&] {
+      hipLaunchKernelGGL(k_scatter<D>, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p, vx, dst[0], dst[1],
+                         dst[2], dst[3], vp);
+    }, 1);
     HIP_OK(hipGetLastError());
     vcur_ = nxt;
     nvox_ = nchild;
@@ -1456,8 +1519,10 @@ int64_t Engine<D>::single_draw(int nshots, uint64_t maximal, bool may_outreach, 
     reset_ctr();
     const size_t shmem = sizeof(uint32_t) * ((nwin + 31) / 32 + 1);
     if (shmem > 60000) throw Error("word window too large for one workgroup");
-    hipLaunchKernelGGL(k_chain<D>, dim3(1), dim3(1024), shmem, st_, ring_.p, kRing - 1, w0, nwin, nshots, N, thr,
-                       vpos_.p, dctr_.p);
+    timed("k_chain", (double)nwin * 4.0 + nshots * 4.0, [&] {
+      hipLaunchKernelGGL(k_chain<D>, dim3(1), dim3(1024), shmem, st_, ring_.p, kRing - 1, w0, nwin, nshots, N, thr,
+                         vpos_.p, dctr_.p);
+    });
     read_ctr();
     if (!hctr_[C_CHAIN_OVF]) break;
     nwin *= 2;
@@ -1481,8 +1546,10 @@ int64_t Engine<D>::single_draw(int nshots, uint64_t maximal, bool may_outreach, 
     rounds++;
     // per-round counters
     HIP_OK(hipMemsetAsync(dctr_.p + C_CURSOR, 0, sizeof(int) * 4, st_));
-    hipLaunchKernelGGL(k_filter<D>, dim3(nbs), dim3(WG), 0, st_, geo_, cs(), nshots, cpos_.p, crad_.p, ccell_.p,
-                       ctag_.p, cell2t_.p, seg_.p, len_.p, ord_.p);
+    timed("k_filter", (double)nshots * (8.0 * (D + 1) + 5.0), [&] {
+      hipLaunchKernelGGL(k_filter<D>, dim3(nbs), dim3(WG), 0, st_, geo_, cs(), nshots, cpos_.p, crad_.p, ccell_.p,
+                         ctag_.p, cell2t_.p, seg_.p, len_.p, ord_.p);
+    });
     hipLaunchKernelGGL(k_list_remove, dim3(nbs), dim3(WG), 0, st_, 0, seg_.p, len_.p, ord_.p, ctag_.p, dctr_.p);
     for (;;) {
       HIP_OK(hipMemsetAsync(dctr_.p + C_CURSOR, 0, sizeof(int), st_));
@@ -1497,8 +1564,10 @@ int64_t Engine<D>::single_draw(int nshots, uint64_t maximal, bool may_outreach, 
       }
       break;
     }
-    hipLaunchKernelGGL(k_label, dim3(1), dim3(1024), 0, st_, nshots, ctag_.p, lab_.p, poff_.p, pcnt_.p, plist_.p,
-                       dctr_.p);
+    timed("k_label", (double)nshots * 10.0, [&] {
+      hipLaunchKernelGGL(k_label, dim3(1), dim3(1024), 0, st_, nshots, ctag_.p, lab_.p, poff_.p, pcnt_.p, plist_.p,
+                         dctr_.p);
+    });
     read_ctr();
     if (hctr_[C_LABEL_STUCK]) throw Error("internal: conflict labelling made no progress");
     const uint64_t nb_undecided = (uint64_t)hctr_[C_ACCEPTED];

@@ -32,6 +32,9 @@ class IEngine {
   virtual double volume_of_spheres() = 0;
   virtual uint64_t voxels(int64_t* coords, double* len, uint64_t cap) = 0;
   virtual double total_volume() const = 0;
+  // per-kernel device timings (HIP events on the context's stream)
+  virtual void profile_enable(int level) = 0;  // 0 off, 1 refine kernels, 2 all
+  virtual bool profile_get(int k, const char** name, int64_t* launches, double* seconds, double* bytes) = 0;
 };
 
 IEngine* make_engine(int dim, const double* inf, const double* sup, int ghost_layer, double domain_rad);

@@ -66,3 +66,25 @@ def test_small_3d_bit_exact(L, r, seed):
     o = orc_run(3, L, cls, seed=seed)
     assert_same(g, o)
     assert g["stats"]["words"] == o["words"]
+
+
+def test_cfg2_full_size_matches_golden():
+    """3D r=0.005 unit cube (~734k spheres): the whole packing against the
+    oracle's committed fixture, plus size-independent properties."""
+    import json
+    import os
+    from _common import periodic_overlaps, uncovered_samples, volume_fraction
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cfg2_3d_r0.005_seed0.json")))
+    g = gpu_run(3, 1.0, [(0.005, 1.0, 0)], seed=0)
+    st = g["stats"]
+    print(st)
+    assert len(g["rad"]) == gold["n"]
+    assert set_hash(g["pos"], g["rad"]) == gold["sha256"]
+    assert st["words"] == gold["words"]
+    assert st["draws"] == gold["draws"] and st["shots"] == gold["shots"] and st["refines"] == gold["refines"]
+    assert st["rounds"] == gold["rounds"]
+    assert st["nvox_max"] == gold["nvox_max"]
+    assert st["end_reason"] == 1 and st["nvox_final"] == 0
+    assert periodic_overlaps(g["pos"], g["rad"], [1.0] * 3) == 0
+    assert uncovered_samples(g["pos"], g["rad"], [1.0] * 3, 0.005, n=200000) == 0
+    assert 0.38 < volume_fraction(g["rad"], 3, 1.0) < 0.39
