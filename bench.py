@@ -83,6 +83,13 @@ def main():
                 a[f] += v[f]
     value = total_n / total_t
 
+    # one extra, untimed step with every per-draw kernel bracketed by events
+    dom2, _, dt2 = run_gpu_step(R, profile_level=2)
+    detail = {kk: dict(launches=v["launches"], ms=round(1e3 * v["seconds"], 3),
+                       us_per_launch=round(1e6 * v["seconds"] / max(1, v["launches"]), 2))
+              for kk, v in dom2.kernel_profile().items()}
+    del dom2
+
     parity = None
     gold = os.path.join(ROOT, "tests", "golden", "cfg2_3d_r0.005_seed0.json")
     if os.path.exists(gold):
@@ -111,6 +118,7 @@ def main():
         "roofline": roofline,
         "kernels": {kk: dict(launches=v["launches"] // args.steps, ms=round(1e3 * v["seconds"] / args.steps, 3))
                     for kk, v in prof.items()},
+        "kernels_all_bracketed": dict(step_ms=round(1e3 * dt2, 1), **detail),
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_side),
     }
     print(json.dumps(out), flush=True)

@@ -52,8 +52,16 @@ struct Geo {
   double hi_band_lo[D];  // sup - 2*rad : bottom of the band imaged to the lower ghost layer
   double L[D];           // global_sup - global_inf : periodic shift
   int nst;               // 3^D
-  signed char st[81][4]; // stencil offsets in the reference's visiting order
+  // stencil offsets in the reference's visiting order, one int per entry with
+  // (offset_d + 1) in bits 2d..2d+1; device memory, so that a lane-dependent
+  // entry is a cached load rather than a private copy of this struct
+  const int* stab;
 };
+
+template <int D>
+__device__ __host__ __forceinline__ int stencil_off(int packed, int d) {
+  return ((packed >> (2 * d)) & 3) - 1;
+}
 
 template <int D>
 struct CellStore {
@@ -65,7 +73,7 @@ struct CellStore {
 
 // compute_cell_idx (rsa_grid_traversal.ixx:373-382)
 template <int D>
-__device__ __host__ inline int64_t cell_index(const Geo<D>& g, const double* p) {
+__device__ __host__ __forceinline__ int64_t cell_index(const Geo<D>& g, const double* p) {
   uint64_t ret = 0;
   for (int d = D - 1; d >= 0; d--) {
     const int64_t ijk = (int64_t)((p[d] - g.cshift[d]) * g.cinv[d]);
@@ -75,7 +83,7 @@ __device__ __host__ inline int64_t cell_index(const Geo<D>& g, const double* p) 
 }
 
 template <int D>
-__device__ __host__ inline void cell_coords(const Geo<D>& g, int64_t idx, int* c) {
+__device__ __host__ __forceinline__ void cell_coords(const Geo<D>& g, int64_t idx, int* c) {
   for (int d = 0; d < D; d++) {
     c[d] = (int)(idx % g.n[d]);
     idx /= g.n[d];
@@ -83,7 +91,7 @@ __device__ __host__ inline void cell_coords(const Geo<D>& g, int64_t idx, int* c
 }
 
 template <int D>
-__device__ __host__ inline bool coords_ghost(const Geo<D>& g, const int* c) {
+__device__ __host__ __forceinline__ bool coords_ghost(const Geo<D>& g, const int* c) {
   for (int d = 0; d < D; d++)
     if (c[d] < g.gl || c[d] > g.n[d] - 1 - g.gl) return true;
   return false;
@@ -102,11 +110,13 @@ struct Nb {
 };
 
 template <int D>
-__device__ __host__ inline bool neighbour(const Geo<D>& g, const int* c, int s, Nb<D>& nb) {
+__device__ __forceinline__ bool neighbour(const Geo<D>& g, const int* c, int s, Nb<D>& nb) {
   int64_t idx = 0;
   nb.ghost = false;
+  const int packed = g.stab[s];
+#pragma unroll
   for (int d = D - 1; d >= 0; d--) {
-    int k = c[d] + g.st[s][d];
+    int k = c[d] + stencil_off<D>(packed, d);
     signed char m = 0;
     const int inner = g.n[d] - 2 * g.gl;
     if (k < g.gl) {
@@ -130,33 +140,25 @@ __device__ __host__ inline bool neighbour(const Geo<D>& g, const int* c, int s, 
 // (helper_check_interval over the area's closed box, helper.ixx:26-34) and its
 // shifted image (rsa_ghost_area.ixx:22-30).
 template <int D>
-__device__ __host__ inline bool image_of(const Geo<D>& g, const Nb<D>& nb, const double* p, double* q) {
+__device__ __host__ __forceinline__ bool image_of(const Geo<D>& g, const Nb<D>& nb, const double* p, double* q) {
+  bool in = true;
+#pragma unroll
   for (int d = 0; d < D; d++) {
-    double lo, hi;
-    if (nb.m[d] == -1) {  // area at relative position +1: [sup - 2 rad, sup]
-      lo = g.hi_band_lo[d];
-      hi = g.sup[d];
-    } else if (nb.m[d] == 1) {  // relative position -1: [inf, inf + 2 rad]
-      lo = g.inf[d];
-      hi = g.lo_band_hi[d];
-    } else {
-      lo = g.inf[d];
-      hi = g.sup[d];
-    }
-    if (p[d] < lo) return false;
-    if (p[d] > hi) return false;
+    // values first, then selects: a selected address would force the struct into scratch
+    const double inf = g.inf[d], sup = g.sup[d], lbh = g.lo_band_hi[d], hbl = g.hi_band_lo[d], L = g.L[d];
+    const int m = nb.m[d];
+    const double lo = m == -1 ? hbl : inf;  // m == -1: area at relative position +1, [sup - 2 rad, sup]
+    const double hi = m == 1 ? lbh : sup;   // m == +1: relative position -1, [inf, inf + 2 rad]
+    in = in && !(p[d] < lo) && !(p[d] > hi);
+    const double sh = m == -1 ? -L : L;
+    q[d] = (m != 0 && fabs(sh) >= 1E-16) ? p[d] + sh : p[d];
   }
-  for (int d = 0; d < D; d++) {
-    const double sh = nb.m[d] == -1 ? -g.L[d] : (nb.m[d] == 1 ? g.L[d] : 0.0);
-    q[d] = p[d];
-    if (nb.m[d] != 0 && fabs(sh) >= 1E-16) q[d] = p[d] + sh;
-  }
-  return true;
+  return in;
 }
 
 // is_there_collision (operator_cells_and_data_storage.ixx:5-14)
 template <int D>
-__device__ __host__ inline bool collide(double r1, const double* p1, double r2, const double* p2) {
+__device__ __host__ __forceinline__ bool collide(double r1, const double* p1, double r2, const double* p2) {
   double s = 0;
   for (int d = 0; d < D; d++) {
     const double dr = p1[d] - p2[d];
@@ -167,7 +169,7 @@ __device__ __host__ inline bool collide(double r1, const double* p1, double r2, 
 
 // libstdc++ generate_canonical<double, 53> from two MT words (random.tcc:3346-3379):
 // (g1 + g2 * 2^32) / 2^64, values >= 1 replaced by the largest double below 1.
-__device__ __host__ inline double canonical(uint32_t g1, uint32_t g2) {
+__device__ __host__ __forceinline__ double canonical(uint32_t g1, uint32_t g2) {
   double sum = (double)g1;
   sum += (double)g2 * 4294967296.0;
   double r = sum * 0x1p-64;  // exact division by 2^64

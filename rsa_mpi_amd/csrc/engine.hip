@@ -18,7 +18,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -89,7 +92,7 @@ enum Ctr : int {
 
 // ============================================================== kernels: word stream
 
-__device__ inline uint32_t ring_at(const uint32_t* ring, uint64_t mask, uint64_t pos) { return ring[pos & mask]; }
+__device__ __forceinline__ uint32_t ring_at(const uint32_t* ring, uint64_t mask, uint64_t pos) { return ring[pos & mask]; }
 
 // Voxel-index words drawn by uniform_int_distribution<int>(0, N-1) use Lemire's
 // multiply-shift with rejection of low 32 bits below (2^32 - N) mod N
@@ -288,7 +291,7 @@ __global__ void k_bin_sort(const int* __restrict__ seg, const int* __restrict__ 
 // ============================================================== kernels: conflict rounds
 
 template <int D>
-__device__ inline void cand_pos(const double* cpos, int n, int i, double* p) {
+__device__ __forceinline__ void cand_pos(const double* cpos, int n, int i, double* p) {
   for (int d = 0; d < D; d++) p[d] = cpos[(size_t)d * n + i];
 }
 
@@ -374,10 +377,10 @@ __global__ void k_list_remove(int mode, const int* __restrict__ seg, int* __rest
 // priority that collides with it, in stencil order then in-cell order; a
 // partner seen through a ghost neighbour is its periodic image (bit 31 set).
 template <int D>
-__device__ inline int scan_partners(const Geo<D>& g, int nshots, int i, const double* cpos, const double* crad,
+__device__ __forceinline__ int scan_partners(const Geo<D>& g, int nshots, int i, const double* cpos, const double* crad,
                                     const uint32_t* cprio, const int* ccell, const uint8_t* ctag,
                                     const int* cell2t, const int* seg, const int* len, const int* ord, int* out,
-                                    int* ctr) {
+                                    int* mismatch) {
   double p[D];
   cand_pos<D>(cpos, nshots, i, p);
   const double r = crad[i];
@@ -404,8 +407,8 @@ __device__ inline int scan_partners(const Geo<D>& g, int nshots, int i, const do
         if (out == nullptr) {
           int64_t ic = cell_index<D>(g, qp);
           int64_t want = 0;
-          for (int d = D - 1; d >= 0; d--) want = want * g.n[d] + (cc[d] + g.st[s][d]);
-          if (ic != want) atomicAdd(&ctr[C_IMG_MISMATCH], 1);
+          for (int d = D - 1; d >= 0; d--) want = want * g.n[d] + (cc[d] + stencil_off<D>(g.stab[s], d));
+          if (ic != want) atomicAdd(mismatch, 1);
         }
       } else {
         for (int d = 0; d < D; d++) qp[d] = jp[d];
@@ -432,7 +435,8 @@ __global__ void k_partners(Geo<D> g, int nshots, const double* __restrict__ cpos
     pcnt[i] = 0;
     return;
   }
-  const int n = scan_partners<D>(g, nshots, i, cpos, crad, cprio, ccell, ctag, cell2t, seg, len, ord, nullptr, ctr);
+  const int n = scan_partners<D>(g, nshots, i, cpos, crad, cprio, ccell, ctag, cell2t, seg, len, ord, nullptr,
+                                 &ctr[C_IMG_MISMATCH]);
   pcnt[i] = n;
   if (n == 0) return;
   const int off = atomicAdd(&ctr[C_CURSOR], n);
@@ -441,7 +445,8 @@ __global__ void k_partners(Geo<D> g, int nshots, const double* __restrict__ cpos
     atomicMax(&ctr[C_OVERFLOW], 2);
     return;
   }
-  scan_partners<D>(g, nshots, i, cpos, crad, cprio, ccell, ctag, cell2t, seg, len, ord, plist + off, ctr);
+  scan_partners<D>(g, nshots, i, cpos, crad, cprio, ccell, ctag, cell2t, seg, len, ord, plist + off,
+                   &ctr[C_IMG_MISMATCH]);
 }
 
 // The sequential priority sweep as a dependency evaluation: a candidate's
@@ -601,6 +606,573 @@ __global__ void k_commit(CellStore<D> cs, int nshots, const double* __restrict__
   cellcc[c] = 0;
 }
 
+// ============================================================== kernels: one draw, one host round trip
+//
+// A draw is five kernels: decode + validated filter + cell binning
+// (k_decode_filter), per-cell ordered lists (k_lists), partner lists
+// (k_partner_lists), sweep labels + publication to the host (k_sweep_labels, one
+// workgroup), tag update + commit (k_update_commit).  Gathers over the 3^D
+// stencil are spread over a lane group per candidate (one lane per stencil
+// cell) so the loads of all neighbour cells are in flight together.  No kernel
+// uses a device-wide fence or a counter shared by all lanes: touched cells are
+// identified by their first candidate ("owner"), and partner lists live in a
+// fixed slab per candidate.
+
+template <int D>
+struct LaneGroup {
+  static constexpr int NST = ipow_c(3, D);
+  static constexpr int LPC = D == 2 ? 16 : (D == 3 ? 32 : 64);  // lanes per candidate
+  static constexpr int CPL = (NST + LPC - 1) / LPC;              // stencil cells per lane
+  __device__ static inline uint64_t mask(int lane) {
+    if (LPC == 64) return ~0ull;
+    return ((1ull << LPC) - 1ull) << ((lane / LPC) * LPC);
+  }
+};
+
+constexpr int CSLOT = 8;  // candidates of a cell kept in its fixed slots; more go to an overflow list
+constexpr int KMAX = 16;  // partners kept in a candidate's slab; longer lists go to a pool
+
+// Control block of the fast draw path (device side).
+struct DrawCtl {
+  int dropped;
+  int img_mismatch;
+  int seg_cursor;
+  int pool_cursor;
+  int pool_overflow;
+  int need_pool;
+  int acc, con;
+  int label_stuck;
+  int cell_overflow;
+};
+// What k_label publishes to pinned host memory (seq last).
+struct DrawPub {
+  int acc, con, label_stuck, dropped, img_mismatch, pool_overflow, need_pool, cell_overflow;
+  int seq;
+};
+
+// pick_points + generate_priority + constant-radius class (list_of_voxels.ixx:107-129,
+// operator_algorithm.ixx:373-381, radius_generator.ixx:50-60); build_grid's
+// cell (rsa_grid.ixx:182-187); round-0 remove_collisions_with_validated_spheres
+// (operator_cells_and_data_storage.ixx:54-76: at this point only placed spheres
+// and their images are validated); each candidate takes an arrival slot in its
+// cell (shot order is restored in k_lists).
+template <int D>
+__global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
+                                                      int nshots, uint32_t N, const uint32_t* __restrict__ vpos,
+                                                      int end_rel, VoxView<D> vx, ClassDev cls, Geo<D> g,
+                                                      CellStore<D> cs, double* __restrict__ cpos,
+                                                      double* __restrict__ crad, uint32_t* __restrict__ cprio,
+                                                      uint32_t* __restrict__ cphase, int* __restrict__ ccell,
+                                                      uint8_t* __restrict__ ctag, uint8_t* __restrict__ owner,
+                                                      int* __restrict__ cellcc, int* __restrict__ cslot,
+                                                      int* __restrict__ ovhead, int* __restrict__ next,
+                                                      DrawCtl* __restrict__ ctl) {
+  using LG = LaneGroup<D>;
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = gtid / LG::LPC;
+  const int sub = gtid % LG::LPC;
+  const bool valid = i < nshots;
+  double p[D];
+  int64_t cell = -1;
+  bool ok = false;
+  if (valid) {
+    const uint32_t v = vpos[i];
+    const uint32_t gw = ring_at(ring, mask, w0 + v);
+    const uint32_t id = (uint32_t)(((uint64_t)gw * (uint64_t)N) >> 32);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const double o = vx.origin[d] + (double)vx.c[d][id] * vx.len[d];
+      const double u = canonical(ring_at(ring, mask, w0 + v + 1 + 2 * d), ring_at(ring, mask, w0 + v + 2 + 2 * d));
+      p[d] = o + vx.len[d] * u;
+    }
+    cell = cell_index<D>(g, p);
+    ok = cell >= 0 && cell < g.ncells;
+    if (ok) {
+      int cc[D];
+      cell_coords<D>(g, cell, cc);
+      ok = !coords_ghost<D>(g, cc);
+    }
+  }
+  bool hit = false;
+  if (valid && ok) {
+    const double r = cls.radius;
+    int cc[D];
+    cell_coords<D>(g, cell, cc);
+#pragma unroll
+    for (int jj = 0; jj < LG::CPL; jj++) {
+      const int s = sub + jj * LG::LPC;
+      if (s >= LG::NST || hit) continue;
+      Nb<D> nb;
+      if (!neighbour<D>(g, cc, s, nb)) continue;
+      const int nv = cs.cnt[nb.src];
+      const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
+      for (int k = 0; k < nv && !hit; k++) {
+        const Slot<D> q = sl[k];
+        double qp[D];
+        if (nb.ghost) {
+          if (!image_of<D>(g, nb, q.c, qp)) continue;
+        } else {
+#pragma unroll
+          for (int d = 0; d < D; d++) qp[d] = q.c[d];
+        }
+        if (collide<D>(r, p, q.r, qp)) hit = true;
+      }
+    }
+  }
+  const bool anyhit = (__ballot(hit) & LG::mask(lane)) != 0;
+  if (valid && sub == 0) {
+#pragma unroll
+    for (int d = 0; d < D; d++) cpos[(size_t)d * nshots + i] = p[d];
+    cprio[i] = ring_at(ring, mask, w0 + (uint64_t)end_rel + i) >> 1;
+    crad[i] = cls.radius;
+    cphase[i] = cls.phase;
+    if (!ok) {
+      ccell[i] = -1;
+      ctag[i] = T_DROPPED;
+      owner[i] = 0;
+      atomicAdd(&ctl->dropped, 1);
+    } else {
+      ccell[i] = (int)cell;
+      ctag[i] = anyhit ? T_SKIP : T_UNDECIDED;
+      const int a = atomicAdd(&cellcc[cell], 1);
+      if (a < CSLOT)
+        cslot[(size_t)cell * CSLOT + a] = i;
+      else
+        next[i] = atomicExch(&ovhead[cell], i);
+      owner[i] = a == 0;
+    }
+  }
+}
+
+// One thread per touched cell (its owner candidate): the cell's candidates in
+// shot order (build_grid appends in candidate order,
+// operator_cells_and_data_storage.ixx:153-159), then the round-0 Skip removal
+// from the back with swap-with-last (rsa_data_storage.ixx:108-116, 305-315).
+__global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict__ ccell,
+                                              const uint8_t* __restrict__ owner, const uint8_t* __restrict__ ctag,
+                                              int* __restrict__ cellcc, const int* __restrict__ cslot,
+                                              int* __restrict__ ovhead, const int* __restrict__ next,
+                                              int* __restrict__ seg, int* __restrict__ len, int* __restrict__ ord,
+                                              int* __restrict__ cell2t, DrawCtl* __restrict__ ctl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = i < nshots && owner[i];
+  const int c = on ? ccell[i] : 0;
+  const int n = on ? cellcc[c] : 0;
+  // segment allocation: block prefix + one atomic per block
+  __shared__ int s_w[WG / 64];
+  __shared__ int s_base;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = n;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += u;
+  }
+  if (lane == 63) s_w[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int k = 0; k < WG / 64; k++) {
+      const int v = s_w[k];
+      s_w[k] = tot;
+      tot += v;
+    }
+    s_base = tot ? atomicAdd(&ctl->seg_cursor, tot) : 0;
+  }
+  __syncthreads();
+  if (!on) return;
+  const int base = s_base + s_w[w] + incl - n;
+  int* a = ord + base;
+  const int nf = n < CSLOT ? n : CSLOT;
+  for (int k = 0; k < nf; k++) a[k] = cslot[(size_t)c * CSLOT + k];
+  int k = nf;
+  for (int j = ovhead[c]; j != -1 && k < n; j = next[j]) a[k++] = j;
+  // shot order (Shell sort: lists are short except in clustered late draws)
+  for (int gap = n / 2; gap > 0; gap /= 2)
+    for (int q = gap; q < n; q++) {
+      const int v = a[q];
+      int j = q;
+      while (j >= gap && a[j - gap] > v) {
+        a[j] = a[j - gap];
+        j -= gap;
+      }
+      a[j] = v;
+    }
+  int m = n;
+  for (int q = m - 1; q >= 0; q--) {
+    if (ctag[a[q]] == T_SKIP) {
+      a[q] = a[m - 1];
+      m--;
+    }
+  }
+  seg[i] = base;
+  len[i] = m;
+  cell2t[c] = i;
+  cellcc[c] = 0;
+  ovhead[c] = -1;
+}
+
+// Rounds >= 1 filter (operator_cells_and_data_storage.ixx:54-76): placed
+// spheres, spheres validated by earlier rounds of this draw, and their images.
+template <int D>
+__global__ void __launch_bounds__(WG) k_refilter(Geo<D> g, CellStore<D> cs, int nshots,
+                                                 const double* __restrict__ cpos, const double* __restrict__ crad,
+                                                 const int* __restrict__ ccell, uint8_t* __restrict__ ctag,
+                                                 const int* __restrict__ cell2t, const int* __restrict__ seg,
+                                                 const int* __restrict__ len, const int* __restrict__ ord) {
+  using LG = LaneGroup<D>;
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = gtid / LG::LPC;
+  const int sub = gtid % LG::LPC;
+  const bool active = i < nshots && ctag[i] == T_UNDECIDED;
+  bool hit = false;
+  if (active) {
+    double p[D];
+    cand_pos<D>(cpos, nshots, i, p);
+    const double r = crad[i];
+    int cc[D];
+    cell_coords<D>(g, ccell[i], cc);
+#pragma unroll
+    for (int jj = 0; jj < LG::CPL; jj++) {
+      const int s = sub + jj * LG::LPC;
+      if (s >= LG::NST || hit) continue;
+      Nb<D> nb;
+      if (!neighbour<D>(g, cc, s, nb)) continue;
+      const int nv = cs.cnt[nb.src];
+      const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
+      for (int k = 0; k < nv && !hit; k++) {
+        const Slot<D> q = sl[k];
+        double qp[D];
+        if (nb.ghost) {
+          if (!image_of<D>(g, nb, q.c, qp)) continue;
+        } else {
+#pragma unroll
+          for (int d = 0; d < D; d++) qp[d] = q.c[d];
+        }
+        if (collide<D>(r, p, q.r, qp)) hit = true;
+      }
+      const int t = cell2t[nb.src];
+      if (t < 0 || hit) continue;
+      const int* o = ord + seg[t];
+      const int ln = len[t];
+      for (int k = 0; k < ln && !hit; k++) {
+        const int j = o[k];
+        if (ctag[j] != T_VALIDATED) continue;
+        double jp[D], qp[D];
+        cand_pos<D>(cpos, nshots, j, jp);
+        if (nb.ghost) {
+          if (!image_of<D>(g, nb, jp, qp)) continue;
+        } else {
+#pragma unroll
+          for (int d = 0; d < D; d++) qp[d] = jp[d];
+        }
+        if (collide<D>(r, p, crad[j], qp)) hit = true;
+      }
+    }
+  }
+  const bool anyhit = (__ballot(hit) & LG::mask(lane)) != 0;
+  if (active && sub == 0 && anyhit) ctag[i] = T_SKIP;
+}
+
+__global__ void k_remove_skips(int nshots, const uint8_t* __restrict__ owner, const int* __restrict__ seg,
+                               int* __restrict__ len, int* __restrict__ ord, const uint8_t* __restrict__ ctag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots || !owner[i]) return;
+  int* a = ord + seg[i];
+  int n = len[i];
+  for (int q = n - 1; q >= 0; q--) {
+    if (ctag[a[q]] == T_SKIP) {
+      a[q] = a[n - 1];
+      n--;
+    }
+  }
+  len[i] = n;
+}
+
+// Partners of one candidate held by one lane's stencil cells, in stencil then
+// in-cell order (find_solve_conflicts' detection, operator_cells_and_data_storage.ixx:16-52,
+// 127-142): Undecided candidates of strictly smaller priority that collide;
+// through a ghost neighbour the partner is its periodic image (bit 31 set).
+template <int D>
+__device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int sub, int nshots, const double* p,
+                                             double r, uint32_t pi, const double* cpos, const double* crad,
+                                             const uint32_t* cprio, const uint8_t* ctag, const int* cell2t,
+                                             const int* seg, const int* len, const int* ord, int* out,
+                                             bool& first_ghost, int* mismatch) {
+  using LG = LaneGroup<D>;
+  int n = 0;
+  first_ghost = false;
+#pragma unroll
+  for (int jj = 0; jj < LG::CPL; jj++) {
+    const int s = sub + jj * LG::LPC;
+    if (s >= LG::NST) continue;
+    Nb<D> nb;
+    if (!neighbour<D>(g, cc, s, nb)) continue;
+    const int t = cell2t[nb.src];
+    if (t < 0) continue;
+    const int* o = ord + seg[t];
+    const int ln = len[t];
+    for (int k = 0; k < ln; k++) {
+      const int j = o[k];
+      if (ctag[j] != T_UNDECIDED) continue;
+      if (!(cprio[j] < pi)) continue;
+      double jp[D], qp[D];
+      cand_pos<D>(cpos, nshots, j, jp);
+      if (nb.ghost) {
+        if (!image_of<D>(g, nb, jp, qp)) continue;
+        if (out == nullptr && mismatch) {  // the image must fall in this ghost cell
+          const int64_t ic = cell_index<D>(g, qp);
+          int64_t want = 0;
+          for (int d = D - 1; d >= 0; d--) want = want * g.n[d] + (cc[d] + stencil_off<D>(g.stab[s], d));
+          if (ic != want) atomicAdd(mismatch, 1);
+        }
+      } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) qp[d] = jp[d];
+      }
+      if (collide<D>(r, p, crad[j], qp)) {
+        if (n == 0) first_ghost = nb.ghost;
+        if (out) out[n] = j | (nb.ghost ? (int)0x80000000 : 0);
+        n++;
+      }
+    }
+  }
+  return n;
+}
+
+// Sweep labels, first step (see k_sweep_labels for the semantics): candidates without
+// partners stay Undecided, those whose first partner is an image become
+// Conflict; the others get their ordered partner list (slab of KMAX entries,
+// longer lists from a pool) and are left unresolved for k_sweep_labels.
+template <int D>
+__global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, const double* __restrict__ cpos,
+                                                 const double* __restrict__ crad, const uint32_t* __restrict__ cprio,
+                                                 const int* __restrict__ ccell, const uint8_t* __restrict__ ctag,
+                                                 uint8_t* __restrict__ lab, const int* __restrict__ cell2t,
+                                                 const int* __restrict__ seg, const int* __restrict__ len,
+                                                 const int* __restrict__ ord, int* __restrict__ poff,
+                                                 int* __restrict__ pcnt, int* __restrict__ slab,
+                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl) {
+  using LG = LaneGroup<D>;
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = gtid / LG::LPC;
+  const int sub = gtid % LG::LPC;
+  const bool active = i < nshots && ctag[i] == T_UNDECIDED;
+  int n = 0;
+  bool fg = false;
+  double p[D];
+  double r = 0;
+  uint32_t pi = 0;
+  int cc[D];
+  if (active) {
+    cand_pos<D>(cpos, nshots, i, p);
+    r = crad[i];
+    pi = cprio[i];
+    cell_coords<D>(g, ccell[i], cc);
+    n = lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, nullptr, fg,
+                         &ctl->img_mismatch);
+  }
+  int incl = n;  // inclusive scan over the lane group (stencil order)
+#pragma unroll
+  for (int off = 1; off < LG::LPC; off <<= 1) {
+    const int v = __shfl_up(incl, off, LG::LPC);
+    if (sub >= off) incl += v;
+  }
+  const int total = __shfl(incl, LG::LPC - 1, LG::LPC);
+  const uint64_t has = __ballot(n > 0) & LG::mask(lane);
+  const int first_lane = has ? (__ffsll((unsigned long long)has) - 1) : 0;
+  const int fgi = __shfl(fg ? 1 : 0, first_lane, 64);
+  const bool pend = active && total > 0 && !fgi;
+  int* dst = nullptr;
+  int off = -2;
+  if (active && sub == 0) {
+    if (total == 0) {
+      lab[i] = T_UNDECIDED;
+    } else if (fgi) {
+      lab[i] = T_CONFLICT;
+    } else {
+      lab[i] = T_UNRESOLVED;
+      pcnt[i] = total;
+      if (total <= KMAX) {
+        off = -1;
+      } else {
+        off = atomicAdd(&ctl->pool_cursor, total);
+        if (off + total > pool_cap) {
+          atomicMax(&ctl->pool_overflow, 1);
+          atomicMax(&ctl->need_pool, off + total);
+          off = -2;
+        }
+      }
+      poff[i] = off;
+    }
+  }
+  off = __shfl(off, (lane / LG::LPC) * LG::LPC, 64);
+  if (pend && off != -2) {
+    dst = off == -1 ? slab + (size_t)i * KMAX : pool + off;
+    if (n > 0) {
+      bool dummy;
+      lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord,
+                       dst + (incl - n), dummy, nullptr);
+    }
+  }
+}
+
+// The sequential priority sweep (find_solve_conflicts,
+// operator_cells_and_data_storage.ixx:127-142) as a dependency evaluation: a
+// candidate's label is fixed by its first partner (detection order) that is not
+// Skip -- an image or a Conflict partner gives Conflict, an Undecided real
+// partner gives Skip, none leaves it Undecided.  Partners have strictly smaller
+// priorities, so in the reference's sorted sweep they are labelled first; the
+// passes reach the same labels.  One workgroup, labels in LDS (nshots bytes of
+// dynamic shared memory).  The round's counts are then published to pinned
+// host memory.
+__global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
+                                                const int* __restrict__ poff, const int* __restrict__ pcnt,
+                                                const int* __restrict__ slab, const int* __restrict__ pool,
+                                                DrawCtl* ctl, DrawPub* host, int seq) {
+  extern __shared__ uint8_t s_lab[];
+  __shared__ int s_prog, s_left, s_acc, s_con, s_stuck;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_acc = 0;
+    s_con = 0;
+    s_stuck = 0;
+  }
+  const bool ovf = ctl->pool_overflow != 0;
+  for (int q = tid; q < nshots; q += blockDim.x) s_lab[q] = ctag[q] == T_UNDECIDED ? lab[q] : T_DROPPED;
+  __syncthreads();
+  if (!ovf) {
+    for (int pass = 0; pass < nshots + 2; pass++) {
+      if (tid == 0) {
+        s_prog = 0;
+        s_left = 0;
+      }
+      __syncthreads();
+      int prog = 0, left = 0;
+      for (int q = tid; q < nshots; q += blockDim.x) {
+        if (s_lab[q] != T_UNRESOLVED) continue;
+        const int m = pcnt[q];
+        const int o = poff[q];
+        const int* pl = o == -1 ? slab + (size_t)q * KMAX : pool + o;
+        uint8_t res = T_UNDECIDED;
+        bool wait = false;
+        for (int k = 0; k < m; k++) {
+          const int e = pl[k];
+          if (e < 0) {
+            res = T_CONFLICT;
+            break;
+          }
+          const uint8_t lj = s_lab[e];
+          if (lj == T_UNRESOLVED) {
+            wait = true;
+            break;
+          }
+          if (lj == T_SKIP) continue;
+          res = (lj == T_CONFLICT) ? T_CONFLICT : T_SKIP;
+          break;
+        }
+        if (wait) {
+          left++;
+        } else {
+          s_lab[q] = res;
+          lab[q] = res;
+          prog++;
+        }
+      }
+      if (prog) atomicAdd(&s_prog, prog);
+      if (left) atomicAdd(&s_left, left);
+      __syncthreads();
+      const int sp = s_prog, sl = s_left;
+      __syncthreads();
+      if (sl == 0) break;
+      if (sp == 0) {
+        if (tid == 0) s_stuck = 1;
+        break;
+      }
+    }
+    int acc = 0, con = 0;
+    for (int q = tid; q < nshots; q += blockDim.x) {
+      acc += s_lab[q] == T_UNDECIDED;
+      con += s_lab[q] == T_CONFLICT;
+    }
+    if (acc) atomicAdd(&s_acc, acc);
+    if (con) atomicAdd(&s_con, con);
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  ctl->acc = s_acc;
+  ctl->con = s_con;
+  ctl->label_stuck = s_stuck;
+  const int v[8] = {s_acc, s_con, s_stuck, ctl->dropped, ctl->img_mismatch, ctl->pool_overflow, ctl->need_pool,
+                    ctl->cell_overflow};
+  for (int k = 0; k < 8; k++) __atomic_store_n(&reinterpret_cast<int*>(host)[k], v[k], __ATOMIC_RELAXED);
+  __threadfence_system();
+  __atomic_store_n(&host->seq, seq, __ATOMIC_RELEASE);
+  // counters whose values are now published
+  ctl->dropped = 0;
+  ctl->img_mismatch = 0;
+  ctl->seg_cursor = 0;
+  if (!ovf) ctl->pool_cursor = 0;
+}
+
+// update_sphere_tags on the touched real cells (operator_algorithm.ixx:446-473),
+// one thread per cell owner: Skip removed from the back, Undecided ->
+// Validated, Conflict -> Undecided.  When the round left no Conflict the draw
+// is over and the survivors are appended to their cells in tail order.
+template <int D>
+__global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshots, const double* __restrict__ cpos,
+                                                      const double* __restrict__ crad,
+                                                      const uint32_t* __restrict__ cphase,
+                                                      const int* __restrict__ ccell, const uint8_t* __restrict__ owner,
+                                                      uint8_t* __restrict__ ctag, const uint8_t* __restrict__ lab,
+                                                      const int* __restrict__ seg, int* __restrict__ len,
+                                                      int* __restrict__ ord, int* __restrict__ cell2t,
+                                                      DrawCtl* __restrict__ ctl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshots || !owner[i]) return;
+  if (ctl->pool_overflow || ctl->label_stuck) return;
+  const bool commit = ctl->con == 0;
+  int* a = ord + seg[i];
+  int n = len[i];
+  for (int q = n - 1; q >= 0; q--) {
+    const int j = a[q];
+    const uint8_t e = ctag[j] == T_UNDECIDED ? lab[j] : ctag[j];
+    if (e == T_SKIP) {
+      ctag[j] = T_SKIP;
+      a[q] = a[n - 1];
+      n--;
+    }
+  }
+  len[i] = n;
+  for (int q = 0; q < n; q++) {
+    const int j = a[q];
+    if (ctag[j] == T_UNDECIDED && lab[j] == T_UNDECIDED) ctag[j] = T_VALIDATED;
+    // Conflict candidates keep the Undecided tag for the next round
+  }
+  if (!commit) return;
+  const int c = ccell[i];
+  int k0 = cs.cnt[c];
+  for (int q = 0; q < n; q++) {
+    const int j = a[q];
+    if (k0 >= cs.cap) {
+      atomicMax(&ctl->cell_overflow, 1);
+      break;
+    }
+    Slot<D> sl;
+#pragma unroll
+    for (int d = 0; d < D; d++) sl.c[d] = cpos[(size_t)d * nshots + j];
+    sl.r = crad[j];
+    cs.sph[(size_t)c * cs.cap + k0] = sl;
+    cs.phase[(size_t)c * cs.cap + k0] = cphase[j];
+    k0++;
+  }
+  cs.cnt[c] = (uint8_t)k0;
+  cell2t[c] = -1;
+}
+
 // ============================================================== kernels: voxel refine
 
 template <int D>
@@ -615,7 +1187,7 @@ struct VoxParams {
 
 // auxi::is_in_sphere (list_of_voxels.ixx:225-238)
 template <int D>
-__device__ inline bool in_sphere(const double (*corners)[D], const double* delta, double r, double rmin) {
+__device__ __forceinline__ bool in_sphere(const double (*corners)[D], const double* delta, double r, double rmin) {
   constexpr int NCH = ipow_c(2, D);
   const double R2 = (r + rmin) * (r + rmin);
 #pragma unroll
@@ -745,7 +1317,7 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(int* __restrict__ blk, int
 }
 
 // exclusive rank of element i among flagged elements: block offset + in-block prefix
-__device__ inline int block_rank(bool f, int blkoff) {
+__device__ __forceinline__ int block_rank(bool f, int blkoff) {
   __shared__ int wsum[WG / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t b = __ballot(f);
@@ -966,9 +1538,15 @@ class Engine final : public IEngine {
     HIP_OK(hipHostMalloc((void**)&hctr_, sizeof(int) * C_NUM, hipHostMallocDefault));
     dctr_.reserve(C_NUM);
     ring_.reserve(kRing);
+    HIP_OK(hipHostMalloc((void**)&hs_.buf, sizeof(uint32_t) * kRing, hipHostMallocDefault));
     aimed_ = D <= 3 ? 0.95 : 1 - 0.05 * puissance<(D > 3 ? D - 3 : 0)>(0.1);
   }
   ~Engine() override {
+    hs_.halt();
+    for (auto* h : hvpos_)
+      if (h) (void)hipHostFree(h);
+    if (hs_.buf) (void)hipHostFree(hs_.buf);
+    if (hpub_) (void)hipHostFree(hpub_);
     if (pe0_) (void)hipEventDestroy(pe0_);
     if (pe1_) (void)hipEventDestroy(pe1_);
     if (st_) (void)hipStreamDestroy(st_);
@@ -1178,10 +1756,67 @@ class Engine final : public IEngine {
   DevBuf<uint32_t> code_;
   DevBuf<int> blk_, holes_;
 
-  // word stream
-  MTStream hs_;
+  // word stream: generated on the host into a pinned ring (the MT19937
+  // recurrence is sequential), copied asynchronously into the device ring
+  struct HostRing {
+    MT19937 mt;
+    uint32_t* buf = nullptr;
+    std::atomic<uint64_t> hi{0};   // words [0, hi) generated; the last kRing of them are in buf
+    std::atomic<uint64_t> low{0};  // words below low are no longer read (host or copy engine)
+    std::atomic<bool> stop{false};
+    std::thread th;
+    // The generator runs ahead on its own thread, at most kRing words past
+    // `low`, so a slot is rewritten only after its word has been consumed.
+    void start(uint64_t seed) {
+      halt();
+      mt.seed_seq2(0u, (uint32_t)seed);
+      hi.store(0);
+      low.store(0);
+      stop.store(false);
+      th = std::thread([this] {
+        uint32_t blk[MT19937::N];
+        while (!stop.load(std::memory_order_relaxed)) {
+          const uint64_t h = hi.load(std::memory_order_relaxed);
+          if (h + MT19937::N > low.load(std::memory_order_acquire) + kRing) {
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            continue;
+          }
+          mt.next_block(blk);
+          for (int k = 0; k < MT19937::N; k++) buf[(h + k) & (kRing - 1)] = blk[k];
+          hi.store(h + MT19937::N, std::memory_order_release);
+        }
+      });
+    }
+    void halt() {
+      if (th.joinable()) {
+        stop.store(true);
+        th.join();
+      }
+    }
+    void set_low(uint64_t v) { low.store(v, std::memory_order_release); }
+    void ensure(uint64_t upto) {
+      if (upto > low.load(std::memory_order_relaxed) + kRing) throw Error("word window larger than the ring");
+      while (hi.load(std::memory_order_acquire) < upto) {
+      }
+    }
+    uint32_t at(uint64_t pos) {
+      if (hi.load(std::memory_order_acquire) <= pos) ensure(pos + 1);
+      return buf[pos & (kRing - 1)];
+    }
+    ~HostRing() { halt(); }
+  };
+  // URNG over the host ring with std::mt19937's result_type/min/max (see StreamURNG)
+  struct RingURNG {
+    using result_type = unsigned long;
+    HostRing* s;
+    uint64_t pos;
+    static constexpr result_type min() { return 0; }
+    static constexpr result_type max() { return 0xffffffffUL; }
+    result_type operator()() { return s->at(pos++); }
+  };
+  HostRing hs_;
   DevBuf<uint32_t> ring_;
-  uint64_t dev_hi_ = 0;  // words [0, dev_hi_) have been written to the ring
+  uint64_t dev_hi_ = 0;  // words [0, dev_hi_) have been copied to the device ring
 
   CellStore<D> cs() { return CellStore<D>{cells_cnt_.p, cells_sph_.p, cells_ph_.p, cap_}; }
 
@@ -1262,8 +1897,15 @@ class Engine final : public IEngine {
     };
     std::sort(order.begin(), order.end(), [&](int a, int b) { return norm2(vec[a]) < norm2(vec[b]); });
     geo_.nst = n3;
-    for (int i = 0; i < n3; i++)
-      for (int d = 0; d < 4; d++) geo_.st[i][d] = d < D ? (signed char)vec[order[i]][d] : 0;
+    std::vector<int> packed(n3);
+    for (int i = 0; i < n3; i++) {
+      int v = 0;
+      for (int d = 0; d < D; d++) v |= ((int)vec[order[i]][d] + 1) << (2 * d);
+      packed[i] = v;
+    }
+    stab_.reserve(n3);
+    HIP_OK(hipMemcpy(stab_.p, packed.data(), n3 * sizeof(int), hipMemcpyHostToDevice));
+    geo_.stab = stab_.p;
   }
 
   void set_classes(const std::vector<RadiusClass>& cls, double exclusion) {  // radius_generator.ixx:9-28
@@ -1299,6 +1941,9 @@ class Engine final : public IEngine {
     cells_ph_.reserve((size_t)geo_.ncells * cap_);
     cell2t_.reserve(geo_.ncells);
     cellcc_.reserve(geo_.ncells);
+    ovhead_.reserve(geo_.ncells);
+    HIP_OK(hipMemset(ovhead_.p, 0xff, (size_t)geo_.ncells * sizeof(int)));
+    cslot_.reserve((size_t)geo_.ncells * CSLOT);
     HIP_OK(hipMemset(cells_cnt_.p, 0, geo_.ncells));
     HIP_OK(hipMemset(cell2t_.p, 0xff, (size_t)geo_.ncells * sizeof(int)));
     HIP_OK(hipMemset(cellcc_.p, 0, (size_t)geo_.ncells * sizeof(int)));
@@ -1318,17 +1963,16 @@ class Engine final : public IEngine {
     HIP_OK(hipStreamSynchronize(st_));
   }
 
-  // copy words [dev_hi_, upto) of the host stream into the ring
+  // copy words [dev_hi_, upto) of the host ring into the device ring (stream-ordered).
+  // A pinned-ring slot is rewritten kRing words later, long after its copy ran.
   void upload_words(uint64_t upto) {
     hs_.ensure(upto);
     while (dev_hi_ < upto) {
       const uint64_t off = dev_hi_ & (kRing - 1);
       const uint64_t n = std::min<uint64_t>(upto - dev_hi_, kRing - off);
-      HIP_OK(hipMemcpyAsync(ring_.p + off, hs_.buf.data() + (dev_hi_ - hs_.base), n * sizeof(uint32_t),
-                            hipMemcpyHostToDevice, st_));
+      HIP_OK(hipMemcpyAsync(ring_.p + off, hs_.buf + off, n * sizeof(uint32_t), hipMemcpyHostToDevice, st_));
       dev_hi_ += n;
     }
-    HIP_OK(hipStreamSynchronize(st_));
   }
 
   void ensure_cand(int n) {
@@ -1350,6 +1994,9 @@ class Engine final : public IEngine {
     pcnt_.reserve(c);
     ctag_.reserve(c);
     lab_.reserve(c);
+    next_.reserve(c);
+    owner_.reserve(c);
+    slab_.reserve(c * KMAX);
     if (plist_.cap < 8 * c) plist_.reserve(8 * c);
   }
 
@@ -1451,8 +2098,33 @@ class Engine final : public IEngine {
   }
 
   double mom_pivot(std::vector<uint64_t> values, int64_t nb_first);
-  int64_t single_draw(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
+  // A draw's host-side inputs: its Poisson shot count and the voxel words of its
+  // shots (Lemire rejection chain), computed from a stream position.
+  struct Prep {
+    int64_t nv = -1;         // voxel count the Poisson intensity used (-1: none)
+    uint64_t w_start = 0;    // stream position before the Poisson draws
+    int64_t zero_iters = 0;  // Poisson draws that gave no shot
+    int nshots = 0;
+    uint64_t w_shots = 0;    // stream position of the first shot word
+    int end_rel = 0;         // words of the shots' positions
+    int buf = 0;             // pinned voxel-word buffer holding the chain
+  };
+  int64_t single_draw(const Prep& pr, uint64_t maximal, bool may_outreach, Prep* spec, int64_t& rounds,
                       uint64_t& w_next, rsa_gpu_stats* st);
+  int64_t single_draw_slow(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
+                           uint64_t& w_next, rsa_gpu_stats* st);
+  int host_chain(int nshots, uint64_t w0, int b);
+  Prep prepare(uint64_t w, int b);
+  void wait_pub(int seq);
+  DevBuf<DrawCtl> dctl_;
+  DrawPub* hpub_ = nullptr;      // pinned, coherent: written by k_sweep_labels
+  DrawPub* hpub_dev_ = nullptr;  // its device address
+  int seq_ = 0;
+  uint32_t* hvpos_[2] = {nullptr, nullptr};  // pinned voxel-word offsets of the shots (double buffer)
+  size_t hvpos_cap_[2] = {0, 0};
+  DevBuf<int> next_, cslot_, ovhead_, slab_, pool_;
+  DevBuf<uint8_t> owner_;
+  DevBuf<int> stab_;  // packed stencil offsets (Geo::stab)
 };
 
 // median_of_medians::find_pivot_for_first_elements on one rank (median_of_medians.ixx:8-178)
@@ -1505,9 +2177,172 @@ double Engine<D>::mom_pivot(std::vector<uint64_t> values, int64_t nb_first) {
   return pivot;
 }
 
-// generate_spheres + add_to_sample loop (operator_algorithm.ixx:534-582, 383-443)
+// Voxel-index words of the shots (uniform_int_distribution's Lemire rejection,
+// uniform_int_dist.h:246-268): the host holds the word stream anyway (Poisson
+// draws from it), so it walks the rejections; the decode itself runs on the GPU.
 template <int D>
-int64_t Engine<D>::single_draw(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
+int Engine<D>::host_chain(int nshots, uint64_t w0, int b) {
+  const uint32_t N = (uint32_t)nvox_;
+  const uint32_t thr = (uint32_t)(0u - N) % N;
+  const uint64_t stride = 2 * D + 1;
+  if ((size_t)nshots > hvpos_cap_[b]) {  // the other buffer may still feed a copy; this one is idle
+    if (hvpos_[b]) HIP_OK(hipHostFree(hvpos_[b]));
+    hvpos_cap_[b] = (size_t)nshots * 5 / 4 + 1024;
+    HIP_OK(hipHostMalloc((void**)&hvpos_[b], hvpos_cap_[b] * sizeof(uint32_t), hipHostMallocDefault));
+  }
+  uint32_t* out = hvpos_[b];
+  hs_.ensure(w0 + (uint64_t)nshots * stride + 64);
+  uint64_t p = w0;
+  for (int i = 0; i < nshots; i++) {
+    while ((uint32_t)((uint64_t)hs_.at(p) * (uint64_t)N) < thr) p++;
+    out[i] = (uint32_t)(p - w0);
+    p += stride;
+  }
+  return (int)(p - w0);
+}
+
+// Poisson shot count(s) (compute_nb_shots_voxel, operator_algorithm.ixx:208-223)
+// from position w until a draw has shots, then its rejection chain into pinned
+// buffer b.  A prepared draw stays valid while the voxel count is unchanged.
+template <int D>
+typename Engine<D>::Prep Engine<D>::prepare(uint64_t w, int b) {
+  Prep p;
+  p.nv = nvox_;
+  p.w_start = w;
+  p.buf = b;
+  const int nv = (int)nvox_;
+  if (nv == 0) return p;
+  const double intensity = 0.01 * nv;  // compute_intensity_poisson (:221-223)
+  const double max_intensity = intensity;
+  double mult = nbshots_ / max_intensity;
+  if (mult > 1) mult = 1.;
+  for (;;) {
+    RingURNG g{&hs_, w};
+    std::poisson_distribution<> pd(mult * intensity);
+    const int n = pd(g);
+    w = g.pos;
+    if (n == 0) {
+      p.zero_iters++;
+      continue;
+    }
+    if ((uint64_t)n > (uint64_t)INT32_MAX / 16) throw Error("too many shots in one draw");
+    p.nshots = n;
+    break;
+  }
+  p.w_shots = w;
+  p.end_rel = host_chain(p.nshots, w, b);
+  return p;
+}
+
+template <int D>
+void Engine<D>::wait_pub(int seq) {
+  const auto t0 = std::chrono::steady_clock::now();
+  volatile int* s = &hpub_->seq;
+  while (*s != seq) {
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60)) {
+      (void)hipStreamSynchronize(st_);
+      if (*s == seq) break;
+      throw Error("internal: draw results not published");
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+}
+
+// generate_spheres + add_to_sample loop (operator_algorithm.ixx:534-582, 383-443)
+// for draws that cannot overshoot the class target: five kernels per round and
+// one host round trip (k_sweep_labels publishes the round's counts to pinned
+// host memory; the tag update and commit run while the host moves on).
+template <int D>
+int64_t Engine<D>::single_draw(const Prep& pr, uint64_t maximal, bool may_outreach, Prep* spec, int64_t& rounds,
+                               uint64_t& w_next, rsa_gpu_stats* st) {
+  const int nshots = pr.nshots;
+  const uint64_t w0 = pr.w_shots;
+  if (may_outreach || nshots > 48000) return single_draw_slow(nshots, maximal, may_outreach, w0, rounds, w_next, st);
+  ensure_cand(nshots);
+  if (!hpub_) {
+    dctl_.reserve(1);
+    HIP_OK(hipMemset(dctl_.p, 0, sizeof(DrawCtl)));
+    HIP_OK(hipHostMalloc((void**)&hpub_, sizeof(DrawPub), hipHostMallocCoherent | hipHostMallocMapped));
+    HIP_OK(hipHostGetDevicePointer((void**)&hpub_dev_, hpub_, 0));
+    std::memset(hpub_, 0, sizeof(DrawPub));
+    pool_.reserve(1 << 16);
+  }
+  const int end_rel = pr.end_rel;
+  upload_words(w0 + (uint64_t)end_rel + (uint64_t)nshots);
+  HIP_OK(hipMemcpyAsync(vpos_.p, hvpos_[pr.buf], (size_t)nshots * sizeof(uint32_t), hipMemcpyHostToDevice, st_));
+  const uint32_t N = (uint32_t)nvox_;
+  ClassDev cls{std::get<0>(classes_[cur_]), (uint32_t)std::get<2>(classes_[cur_])};
+  using LG = LaneGroup<D>;
+  const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
+  const unsigned nbs = nblocks(nshots);
+  timed("k_decode_filter", (double)nshots * (4.0 * (2 * D + 2) + 4.0 * D + 8.0 * (D + 1) + 13.0), [&] {
+    hipLaunchKernelGGL(k_decode_filter<D>, dim3(nbl), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N, vpos_.p,
+                       end_rel, vview(), cls, geo_, cs(), cpos_.p, crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p,
+                       owner_.p, cellcc_.p, cslot_.p, ovhead_.p, next_.p, dctl_.p);
+  });
+  timed("k_lists", (double)nshots * 24.0, [&] {
+    hipLaunchKernelGGL(k_lists, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, owner_.p, ctag_.p, cellcc_.p, cslot_.p,
+                       ovhead_.p, next_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p);
+  });
+  auto round = [&](int seq) {
+    timed("k_partner_lists", (double)nshots * (8.0 * (D + 1) + 10.0), [&] {
+      hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, cpos_.p, crad_.p, cprio_.p,
+                         ccell_.p, ctag_.p, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, poff_.p, pcnt_.p, slab_.p,
+                         pool_.p, (int)pool_.cap, dctl_.p);
+    });
+    timed("k_sweep_labels", (double)nshots * 4.0, [&] {
+      hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(1024), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, poff_.p,
+                         pcnt_.p, slab_.p, pool_.p, dctl_.p, hpub_dev_, seq);
+    });
+    timed("k_update_commit", (double)nshots * (8.0 * (D + 1) + 12.0), [&] {
+      hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
+                         ccell_.p, owner_.p, ctag_.p, lab_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p);
+    });
+    HIP_OK(hipGetLastError());
+  };
+  int seq = ++seq_;
+  round(seq);
+  w_next = w0 + (uint64_t)end_rel + (uint64_t)nshots;
+  // While the GPU runs the first round: the next draw's Poisson count and
+  // rejection chain, valid if no refinement changes the voxel count.  Its chain
+  // goes to the other pinned buffer, whose last copy ran before this draw.
+  if (spec) *spec = prepare(w_next, pr.buf ^ 1);
+  int64_t local_new = 0;
+  int r = 0;
+  for (;;) {
+    wait_pub(seq);
+    const DrawPub h = *hpub_;
+    if (h.label_stuck) throw Error("internal: conflict labelling made no progress");
+    st->ghost_cell_drops += h.dropped;
+    st->image_cell_mismatch += h.img_mismatch;
+    if (h.pool_overflow) {  // long partner lists: grow the pool and redo the round
+      pool_.reserve((size_t)h.need_pool * 2);
+      HIP_OK(hipMemsetAsync(&dctl_.p->pool_cursor, 0, 3 * sizeof(int), st_));
+      seq = ++seq_;
+      round(seq);
+      continue;
+    }
+    local_new += h.acc;
+    if (h.con == 0) {
+      rounds = r + 1;
+      break;
+    }
+    if (++r >= 100000) throw Error("too many conflict rounds in one draw");
+    timed("k_refilter", (double)nshots * (8.0 * (D + 1) + 5.0), [&] {
+      hipLaunchKernelGGL(k_refilter<D>, dim3(nbl), dim3(WG), 0, st_, geo_, cs(), nshots, cpos_.p, crad_.p, ccell_.p,
+                         ctag_.p, cell2t_.p, seg_.p, len_.p, ord_.p);
+    });
+    hipLaunchKernelGGL(k_remove_skips, dim3(nbs), dim3(WG), 0, st_, nshots, owner_.p, seg_.p, len_.p, ord_.p, ctag_.p);
+    seq = ++seq_;
+    round(seq);
+  }
+  return local_new;
+}
+
+// Host-driven form of a draw; used when the draw may overshoot the class target
+// (retain_only_first_elements needs the host-side pivot between sweep and update).
+template <int D>
+int64_t Engine<D>::single_draw_slow(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
                                uint64_t& w_next, rsa_gpu_stats* st) {
   ensure_cand(nshots);
   const uint32_t N = (uint32_t)nvox_;
@@ -1621,8 +2456,14 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
   if (!algo_ready_) throw Error("rsa_algo not initialised");
   const auto t0 = std::chrono::steady_clock::now();
   std::memset(st, 0, sizeof(*st));
-  hs_.reset(seed);
+  hs_.start(seed);
+  struct Halt {
+    HostRing& h;
+    ~Halt() { h.halt(); }
+  } halt_at_exit{hs_};
   dev_hi_ = 0;
+  Prep spec;
+  int nextbuf = 0;
   uint64_t w = 0;
   init_voxels(r_max_ + gmin_);
   st->nvox_max = nvox_;
@@ -1637,30 +2478,31 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
       st->end_reason = 3;
       break;
     }
-    const int nv = (int)nvox_;
-    const double intensity = 0.01 * nv;
-    const double max_intensity = intensity;
-    const uint64_t total_vox = (uint64_t)nvox_;
-    if (total_vox == 0) {
+    if (nvox_ == 0) {
       st->end_reason = 1;
       break;
     }
-    double mult = nbshots_ / max_intensity;
-    if (mult > 1) mult = 1.;
-    StreamURNG g{&hs_, w};
-    std::poisson_distribution<> pd(mult * intensity);
-    const int nshots_i = pd(g);
-    w = g.pos;
-    const uint64_t nshots = (uint64_t)nshots_i;
-    if (nshots == 0) continue;
-    if (nshots > (uint64_t)INT32_MAX / 16) throw Error("too many shots in one draw");
+    // every copy enqueued so far has completed (each draw and refinement ends
+    // with a host wait on the stream): the producer may reuse older ring slots
+    hs_.set_low(std::min(w, dev_hi_));
+    Prep pr;
+    if (spec.nv == nvox_ && spec.w_start == w) {
+      pr = spec;
+    } else {
+      pr = prepare(w, nextbuf);
+    }
+    nextbuf = pr.buf ^ 1;
+    spec.nv = -1;
+    st->loop_iterations += pr.zero_iters;  // the iterations whose Poisson draw gave no shot
+    const uint64_t nshots = (uint64_t)pr.nshots;
     const bool may_outreach = nshots >= current_number();
     const uint64_t maximal = current_number();
     int64_t rounds = 0;
     uint64_t w_next = 0;
-    const uint64_t nb_new = (uint64_t)single_draw((int)nshots, maximal, may_outreach, w, rounds, w_next, st);
+    const bool may_continue = max_draws < 0 || st->draws + 1 < max_draws;
+    const uint64_t nb_new =
+        (uint64_t)single_draw(pr, maximal, may_outreach, may_continue ? &spec : nullptr, rounds, w_next, st);
     w = w_next;
-    hs_.drop_below(w);
     st->draws++;
     st->shots += (int64_t)nshots;
     st->accepted += (int64_t)nb_new;
