@@ -1179,28 +1179,26 @@ template <int D>
 struct VoxParams {
   static constexpr int NCH = ipow_c(2, D);
   double origin[D], len[D], nl[D];
-  double corners[NCH][D];   // parent corners: tab[i][d] * len[d]
-  double ncorners[NCH][D];  // child corners: tab[i][d] * nl[d]
   signed char tab[NCH][D];
+  // cm[d]: bit t set when some TabCorner row has component d equal to t.  The
+  // corner set {tab[i] * len} is the product over d of these values.
+  unsigned char cm[D];
   double rmin;
 };
 
-// auxi::is_in_sphere (list_of_voxels.ixx:225-238)
-template <int D>
-__device__ __forceinline__ bool in_sphere(const double (*corners)[D], const double* delta, double r, double rmin) {
-  constexpr int NCH = ipow_c(2, D);
-  const double R2 = (r + rmin) * (r + rmin);
-#pragma unroll
-  for (int i = 0; i < NCH; i++) {
-    double s = 0.;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      const double t = corners[i][d] - delta[d];
-      s += t * t;
-    }
-    if (s > R2) return false;
-  }
-  return true;
+// auxi::is_in_sphere (list_of_voxels.ixx:225-238) asks whether every corner
+// tab[i] * len of a box lies within r + rmin of the sphere centre, the corner
+// term of dimension d being t = tab[i][d] * len[d] - delta[d], summed as
+// s = ((0 + t0^2) + t1^2) + ... in dimension order.  The corner set is a
+// product set, and IEEE addition and squaring are monotone (in |t| for the
+// square), so the largest corner sum is the same sum over the per-dimension
+// largest terms: testing that one sum gives the all-corner loop's answer bit
+// for bit.  far_term is that largest term of one dimension.
+__device__ __forceinline__ double far_term(double delta, double len, unsigned cm) {
+  const double a = 0.0 - delta;  // tab 0: 0 * len - delta
+  const double b = len - delta;  // tab 1: 1 * len - delta
+  const double aa = a * a, bb = b * b;
+  return cm == 1 ? aa : (cm == 2 ? bb : fmax(aa, bb));
 }
 
 // remove_covered's coverage flag (list_of_voxels.ixx:30-40, 240-259) and
@@ -1215,22 +1213,18 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t code = 0;
   if (i < nvox) {
-    int64_t c[D];
-    double ov[D], ca[D], cb[D];
+    double ov[D], ca[D], cb[D], ok0[D], ok1[D];
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      c[d] = vx.c[d][i];
-      ov[d] = vp.origin[d] + (double)c[d] * vp.len[d];  // orig_voxel (list_of_voxels.ixx:146-153)
-      ca[d] = ov[d] + 0.5 * vp.len[d];                  // is_covered's centre (:245)
-      cb[d] = vp.origin[d] + ((double)c[d] + 0.5) * vp.len[d];  // center() used by subdivide (:155-162)
+      const int64_t c = vx.c[d][i];
+      ov[d] = vp.origin[d] + (double)c * vp.len[d];  // orig_voxel (list_of_voxels.ixx:146-153)
+      ca[d] = ov[d] + 0.5 * vp.len[d];                // is_covered's centre (:245)
+      cb[d] = vp.origin[d] + ((double)c + 0.5) * vp.len[d];  // center() used by subdivide (:155-162)
+      ok0[d] = vp.origin[d] + (double)(2 * c) * vp.nl[d];      // children origins (:82-86), tab 0
+      ok1[d] = vp.origin[d] + (double)(2 * c + 1) * vp.nl[d];  // and tab 1
     }
     const int64_t cella = cell_index<D>(g, ca);
     const int64_t cellb = cell_index<D>(g, cb);
-    double ovk[NCH][D];  // children origins (list_of_voxels.ixx:82-86)
-#pragma unroll
-    for (int k = 0; k < NCH; k++)
-#pragma unroll
-      for (int d = 0; d < D; d++) ovk[k][d] = vp.origin[d] + (double)(2 * c[d] + vp.tab[k][d]) * vp.nl[d];
     uint32_t ccov = 0;
     bool covered = false;
     // pass 0: coverage around cella (and children when cellb == cella); pass 1: children around cellb
@@ -1255,11 +1249,13 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
 #pragma unroll
             for (int d = 0; d < D; d++) qp[d] = q.c[d];
           }
+          const double rr = q.r + vp.rmin;
+          const double R2 = rr * rr;
           if (do_cov) {
-            double delta[D];
+            double sf = 0.;
 #pragma unroll
-            for (int d = 0; d < D; d++) delta[d] = qp[d] - ov[d];
-            if (in_sphere<D>(vp.corners, delta, q.r, vp.rmin)) {
+            for (int d = 0; d < D; d++) sf += far_term(qp[d] - ov[d], vp.len[d], vp.cm[d]);
+            if (!(sf > R2)) {
               covered = true;
               break;
             }
@@ -1271,15 +1267,19 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
               const double t = qp[d] - cb[d];
               s2 += t * t;
             }
-            const double rr = q.r + vp.rmin;
-            if (s2 < rr * rr) {
+            if (s2 < R2) {
+              double f0[D], f1[D];
+#pragma unroll
+              for (int d = 0; d < D; d++) {
+                f0[d] = far_term(qp[d] - ok0[d], vp.nl[d], vp.cm[d]);
+                f1[d] = far_term(qp[d] - ok1[d], vp.nl[d], vp.cm[d]);
+              }
 #pragma unroll
               for (int ch = 0; ch < NCH; ch++) {
-                if (ccov & (1u << ch)) continue;
-                double delta[D];
+                double sf = 0.;
 #pragma unroll
-                for (int d = 0; d < D; d++) delta[d] = qp[d] - ovk[ch][d];
-                if (in_sphere<D>(vp.ncorners, delta, q.r, vp.rmin)) ccov |= (1u << ch);
+                for (int d = 0; d < D; d++) sf += vp.tab[ch][d] ? f1[d] : f0[d];
+                if (!(sf > R2)) ccov |= (1u << ch);
               }
             }
           }
@@ -2042,12 +2042,24 @@ class Engine final : public IEngine {
       vp.len[d] = vlen_[d];
       vp.nl[d] = 0.5 * vlen_[d];  // new_voxel_lengths (list_of_voxels.ixx:62)
     }
+    for (int d = 0; d < D; d++) vp.cm[d] = 0;
     for (int k = 0; k < VoxParams<D>::NCH; k++)
       for (int d = 0; d < D; d++) {
+        if (tab_[k][d] != 0 && tab_[k][d] != 1) throw Error("internal: TabCorner entries must be 0 or 1");
         vp.tab[k][d] = (signed char)tab_[k][d];
-        vp.corners[k][d] = tab_[k][d] * vp.len[d];  // create_corners_voxel_inplace (:309-318)
-        vp.ncorners[k][d] = tab_[k][d] * vp.nl[d];
+        vp.cm[d] |= (unsigned char)(1u << tab_[k][d]);
       }
+    // the farthest-corner test in k_cover needs the corners to form a product set
+    int prod = 1;
+    for (int d = 0; d < D; d++) prod *= (vp.cm[d] == 3 ? 2 : 1);
+    std::vector<int> seen(1 << D, 0);
+    int distinct = 0;
+    for (int k = 0; k < VoxParams<D>::NCH; k++) {
+      int key = 0;
+      for (int d = 0; d < D; d++) key |= tab_[k][d] << d;
+      if (!seen[key]++) distinct++;
+    }
+    if (distinct != prod) throw Error("internal: TabCorner rows are not a product set");
     vp.rmin = rmin;
     return vp;
   }

@@ -1,0 +1,129 @@
+"""CPU checks of the library boundary and of the identities the HIP kernels rely
+on where they evaluate the reference's formulas in a different, exactly
+equivalent order."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rsa_mpi_amd.h")
+
+
+# ---------------------------------------------------------------- the C-ABI boundary
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return sorted(set(re.findall(r"\b(rsa_gpu_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    """librsa_mpi_amd.so loads on a host without a GPU and exports each function
+    include/rsa_mpi_amd.h declares (no compute call is made)."""
+    import rsa_mpi_amd._lib as L
+    if not os.path.exists(L.LIB_PATH):
+        pytest.fail(f"{L.LIB_PATH} not built: run rsa_mpi_amd/build.sh")
+    lib = ctypes.CDLL(L.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 17
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(L.EXPORTED) == names
+
+
+def test_binding_fails_loudly_without_library(monkeypatch, tmp_path):
+    import rsa_mpi_amd._lib as L
+    monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(L, "_lib", None)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        L.lib()
+
+
+def test_host_only_entry_points_work_without_gpu():
+    """rsa_gpu_mt_words runs on the host: it reproduces std::mt19937 seeded
+    with seed_seq{rank, seed}.  Errors come back as codes with a message."""
+    import rsa_mpi_amd as R
+    import _oracle as O
+    w = R.mt_words(123, 5000)
+    assert np.array_equal(w, O.mt_words(123, 5000))
+    lib = R.lib()
+    assert lib.rsa_gpu_proceed(None, 0, -1, None) != 0
+    assert b"null context" in lib.rsa_gpu_last_error()
+
+
+# ---------------------------------------------------------------- farthest-corner test
+
+def corner_loop(delta, length, tab, R2):
+    """auxi::is_in_sphere (list_of_voxels.ixx:225-238): every corner
+    tab[i] * length within R2; corner term tab*len - delta, summed in order."""
+    for row in tab:
+        s = 0.0
+        for d in range(len(length)):
+            t = float(row[d]) * length[d] - delta[d]
+            s += t * t
+        if s > R2:
+            return False
+    return True
+
+
+def farthest_corner(delta, length, tab, R2):
+    """The single-sum form k_cover evaluates: per dimension the larger of the
+    terms over the tab values present in that dimension, summed in order."""
+    s = 0.0
+    for d in range(len(length)):
+        vals = sorted(set(int(r[d]) for r in tab))
+        terms = []
+        for v in vals:
+            t = float(v) * length[d] - delta[d]
+            terms.append(t * t)
+        s += max(terms)
+    return not (s > R2)
+
+
+TABS = {
+    2: [[0, 0], [1, 1], [0, 1], [1, 0]],
+    3: [[0, 0, 0], [1, 1, 1], [0, 0, 1], [1, 1, 0], [0, 1, 0], [1, 0, 1], [0, 1, 1], [1, 0, 0]],
+    # the reference's D >= 4 table leaves component 0 at 0 (auxiliary/Path.ixx:105-115)
+    4: [[0] + [(i >> d) & 1 for d in range(1, 4)] for i in range(16)],
+}
+
+
+@pytest.mark.parametrize("dim", [2, 3, 4])
+def test_farthest_corner_equals_corner_loop(dim):
+    rng = np.random.default_rng(dim)
+    tab = TABS[dim]
+    n_true = 0
+    for trial in range(20000):
+        length = rng.uniform(1e-4, 2e-2, size=dim)
+        if trial % 3 == 0:
+            length[:] = length[0]
+        delta = rng.uniform(-0.03, 0.05, size=dim)
+        if trial % 5 == 0:  # centre exactly on a face or a corner coordinate
+            d = rng.integers(dim)
+            delta[d] = length[d] * rng.integers(0, 2)
+        # R2 at, just below and just above one corner's exact sum: ties decide
+        row = tab[rng.integers(len(tab))]
+        s = 0.0
+        for d in range(dim):
+            t = float(row[d]) * length[d] - delta[d]
+            s += t * t
+        for R2 in (s, np.nextafter(s, 0.0), np.nextafter(s, 1.0), rng.uniform(0, 3e-3)):
+            a = corner_loop(delta, length, tab, R2)
+            assert a == farthest_corner(delta, length, tab, R2), (dim, delta, length, R2)
+            n_true += a
+    assert n_true > 1000  # both outcomes are exercised
+
+
+def test_farthest_corner_needs_the_product_set():
+    """A corner set that is not a product set breaks the identity (so k_cover's
+    host check on the table is needed): corners (0,0) and (1,1) only."""
+    tab = [[0, 0], [1, 1]]
+    length = [1.0, 1.0]
+    delta = [0.0, 1.0]  # sphere centred on corner (0, 1), which is not in the set
+    R2 = 1.0
+    assert corner_loop(delta, length, tab, R2) is True
+    assert farthest_corner(delta, length, tab, R2) is False
