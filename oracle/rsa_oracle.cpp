@@ -82,6 +82,8 @@ Result* run_impl(const double* inf, const double* sup, int gl, double drad, int 
     for (int d = 0; d < D; d++) r->pos[i * D + d] = p[i][d];
   r->trace = algo.trace;
   r->words = algo.words_used;
+  r->vox = algo.last_vox;
+  r->vox_len.assign(algo.last_len.begin(), algo.last_len.end());
   r->msg = algo.end_message;
   for (const auto& dd : algo.dumps) {
     Dump o;

@@ -1130,7 +1130,13 @@ struct Algo {
       }
     }
     words_used = rng.used;
+    last_vox.clear();  // the voxel list where this proceed stopped
+    for (const auto& c : vox.c)
+      for (int d = 0; d < D; d++) last_vox.push_back(c[d]);
+    last_len = vox.len;
   }
+  std::vector<int64_t> last_vox;
+  Pt<D> last_len{};
 };
 
 }  // namespace rsa_oracle

@@ -10,6 +10,6 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
   -Xclang -target-feature -Xclang -packed-fp32-ops \
   -Wall -Wno-unused-function \
   -o "$here/lib/librsa_mpi_amd.so.tmp" \
-  "$here/csrc/engine.hip" "$here/csrc/capi.cpp" 2>&1 | grep -v "packed-fp32-ops" || true
+  "$here/csrc/engine.hip" "$here/csrc/capi.cpp" "$here/csrc/mt19937.cpp" 2>&1 | grep -v "packed-fp32-ops" || true
 test -f "$here/lib/librsa_mpi_amd.so.tmp"
 mv "$here/lib/librsa_mpi_amd.so.tmp" "$here/lib/librsa_mpi_amd.so"

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <deque>
 #include <thread>
 #include <chrono>
 #include <cstddef>
@@ -643,12 +644,25 @@ struct DrawCtl {
   int acc, con;
   int label_stuck;
   int cell_overflow;
+  // Draw gate.  The first-round kernels of draw `id` run only while gate == id
+  // (a forced launch of k_decode_filter sets it).  A draw's last sweep opens
+  // the gate for draw id + 1 when the draw is over and its miss rate asks for
+  // no refinement; otherwise it closes it (-1).  The host may thus queue the
+  // next draw before it has seen this one's results: when the gate stays
+  // closed, every kernel of that draw returns at once.
+  int gate;
+  int swept;     // seq of the last sweep that ran (k_update_commit of that round runs)
+  int acc_draw;  // spheres accepted so far in the current draw
 };
-// What k_label publishes to pinned host memory (seq last).
+// What k_sweep_labels publishes to pinned host memory (seq last).
 struct DrawPub {
-  int acc, con, label_stuck, dropped, img_mismatch, pool_overflow, need_pool, cell_overflow;
+  int acc, con, label_stuck, dropped, img_mismatch, pool_overflow, need_pool, cell_overflow, gate_open;
   int seq;
 };
+constexpr int kPubFields = 9;
+// Publications go to slot seq % kPubSlots: a queued next draw may publish
+// before the host has read the current one.
+constexpr int kPubSlots = 8;
 
 // pick_points + generate_priority + constant-radius class (list_of_voxels.ixx:107-129,
 // operator_algorithm.ixx:373-381, radius_generator.ixx:50-60); build_grid's
@@ -666,9 +680,14 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
                                                       uint8_t* __restrict__ ctag, uint8_t* __restrict__ owner,
                                                       int* __restrict__ cellcc, int* __restrict__ cslot,
                                                       int* __restrict__ ovhead, int* __restrict__ next,
-                                                      DrawCtl* __restrict__ ctl) {
+                                                      DrawCtl* __restrict__ ctl, int draw_id, int forced) {
   using LG = LaneGroup<D>;
+  if (!forced && ctl->gate != draw_id) return;  // whole grid: the gate does not change during this kernel
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gtid == 0) {
+    ctl->acc_draw = 0;
+    if (forced) ctl->gate = draw_id;
+  }
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
   const int sub = gtid % LG::LPC;
@@ -754,7 +773,8 @@ __global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict_
                                               int* __restrict__ cellcc, const int* __restrict__ cslot,
                                               int* __restrict__ ovhead, const int* __restrict__ next,
                                               int* __restrict__ seg, int* __restrict__ len, int* __restrict__ ord,
-                                              int* __restrict__ cell2t, DrawCtl* __restrict__ ctl) {
+                                              int* __restrict__ cell2t, DrawCtl* __restrict__ ctl, int draw_id) {
+  if (ctl->gate != draw_id) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = i < nshots && owner[i];
   const int c = on ? ccell[i] : 0;
@@ -954,8 +974,9 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
                                                  const int* __restrict__ seg, const int* __restrict__ len,
                                                  const int* __restrict__ ord, int* __restrict__ poff,
                                                  int* __restrict__ pcnt, int* __restrict__ slab,
-                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl) {
+                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int gate_id) {
   using LG = LaneGroup<D>;
+  if (gate_id >= 0 && ctl->gate != gate_id) return;
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
@@ -1032,9 +1053,11 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
 __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
                                                 const int* __restrict__ poff, const int* __restrict__ pcnt,
                                                 const int* __restrict__ slab, const int* __restrict__ pool,
-                                                DrawCtl* ctl, DrawPub* host, int seq) {
+                                                DrawCtl* ctl, DrawPub* host, int seq, int gate_id, int draw_id,
+                                                double aimed, uint64_t miss_base) {
   extern __shared__ uint8_t s_lab[];
   __shared__ int s_prog, s_left, s_acc, s_con, s_stuck;
+  if (gate_id >= 0 && ctl->gate != gate_id) return;
   const int tid = threadIdx.x;
   if (tid == 0) {
     s_acc = 0;
@@ -1106,11 +1129,26 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
   ctl->acc = s_acc;
   ctl->con = s_con;
   ctl->label_stuck = s_stuck;
-  const int v[8] = {s_acc, s_con, s_stuck, ctl->dropped, ctl->img_mismatch, ctl->pool_overflow, ctl->need_pool,
-                    ctl->cell_overflow};
-  for (int k = 0; k < 8; k++) __atomic_store_n(&reinterpret_cast<int*>(host)[k], v[k], __ATOMIC_RELAXED);
+  // the draw's refinement decision, as the host makes it after the draw
+  // (operator_algorithm.ixx:170-180): miss rate = misses / (1e-6 + shots),
+  // misses = shots - (spheres now - spheres after the previous draw of this
+  // proceed) in unsigned 64-bit arithmetic; miss_base carries the part of the
+  // difference placed before the draw (non-zero on a proceed's first draw).
+  int open = 0;
+  if (!ovf) ctl->acc_draw += s_acc;
+  if (!ovf && !s_stuck && s_con == 0) {
+    const uint64_t miss = (uint64_t)nshots - (uint64_t)ctl->acc_draw - miss_base;
+    const double rate = (double)miss / (0.000001 + (double)(uint64_t)nshots);
+    open = !(rate > aimed);
+  }
+  ctl->gate = open ? draw_id + 1 : -1;
+  ctl->swept = seq;
+  const int v[kPubFields] = {s_acc,           s_con,          s_stuck, ctl->dropped, ctl->img_mismatch,
+                             ctl->pool_overflow, ctl->need_pool, ctl->cell_overflow, open};
+  DrawPub* hp = host + (seq & (kPubSlots - 1));
+  for (int k = 0; k < kPubFields; k++) __atomic_store_n(&reinterpret_cast<int*>(hp)[k], v[k], __ATOMIC_RELAXED);
   __threadfence_system();
-  __atomic_store_n(&host->seq, seq, __ATOMIC_RELEASE);
+  __atomic_store_n(&hp->seq, seq, __ATOMIC_RELEASE);
   // counters whose values are now published
   ctl->dropped = 0;
   ctl->img_mismatch = 0;
@@ -1130,8 +1168,9 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
                                                       uint8_t* __restrict__ ctag, const uint8_t* __restrict__ lab,
                                                       const int* __restrict__ seg, int* __restrict__ len,
                                                       int* __restrict__ ord, int* __restrict__ cell2t,
-                                                      DrawCtl* __restrict__ ctl) {
+                                                      DrawCtl* __restrict__ ctl, int seq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ctl->swept != seq) return;  // its sweep did not run (closed gate)
   if (i >= nshots || !owner[i]) return;
   if (ctl->pool_overflow || ctl->label_stuck) return;
   const bool commit = ctl->con == 0;
@@ -1535,6 +1574,7 @@ class Engine final : public IEngine {
     for (int d = 1; d < D; d++)
       if (sup_[d] - inf_[d] < 2 * domain_rad) throw Error("The domain is not sufficiently large!");
     HIP_OK(hipStreamCreate(&st_));
+    HIP_OK(hipStreamCreate(&cst_));
     HIP_OK(hipHostMalloc((void**)&hctr_, sizeof(int) * C_NUM, hipHostMallocDefault));
     dctr_.reserve(C_NUM);
     ring_.reserve(kRing);
@@ -1549,6 +1589,10 @@ class Engine final : public IEngine {
     if (hpub_) (void)hipHostFree(hpub_);
     if (pe0_) (void)hipEventDestroy(pe0_);
     if (pe1_) (void)hipEventDestroy(pe1_);
+    if (cst_) (void)hipStreamSynchronize(cst_);
+    for (auto& c : chunks_) (void)hipEventDestroy(c.ev);
+    for (auto e : free_ev_) (void)hipEventDestroy(e);
+    if (cst_) (void)hipStreamDestroy(cst_);
     if (st_) (void)hipStreamDestroy(st_);
     if (hctr_) (void)hipHostFree(hctr_);
   }
@@ -1591,6 +1635,9 @@ class Engine final : public IEngine {
       throw Error("Incorrect maximal radius");
     }
     r_max_ = std::max(r_max_, gmax_);
+    // the cell store keeps the spheres already placed: size it for every radius
+    store_rmin_ = std::min(store_rmin_, gmin_);
+    grow_cells(cell_cap(store_rmin_, r_max_ > store_rmin_ ? r_max_ : store_rmin_));
   }
 
   void set_miss_rate(double m) override { aimed_ = m; }
@@ -1719,6 +1766,7 @@ class Engine final : public IEngine {
   int gl_ = 1;
   Geo<D> geo_{};
   hipStream_t st_ = nullptr;
+  hipStream_t cst_ = nullptr;  // copies of MT words into the device ring
   int* hctr_ = nullptr;
   DevBuf<int> dctr_;
 
@@ -1781,8 +1829,13 @@ class Engine final : public IEngine {
             std::this_thread::sleep_for(std::chrono::microseconds(20));
             continue;
           }
-          mt.next_block(blk);
-          for (int k = 0; k < MT19937::N; k++) buf[(h + k) & (kRing - 1)] = blk[k];
+          const uint64_t off = h & (kRing - 1);
+          if (off + MT19937::N <= kRing) {
+            mt.next_block(buf + off);
+          } else {  // the block wraps around the ring's end
+            mt.next_block(blk);
+            for (int k = 0; k < MT19937::N; k++) buf[(h + k) & (kRing - 1)] = blk[k];
+          }
           hi.store(h + MT19937::N, std::memory_order_release);
         }
       });
@@ -1816,7 +1869,17 @@ class Engine final : public IEngine {
   };
   HostRing hs_;
   DevBuf<uint32_t> ring_;
-  uint64_t dev_hi_ = 0;  // words [0, dev_hi_) have been copied to the device ring
+  // Words reach the device ring in chunks copied on cst_ ahead of use; st_
+  // waits on a chunk's event once, before the first kernel that reads it.
+  static constexpr uint64_t kChunk = 1ull << 18;  // words per copy (1 MiB)
+  static constexpr uint64_t kAhead = 1ull << 21;  // copy at most this far past the current need
+  struct Chunk {
+    uint64_t start, end;
+    hipEvent_t ev;
+  };
+  std::deque<Chunk> chunks_;  // issued, not yet waited on by st_, in word order
+  std::deque<hipEvent_t> free_ev_;  // reused oldest first
+  uint64_t dev_hi_ = 0;  // words below dev_hi_ are copied or in flight
 
   CellStore<D> cs() { return CellStore<D>{cells_cnt_.p, cells_sph_.p, cells_ph_.p, cap_}; }
 
@@ -1844,6 +1907,9 @@ class Engine final : public IEngine {
     HIP_OK(hipEventSynchronize(pe1_));
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, pe0_, pe1_));
+    prof_add(name, ms * 1e-3, bytes);
+  }
+  void prof_add(const char* name, double seconds, double bytes) {
     KProf* k = nullptr;
     for (auto& e : kprof_)
       if (std::strcmp(e.name, name) == 0) k = &e;
@@ -1852,8 +1918,21 @@ class Engine final : public IEngine {
       k = &kprof_.back();
     }
     k->launches++;
-    k->seconds += ms * 1e-3;
+    k->seconds += seconds;
     k->bytes += bytes;
+  }
+  // host-side time of a step of the draw loop (profiling level >= 1)
+  template <class F>
+  auto host_timed(const char* name, F&& f) {
+    if (!prof_) return f();
+    const auto t0 = std::chrono::steady_clock::now();
+    struct Acc {
+      Engine* e;
+      const char* n;
+      std::chrono::steady_clock::time_point t0;
+      ~Acc() { e->prof_add(n, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), 0); }
+    } acc{this, name, t0};
+    return f();
   }
 
   void define_grid(double rad) {  // rsa_grid_traversal.ixx:11-73
@@ -1919,23 +1998,48 @@ class Engine final : public IEngine {
     cur_ = 0;
   }
 
-  void alloc_cells() {
-    // Spheres per cell.  Disjoint balls of radius r_min centred in the cell lie in
-    // the cell grown by r_min per side (volume bound).  With equal radii, if
-    // the half-cell diagonal is below 2r no two centres share a half-cell, so
-    // at most 2^D (a margin keeps rounding of the collision test harmless).
+  // Spheres per cell for radii in [rmin, rmax].  Disjoint balls of radius rmin
+  // centred in the cell lie in the cell grown by rmin per side (volume bound).
+  // With equal radii, if the half-cell diagonal is below 2r no two centres
+  // share a half-cell, so at most 2^D (a margin keeps rounding of the
+  // collision test harmless).
+  int cell_cap(double rmin, double rmax) const {
     double vol = 1.;
-    for (int d = 0; d < D; d++) vol *= (cl_[d] + 2 * gmin_);
-    const double vb = volume_unit_sphere(D) * puissance<D>(gmin_);
+    for (int d = 0; d < D; d++) vol *= (cl_[d] + 2 * rmin);
+    const double vb = volume_unit_sphere(D) * puissance<D>(rmin);
     int cap = (int)std::ceil(vol / vb);
-    if (gmax_ == gmin_) {
+    if (rmax == rmin) {
       double diag2 = 0.;
       for (int d = 0; d < D; d++) diag2 += 0.25 * cl_[d] * cl_[d];
-      if (std::sqrt(diag2) < 2 * gmin_ * (1 - 1e-9)) cap = std::min(cap, ipow_c(2, D));
+      if (std::sqrt(diag2) < 2 * rmin * (1 - 1e-9)) cap = std::min(cap, ipow_c(2, D));
     }
     cap = std::max(cap, 2);
     if (cap > 255) throw Error("radius ratio too large for the per-cell capacity");
+    return cap;
+  }
+  // grow the per-cell slot arrays to `cap`, keeping every stored sphere
+  void grow_cells(int cap) {
+    if (cap <= cap_) return;
+    HIP_OK(hipStreamSynchronize(st_));
+    DevBuf<Slot<D>> sph;
+    DevBuf<uint32_t> ph;
+    sph.reserve((size_t)geo_.ncells * cap);
+    ph.reserve((size_t)geo_.ncells * cap);
+    HIP_OK(hipMemcpy2D(sph.p, cap * sizeof(Slot<D>), cells_sph_.p, cap_ * sizeof(Slot<D>), cap_ * sizeof(Slot<D>),
+                       geo_.ncells, hipMemcpyDeviceToDevice));
+    HIP_OK(hipMemcpy2D(ph.p, cap * sizeof(uint32_t), cells_ph_.p, cap_ * sizeof(uint32_t), cap_ * sizeof(uint32_t),
+                       geo_.ncells, hipMemcpyDeviceToDevice));
+    std::swap(cells_sph_.p, sph.p);
+    std::swap(cells_sph_.cap, sph.cap);
+    std::swap(cells_ph_.p, ph.p);
+    std::swap(cells_ph_.cap, ph.cap);
     cap_ = cap;
+  }
+  double store_rmin_ = 0;  // smallest radius any class so far may place
+
+  void alloc_cells() {
+    store_rmin_ = gmin_;
+    cap_ = cell_cap(gmin_, gmax_);
     cells_cnt_.reserve(geo_.ncells);
     cells_sph_.reserve((size_t)geo_.ncells * cap_);
     cells_ph_.reserve((size_t)geo_.ncells * cap_);
@@ -1963,20 +2067,70 @@ class Engine final : public IEngine {
     HIP_OK(hipStreamSynchronize(st_));
   }
 
-  // copy words [dev_hi_, upto) of the host ring into the device ring (stream-ordered).
-  // A pinned-ring slot is rewritten kRing words later, long after its copy ran.
-  void upload_words(uint64_t upto) {
-    hs_.ensure(upto);
-    while (dev_hi_ < upto) {
+  // one chunk [dev_hi_, end) of generated words onto cst_ (split at the ring's end)
+  void issue_chunk(uint64_t end) {
+    const uint64_t a = dev_hi_;
+    while (dev_hi_ < end) {
       const uint64_t off = dev_hi_ & (kRing - 1);
-      const uint64_t n = std::min<uint64_t>(upto - dev_hi_, kRing - off);
-      HIP_OK(hipMemcpyAsync(ring_.p + off, hs_.buf + off, n * sizeof(uint32_t), hipMemcpyHostToDevice, st_));
+      const uint64_t n = std::min<uint64_t>(end - dev_hi_, kRing - off);
+      HIP_OK(hipMemcpyAsync(ring_.p + off, hs_.buf + off, n * sizeof(uint32_t), hipMemcpyHostToDevice, cst_));
       dev_hi_ += n;
+    }
+    if (dev_hi_ == a) return;
+    hipEvent_t ev;
+    if (free_ev_.size() > 8) {
+      ev = free_ev_.front();
+      free_ev_.pop_front();
+    } else {
+      HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    HIP_OK(hipEventRecord(ev, cst_));
+    chunks_.push_back({a, dev_hi_, ev});
+  }
+  // before a new stream of words: no copy in flight, nothing copied
+  void reset_dev_ring() {
+    HIP_OK(hipStreamSynchronize(cst_));
+    for (auto& c : chunks_) free_ev_.push_back(c.ev);
+    chunks_.clear();
+    dev_hi_ = 0;
+  }
+  // Make device-ring words [from, upto) readable by the next kernel on st_.
+  // Every word below hs_.low has been consumed, so a copy into the slot of word
+  // p (which held word p - kRing) is safe once p < low + kRing; words at or
+  // above low stay in the host ring until their copy has completed, because the
+  // kernel that reads them waits for it and low only moves past them later.
+  void dev_words(uint64_t from, uint64_t upto) {
+    if (dev_hi_ < from) {  // words skipped on the device (read on the host only)
+      dev_hi_ = from;
+    }
+    while (dev_hi_ < upto) {
+      const uint64_t end = std::min(upto, dev_hi_ + kChunk);
+      hs_.ensure(end);
+      issue_chunk(end);
+    }
+    // run ahead with what the producer has already generated
+    const uint64_t lim = std::min(upto + kAhead, hs_.low.load(std::memory_order_relaxed) + kRing);
+    for (;;) {
+      const uint64_t gen = std::min(hs_.hi.load(std::memory_order_acquire), lim);
+      if (gen < dev_hi_ + kChunk) break;
+      issue_chunk(dev_hi_ + kChunk);
+    }
+    // wait for the last chunk holding words below upto (cst_ completes in order)
+    hipEvent_t wait = nullptr;
+    while (!chunks_.empty() && chunks_.front().start < upto) {
+      if (wait) free_ev_.push_back(wait);
+      wait = chunks_.front().ev;
+      chunks_.pop_front();
+    }
+    if (wait) {
+      HIP_OK(hipStreamWaitEvent(st_, wait, 0));
+      free_ev_.push_back(wait);
     }
   }
 
   void ensure_cand(int n) {
     if ((size_t)n <= ccell_.cap && ccell_.p) return;
+    HIP_OK(hipStreamSynchronize(st_));  // queued kernels may still use the old buffers
     const size_t c = (size_t)std::max(n, 1024) * 5 / 4;
     cpos_.reserve(c * D);
     crad_.reserve(c);
@@ -2120,19 +2274,26 @@ class Engine final : public IEngine {
     uint64_t w_shots = 0;    // stream position of the first shot word
     int end_rel = 0;         // words of the shots' positions
     int buf = 0;             // pinned voxel-word buffer holding the chain
+    int id = 0, seq = 0;     // draw id (gate value) and first sweep's seq, once queued
+    uint64_t miss_base = 0;  // spheres placed before the draw since the miss-rate reference point
+    uint64_t w_end() const { return w_shots + (uint64_t)end_rel + (uint64_t)nshots; }
   };
-  int64_t single_draw(const Prep& pr, uint64_t maximal, bool may_outreach, Prep* spec, int64_t& rounds,
-                      uint64_t& w_next, rsa_gpu_stats* st);
+  void init_pub();
+  void enqueue_round(const Prep& pr, int seq, int gate_id);
+  void enqueue_first(Prep& pr, bool forced, uint64_t miss_base);
+  int64_t finish_draw(const Prep& pr, int64_t& rounds, bool* next_live, bool& gate_open, rsa_gpu_stats* st);
+  int draw_id_ = 0;
   int64_t single_draw_slow(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
                            uint64_t& w_next, rsa_gpu_stats* st);
   int host_chain(int nshots, uint64_t w0, int b);
   Prep prepare(uint64_t w, int b);
   void wait_pub(int seq);
   DevBuf<DrawCtl> dctl_;
-  DrawPub* hpub_ = nullptr;      // pinned, coherent: written by k_sweep_labels
+  DrawPub* hpub_ = nullptr;      // pinned, coherent: kPubSlots slots written by k_sweep_labels
   DrawPub* hpub_dev_ = nullptr;  // its device address
   int seq_ = 0;
   uint32_t* hvpos_[2] = {nullptr, nullptr};  // pinned voxel-word offsets of the shots (double buffer)
+  const uint32_t* hvpos_dev_[2] = {nullptr, nullptr};  // their device addresses
   size_t hvpos_cap_[2] = {0, 0};
   DevBuf<int> next_, cslot_, ovhead_, slab_, pool_;
   DevBuf<uint8_t> owner_;
@@ -2197,16 +2358,27 @@ int Engine<D>::host_chain(int nshots, uint64_t w0, int b) {
   const uint32_t N = (uint32_t)nvox_;
   const uint32_t thr = (uint32_t)(0u - N) % N;
   const uint64_t stride = 2 * D + 1;
-  if ((size_t)nshots > hvpos_cap_[b]) {  // the other buffer may still feed a copy; this one is idle
+  if ((size_t)nshots > hvpos_cap_[b]) {  // buffer b is idle (a queued draw may read the other one)
     if (hvpos_[b]) HIP_OK(hipHostFree(hvpos_[b]));
     hvpos_cap_[b] = (size_t)nshots * 5 / 4 + 1024;
-    HIP_OK(hipHostMalloc((void**)&hvpos_[b], hvpos_cap_[b] * sizeof(uint32_t), hipHostMallocDefault));
+    HIP_OK(hipHostMalloc((void**)&hvpos_[b], hvpos_cap_[b] * sizeof(uint32_t),
+                         hipHostMallocCoherent | hipHostMallocMapped));
+    HIP_OK(hipHostGetDevicePointer((void**)&hvpos_dev_[b], hvpos_[b], 0));
   }
   uint32_t* out = hvpos_[b];
-  hs_.ensure(w0 + (uint64_t)nshots * stride + 64);
+  uint64_t avail = w0 + (uint64_t)nshots * stride + 64;  // rejections are rare: extend when passed
+  hs_.ensure(avail);
+  const uint32_t* buf = hs_.buf;
   uint64_t p = w0;
   for (int i = 0; i < nshots; i++) {
-    while ((uint32_t)((uint64_t)hs_.at(p) * (uint64_t)N) < thr) p++;
+    for (;;) {
+      if (p >= avail) {
+        avail = p + (uint64_t)(nshots - i) * stride + 64;
+        hs_.ensure(avail);
+      }
+      if ((uint32_t)((uint64_t)buf[p & (kRing - 1)] * (uint64_t)N) >= thr) break;
+      p++;
+    }
     out[i] = (uint32_t)(p - w0);
     p += stride;
   }
@@ -2249,9 +2421,9 @@ typename Engine<D>::Prep Engine<D>::prepare(uint64_t w, int b) {
 template <int D>
 void Engine<D>::wait_pub(int seq) {
   const auto t0 = std::chrono::steady_clock::now();
-  volatile int* s = &hpub_->seq;
+  volatile int* s = &hpub_[seq & (kPubSlots - 1)].seq;
   while (*s != seq) {
-    if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60)) {
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) {
       (void)hipStreamSynchronize(st_);
       if (*s == seq) break;
       throw Error("internal: draw results not published");
@@ -2264,89 +2436,117 @@ void Engine<D>::wait_pub(int seq) {
 // for draws that cannot overshoot the class target: five kernels per round and
 // one host round trip (k_sweep_labels publishes the round's counts to pinned
 // host memory; the tag update and commit run while the host moves on).
+// enqueue_first queues a draw's first round (gated unless forced);
+// finish_draw waits for its rounds and queues the further ones.
 template <int D>
-int64_t Engine<D>::single_draw(const Prep& pr, uint64_t maximal, bool may_outreach, Prep* spec, int64_t& rounds,
-                               uint64_t& w_next, rsa_gpu_stats* st) {
+void Engine<D>::init_pub() {
+  if (hpub_) return;
+  dctl_.reserve(1);
+  HIP_OK(hipMemset(dctl_.p, 0, sizeof(DrawCtl)));
+  HIP_OK(hipHostMalloc((void**)&hpub_, sizeof(DrawPub) * kPubSlots, hipHostMallocCoherent | hipHostMallocMapped));
+  HIP_OK(hipHostGetDevicePointer((void**)&hpub_dev_, hpub_, 0));
+  std::memset(hpub_, 0, sizeof(DrawPub) * kPubSlots);
+  pool_.reserve(1 << 16);
+}
+
+template <int D>
+void Engine<D>::enqueue_round(const Prep& pr, int seq, int gate_id) {
+  const int nshots = pr.nshots;
+  using LG = LaneGroup<D>;
+  const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
+  const unsigned nbs = nblocks(nshots);
+  timed("k_partner_lists", (double)nshots * (8.0 * (D + 1) + 10.0), [&] {
+    hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, cpos_.p, crad_.p, cprio_.p,
+                       ccell_.p, ctag_.p, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, poff_.p, pcnt_.p, slab_.p,
+                       pool_.p, (int)pool_.cap, dctl_.p, gate_id);
+  });
+  timed("k_sweep_labels", (double)nshots * 4.0, [&] {
+    hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(1024), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, poff_.p,
+                       pcnt_.p, slab_.p, pool_.p, dctl_.p, hpub_dev_, seq, gate_id, pr.id, aimed_, pr.miss_base);
+  });
+  timed("k_update_commit", (double)nshots * (8.0 * (D + 1) + 12.0), [&] {
+    hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
+                       ccell_.p, owner_.p, ctag_.p, lab_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, seq);
+  });
+  HIP_OK(hipGetLastError());
+}
+
+template <int D>
+void Engine<D>::enqueue_first(Prep& pr, bool forced, uint64_t miss_base) {
+  const auto tq = std::chrono::steady_clock::now();
   const int nshots = pr.nshots;
   const uint64_t w0 = pr.w_shots;
-  if (may_outreach || nshots > 48000) return single_draw_slow(nshots, maximal, may_outreach, w0, rounds, w_next, st);
-  ensure_cand(nshots);
-  if (!hpub_) {
-    dctl_.reserve(1);
-    HIP_OK(hipMemset(dctl_.p, 0, sizeof(DrawCtl)));
-    HIP_OK(hipHostMalloc((void**)&hpub_, sizeof(DrawPub), hipHostMallocCoherent | hipHostMallocMapped));
-    HIP_OK(hipHostGetDevicePointer((void**)&hpub_dev_, hpub_, 0));
-    std::memset(hpub_, 0, sizeof(DrawPub));
-    pool_.reserve(1 << 16);
-  }
-  const int end_rel = pr.end_rel;
-  upload_words(w0 + (uint64_t)end_rel + (uint64_t)nshots);
-  HIP_OK(hipMemcpyAsync(vpos_.p, hvpos_[pr.buf], (size_t)nshots * sizeof(uint32_t), hipMemcpyHostToDevice, st_));
+  pr.id = ++draw_id_;
+  pr.seq = ++seq_;
+  pr.miss_base = miss_base;
+  // k_decode_filter reads the shot words from the device ring and the chain
+  // offsets from pinned host memory (written before this launch, untouched
+  // until the draw's results have been published).
+  dev_words(w0, w0 + (uint64_t)pr.end_rel + (uint64_t)nshots);
   const uint32_t N = (uint32_t)nvox_;
   ClassDev cls{std::get<0>(classes_[cur_]), (uint32_t)std::get<2>(classes_[cur_])};
   using LG = LaneGroup<D>;
   const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
   const unsigned nbs = nblocks(nshots);
   timed("k_decode_filter", (double)nshots * (4.0 * (2 * D + 2) + 4.0 * D + 8.0 * (D + 1) + 13.0), [&] {
-    hipLaunchKernelGGL(k_decode_filter<D>, dim3(nbl), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N, vpos_.p,
-                       end_rel, vview(), cls, geo_, cs(), cpos_.p, crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p,
-                       owner_.p, cellcc_.p, cslot_.p, ovhead_.p, next_.p, dctl_.p);
+    hipLaunchKernelGGL(k_decode_filter<D>, dim3(nbl), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N,
+                       hvpos_dev_[pr.buf], pr.end_rel, vview(), cls, geo_, cs(), cpos_.p, crad_.p, cprio_.p,
+                       cphase_.p, ccell_.p, ctag_.p, owner_.p, cellcc_.p, cslot_.p, ovhead_.p, next_.p, dctl_.p, pr.id,
+                       forced ? 1 : 0);
   });
   timed("k_lists", (double)nshots * 24.0, [&] {
     hipLaunchKernelGGL(k_lists, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, owner_.p, ctag_.p, cellcc_.p, cslot_.p,
-                       ovhead_.p, next_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p);
+                       ovhead_.p, next_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, pr.id);
   });
-  auto round = [&](int seq) {
-    timed("k_partner_lists", (double)nshots * (8.0 * (D + 1) + 10.0), [&] {
-      hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, cpos_.p, crad_.p, cprio_.p,
-                         ccell_.p, ctag_.p, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, poff_.p, pcnt_.p, slab_.p,
-                         pool_.p, (int)pool_.cap, dctl_.p);
-    });
-    timed("k_sweep_labels", (double)nshots * 4.0, [&] {
-      hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(1024), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, poff_.p,
-                         pcnt_.p, slab_.p, pool_.p, dctl_.p, hpub_dev_, seq);
-    });
-    timed("k_update_commit", (double)nshots * (8.0 * (D + 1) + 12.0), [&] {
-      hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
-                         ccell_.p, owner_.p, ctag_.p, lab_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p);
-    });
-    HIP_OK(hipGetLastError());
-  };
-  int seq = ++seq_;
-  round(seq);
-  w_next = w0 + (uint64_t)end_rel + (uint64_t)nshots;
-  // While the GPU runs the first round: the next draw's Poisson count and
-  // rejection chain, valid if no refinement changes the voxel count.  Its chain
-  // goes to the other pinned buffer, whose last copy ran before this draw.
-  if (spec) *spec = prepare(w_next, pr.buf ^ 1);
+  enqueue_round(pr, pr.seq, pr.id);
+  if (prof_)
+    prof_add("host_enqueue_draw", std::chrono::duration<double>(std::chrono::steady_clock::now() - tq).count(), 0);
+}
+
+// Waits for the rounds of draw pr (its first round queued), queues its further
+// rounds; any kernel queued after the first round makes a queued next draw moot
+// (its gate stays closed), which *next_live reports.
+template <int D>
+int64_t Engine<D>::finish_draw(const Prep& pr, int64_t& rounds, bool* next_live, bool& gate_open,
+                               rsa_gpu_stats* st) {
+  const int nshots = pr.nshots;
+  using LG = LaneGroup<D>;
+  const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
+  const unsigned nbs = nblocks(nshots);
+  int seq = pr.seq;
   int64_t local_new = 0;
   int r = 0;
   for (;;) {
-    wait_pub(seq);
-    const DrawPub h = *hpub_;
+    host_timed("host_wait_pub", [&] { wait_pub(seq); });
+    const DrawPub h = hpub_[seq & (kPubSlots - 1)];
     if (h.label_stuck) throw Error("internal: conflict labelling made no progress");
+    if (h.cell_overflow) throw Error("internal: cell capacity exceeded");
     st->ghost_cell_drops += h.dropped;
     st->image_cell_mismatch += h.img_mismatch;
     if (h.pool_overflow) {  // long partner lists: grow the pool and redo the round
+      *next_live = false;
+      HIP_OK(hipStreamSynchronize(st_));  // nothing queued may hold the old pool
       pool_.reserve((size_t)h.need_pool * 2);
       HIP_OK(hipMemsetAsync(&dctl_.p->pool_cursor, 0, 3 * sizeof(int), st_));
       seq = ++seq_;
-      round(seq);
+      enqueue_round(pr, seq, -1);
       continue;
     }
     local_new += h.acc;
     if (h.con == 0) {
       rounds = r + 1;
+      gate_open = h.gate_open != 0;
       break;
     }
     if (++r >= 100000) throw Error("too many conflict rounds in one draw");
+    *next_live = false;
     timed("k_refilter", (double)nshots * (8.0 * (D + 1) + 5.0), [&] {
       hipLaunchKernelGGL(k_refilter<D>, dim3(nbl), dim3(WG), 0, st_, geo_, cs(), nshots, cpos_.p, crad_.p, ccell_.p,
                          ctag_.p, cell2t_.p, seg_.p, len_.p, ord_.p);
     });
     hipLaunchKernelGGL(k_remove_skips, dim3(nbs), dim3(WG), 0, st_, nshots, owner_.p, seg_.p, len_.p, ord_.p, ctag_.p);
     seq = ++seq_;
-    round(seq);
+    enqueue_round(pr, seq, -1);
   }
   return local_new;
 }
@@ -2362,7 +2562,7 @@ int64_t Engine<D>::single_draw_slow(int nshots, uint64_t maximal, bool may_outre
   int nwin = nshots * (2 * D + 1) + std::max(1024, nshots / 4);
   const unsigned nbs = nblocks(nshots);
   for (;;) {  // word window: grows only if the rejections overflow it
-    upload_words(w0 + (uint64_t)nwin + (uint64_t)nshots + 64);
+    dev_words(w0, w0 + (uint64_t)nwin + (uint64_t)nshots + 64);
     reset_ctr();
     const size_t shmem = sizeof(uint32_t) * ((nwin + 31) / 32 + 1);
     if (shmem > 60000) throw Error("word window too large for one workgroup");
@@ -2375,7 +2575,7 @@ int64_t Engine<D>::single_draw_slow(int nshots, uint64_t maximal, bool may_outre
     nwin *= 2;
   }
   const int end_rel = hctr_[C_END_REL];
-  upload_words(w0 + (uint64_t)end_rel + (uint64_t)nshots);
+  dev_words(w0, w0 + (uint64_t)end_rel + (uint64_t)nshots);
   ClassDev cls{std::get<0>(classes_[cur_]), (uint32_t)std::get<2>(classes_[cur_])};
   hipLaunchKernelGGL(k_decode<D>, dim3(nbs), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N, vpos_.p,
                      vview(), cls, geo_, cpos_.p, crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p, dctr_.p);
@@ -2468,13 +2668,14 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
   if (!algo_ready_) throw Error("rsa_algo not initialised");
   const auto t0 = std::chrono::steady_clock::now();
   std::memset(st, 0, sizeof(*st));
+  reset_dev_ring();  // no copy may read the host ring once the producer restarts
   hs_.start(seed);
   struct Halt {
     HostRing& h;
     ~Halt() { h.halt(); }
   } halt_at_exit{hs_};
-  dev_hi_ = 0;
-  Prep spec;
+  Prep cur;               // the next draw's inputs (valid while cur.nv == nvox_ and cur.w_start == w)
+  bool cur_live = false;  // its first round is already queued (behind the previous draw)
   int nextbuf = 0;
   uint64_t w = 0;
   init_voxels(r_max_ + gmin_);
@@ -2494,27 +2695,47 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
       st->end_reason = 1;
       break;
     }
-    // every copy enqueued so far has completed (each draw and refinement ends
-    // with a host wait on the stream): the producer may reuse older ring slots
-    hs_.set_low(std::min(w, dev_hi_));
-    Prep pr;
-    if (spec.nv == nvox_ && spec.w_start == w) {
-      pr = spec;
-    } else {
-      pr = prepare(w, nextbuf);
+    // every kernel and copy that read ring words below w has completed (the
+    // previous draw's results were waited for): the producer may reuse them
+    hs_.set_low(w);
+    if (cur.nv != nvox_ || cur.w_start != w) {
+      if (cur_live) throw Error("internal: a queued draw lost its inputs");
+      cur = host_timed("host_prepare", [&] { return prepare(w, nextbuf); });
     }
-    nextbuf = pr.buf ^ 1;
-    spec.nv = -1;
-    st->loop_iterations += pr.zero_iters;  // the iterations whose Poisson draw gave no shot
-    const uint64_t nshots = (uint64_t)pr.nshots;
-    const bool may_outreach = nshots >= current_number();
+    nextbuf = cur.buf ^ 1;
+    st->loop_iterations += cur.zero_iters;  // the iterations whose Poisson draw gave no shot
+    const uint64_t nshots = (uint64_t)cur.nshots;
     const uint64_t maximal = current_number();
+    const bool may_outreach = nshots >= maximal;
     int64_t rounds = 0;
-    uint64_t w_next = 0;
-    const bool may_continue = max_draws < 0 || st->draws + 1 < max_draws;
-    const uint64_t nb_new =
-        (uint64_t)single_draw(pr, maximal, may_outreach, may_continue ? &spec : nullptr, rounds, w_next, st);
-    w = w_next;
+    uint64_t nb_new = 0;
+    bool fast = false, gate_open = false;
+    Prep nxt;
+    bool nxt_live = false;
+    if (may_outreach || nshots > 48000) {
+      if (cur_live) throw Error("internal: a queued draw needs the host-driven form");
+      uint64_t w_next = 0;
+      nb_new = (uint64_t)single_draw_slow(cur.nshots, maximal, may_outreach, cur.w_shots, rounds, w_next, st);
+    } else {
+      fast = true;
+      init_pub();
+      if (!cur_live) {
+        ensure_cand(cur.nshots);
+        enqueue_first(cur, true, (uint64_t)(int)nb_fast_ - old_nb);
+      }
+      // Queue the next draw behind this one.  Its inputs hold unless this draw
+      // triggers a refinement (the device then keeps its gate closed); it cannot
+      // change the class or overshoot its target: shots(cur) + shots(nxt) < maximal.
+      if (max_draws < 0 || st->draws + 2 <= max_draws) {
+        nxt = host_timed("host_prepare_next", [&] { return prepare(cur.w_end(), cur.buf ^ 1); });
+        if (nxt.nshots <= 48000 && (size_t)nxt.nshots <= ccell_.cap && nshots + (uint64_t)nxt.nshots < maximal) {
+          enqueue_first(nxt, false, 0);  // the miss count restarts after each draw
+          nxt_live = true;
+        }
+      }
+      nb_new = (uint64_t)finish_draw(cur, rounds, &nxt_live, gate_open, st);
+    }
+    w = cur.w_end();
     st->draws++;
     st->shots += (int64_t)nshots;
     st->accepted += (int64_t)nb_new;
@@ -2527,7 +2748,10 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     const uint64_t nb_miss = nshots - (uint64_t)(new_nb - old_nb);
     old_nb = (uint64_t)new_nb;
     const double miss_rate = nb_miss / (0.000001 + nshots);
-    if (miss_rate > aimed_) {
+    const bool refine_now = miss_rate > aimed_;
+    if (fast && gate_open == refine_now) throw Error("internal: device and host refinement decisions differ");
+    if (refine_now) {
+      nxt_live = false;  // its gate stayed closed
       HIP_OK(hipEventRecord(e0, st_));
       refine(gmin_);
       HIP_OK(hipEventRecord(e1, st_));
@@ -2539,9 +2763,12 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
       st->nvox_max = std::max<int64_t>(st->nvox_max, nvox_);
     }
     if (!cont) {
+      if (nxt_live) throw Error("internal: a draw was queued past the last class");
       st->end_reason = 2;
       break;
     }
+    cur = nxt;
+    cur_live = nxt_live;
   }
   HIP_OK(hipStreamSynchronize(st_));
   (void)hipEventDestroy(e0);

@@ -11,6 +11,9 @@
 
 namespace rsa_amd {
 
+// twist state x[624] once and write the 624 tempered words to out (mt19937.cpp)
+void mt_block(uint32_t* x, uint32_t* out);
+
 struct MT19937 {
   static constexpr int N = 624, M = 397;
   static constexpr uint32_t MATRIX_A = 0x9908b0dfu, UPPER = 0x80000000u, LOWER = 0x7fffffffu;
@@ -59,21 +62,6 @@ struct MT19937 {
     for (int i = 1; i < N; i++) x[i] = 1812433253u * (x[i - 1] ^ (x[i - 1] >> 30)) + (uint32_t)i;
   }
 
-  // one twist of the whole state
-  void twist() {
-    int k = 0;
-    for (; k < N - M; k++) {
-      const uint32_t y = (x[k] & UPPER) | (x[k + 1] & LOWER);
-      x[k] = x[k + M] ^ (y >> 1) ^ ((y & 1u) ? MATRIX_A : 0u);
-    }
-    for (; k < N - 1; k++) {
-      const uint32_t y = (x[k] & UPPER) | (x[k + 1] & LOWER);
-      x[k] = x[k + (M - N)] ^ (y >> 1) ^ ((y & 1u) ? MATRIX_A : 0u);
-    }
-    const uint32_t y = (x[N - 1] & UPPER) | (x[0] & LOWER);
-    x[N - 1] = x[M - 1] ^ (y >> 1) ^ ((y & 1u) ? MATRIX_A : 0u);
-  }
-
   static inline uint32_t temper(uint32_t z) {
     z ^= (z >> 11);
     z ^= (z << 7) & 0x9d2c5680u;
@@ -83,10 +71,7 @@ struct MT19937 {
   }
 
   // next 624 output words (the engine twists before its first output)
-  void next_block(uint32_t* out) {
-    twist();
-    for (int i = 0; i < N; i++) out[i] = temper(x[i]);
-  }
+  void next_block(uint32_t* out) { mt_block(x, out); }
 };
 
 // A growing window of the stream: words [base, base + size) are kept.

@@ -103,8 +103,13 @@ def run(dim, inf, sup, classes, seed=0, mode=1, nbshots=6000, max_draws=-1, drad
             outr = L.orc_dump(h, k, _d(dp), _d(dr), dpr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                               da.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
             dumps.append(dict(pos=dp.reshape(m, dim), rad=dr, prio=dpr, accepted=da.astype(bool), outreach=outr))
+        nvv = L.orc_num_voxel_values(h)
+        vox = np.zeros(nvv, dtype=np.int64)
+        vlen = np.zeros(dim)
+        L.orc_voxels(h, vox.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _d(vlen))
         return dict(n=n, pos=pos.reshape(n, dim), rad=r, phase=phs, trace=tr, miss=miss,
-                    msg=L.orc_end_message(h).decode(), words=int(L.orc_words_used(h)), dumps=dumps)
+                    msg=L.orc_end_message(h).decode(), words=int(L.orc_words_used(h)), dumps=dumps,
+                    vox=vox.reshape(-1, dim), vox_len=vlen)
     finally:
         L.orc_free(h)
 

@@ -88,3 +88,59 @@ def test_cfg2_full_size_matches_golden():
     assert periodic_overlaps(g["pos"], g["rad"], [1.0] * 3) == 0
     assert uncovered_samples(g["pos"], g["rad"], [1.0] * 3, 0.005, n=200000) == 0
     assert 0.38 < volume_fraction(g["rad"], 3, 1.0) < 0.39
+
+
+@pytest.mark.parametrize("dim,L,r,seed", [(2, 1.0, 0.01, 0), (3, 0.2, 0.005, 1)])
+def test_stop_after_k_draws_bit_exact(dim, L, r, seed):
+    """proceed(seed, max_draws=k): spheres, stream position and the voxel list
+    where the run stopped, for k around the refinements (operator_algorithm.ixx:92-95)."""
+    cls = [(r, 1.0, 0)]
+    tr = orc_run(dim, L, cls, seed=seed)["trace"]
+    ref = np.nonzero(tr[:, 5])[0]
+    ks = sorted({1, 2, 3, int(ref[0]), int(ref[0]) + 1, int(ref[len(ref) // 2]) + 1, len(tr) - 1})
+    for k in ks:
+        print("max_draws", k, flush=True)
+        g = gpu_run(dim, L, cls, seed=seed, max_draws=k)
+        o = orc_run(dim, L, cls, seed=seed, max_draws=k)
+        assert_same(g, o)
+        st = g["stats"]
+        assert st["draws"] == k and st["end_reason"] == 3
+        assert st["words"] == o["words"]
+        co, ln = g["dom"].voxels()
+        assert np.array_equal(co, o["vox"])
+        assert np.array_equal(ln.view(np.uint64), o["vox_len"].view(np.uint64))
+
+
+@pytest.mark.parametrize("dim,L,classes,seed", [
+    (2, 1.0, [(0.02, 60, 0), (0.01, 500, 1), (0.005, 4000, 2)], 0),
+    (2, 0.5, [(0.01, 300, 3), (0.004, 500, 4)], 5),
+    (3, 0.2, [(0.01, 300, 0), (0.005, 3000, 1)], 2),
+])
+def test_count_classes_bit_exact(dim, L, classes, seed):
+    """(radius, count, phase) classes placed largest first: draws that may
+    overshoot a class target keep the first ones by priority
+    (operator_algorithm.ixx:416-421), then the next class starts."""
+    g = gpu_run(dim, L, classes, seed=seed, mode=0)
+    o = orc_run(dim, L, classes, seed=seed, mode=0)
+    assert_same(g, o)
+    assert g["stats"]["words"] == o["words"]
+    for rad, cnt, ph in classes:
+        got = int((g["phase"] == ph).sum())
+        assert got <= cnt
+        assert np.all(g["rad"][g["phase"] == ph] == rad)
+
+
+@pytest.mark.parametrize("dim,L,r0,reiter,seed", [(3, 0.2, 0.01, (0.005,), 4), (2, 1.0, 0.02, (0.01, 0.005), 1)])
+def test_reiteration_bit_exact(dim, L, r0, reiter, seed):
+    """reset_radius_generator + proceed(seed) again on the same domain
+    (scripts/main_max_frac.cpp:43-52): the new class fills the gaps."""
+    cls = [(r0, 1.0, 0)]
+    g = gpu_run(dim, L, cls, seed=seed, reiter=reiter)
+    o = orc_run(dim, L, cls, seed=seed, reiter=reiter)
+    assert_same(g, o)
+    assert periodic_overlaps_ok(g, dim, L)
+
+
+def periodic_overlaps_ok(g, dim, L):
+    from _common import periodic_overlaps
+    return periodic_overlaps(g["pos"], g["rad"], [L] * dim) == 0
