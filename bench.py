@@ -59,7 +59,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-side", type=float, default=0.8)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (reference reads for rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU only: this build runs one rank's whole domain on a single MI355X "
                  "(multi-GPU sharding is a design in DESIGN.md, not a measured path).")
@@ -116,7 +129,10 @@ def main():
                    "draws": stats["draws"], "refines": stats["refines"], "nvox_max": stats["nvox_max"],
                    "mt_words": stats["words"], "parity_vs_oracle": parity},
         "roofline": roofline,
-        "kernels": {kk: dict(launches=v["launches"] // args.steps, ms=round(1e3 * v["seconds"] / args.steps, 3))
+        "phases_ms": {"voxel_refine": round(1e3 * stats["voxel_seconds"], 2),
+                      "draws": round(1e3 * (stats["seconds"] - stats["voxel_seconds"]), 2),
+                      "rounds": stats["rounds"], "rounds_max": stats["rounds_max"]},
+        "profile": {kk: dict(launches=v["launches"] // args.steps, ms=round(1e3 * v["seconds"] / args.steps, 3))
                     for kk, v in prof.items()},
         "kernels_all_bracketed": dict(step_ms=round(1e3 * dt2, 1), **detail),
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_side),

@@ -56,6 +56,9 @@ struct Geo {
   // (offset_d + 1) in bits 2d..2d+1; device memory, so that a lane-dependent
   // entry is a cached load rather than a private copy of this struct
   const int* stab;
+  // the same offsets as linear cell-index differences (valid where the whole
+  // stencil lies inside the grid)
+  const int* slin;
 };
 
 template <int D>

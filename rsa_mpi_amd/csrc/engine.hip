@@ -1223,6 +1223,12 @@ struct VoxParams {
   // corner set {tab[i] * len} is the product over d of these values.
   unsigned char cm[D];
   double rmin;
+  // h2 = sum_d (len_d / 2)^2 when every dimension has both corner values, else
+  // a negative sentinel.  Then the farthest corner of a box is at squared
+  // distance >= |c - centre|^2 + h2 from any point c (per dimension
+  // max(a^2, b^2) >= (a^2 + b^2) / 2 = (delta - len/2)^2 + len^2/4), so a sphere
+  // with |c - centre|^2 > R^2 - h2 cannot cover the box.
+  double h2;
 };
 
 // auxi::is_in_sphere (list_of_voxels.ixx:225-238) asks whether every corner
@@ -1266,6 +1272,43 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
     const int64_t cellb = cell_index<D>(g, cb);
     uint32_t ccov = 0;
     bool covered = false;
+    // One sphere (centre qp, radius qr) against the voxel and its children.
+    // Coverage and each child's flag are ORs over spheres: visiting order is free.
+    auto test = [&](const double* qp, double qr, bool do_cov, bool do_child) {
+      const double rr = qr + vp.rmin;
+      const double R2 = rr * rr;
+      double s2 = 0.;  // get_list_of_spheres pre-filter (:297-298), from the subdivide centre
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const double t = qp[d] - cb[d];
+        s2 += t * t;
+      }
+      // necessary for coverage (vp.h2 above); the 1e-9 R^2 slack absorbs rounding
+      if (do_cov && (vp.h2 < 0. || s2 <= R2 - vp.h2 + 1e-9 * R2)) {
+        double sf = 0.;
+#pragma unroll
+        for (int d = 0; d < D; d++) sf += far_term(qp[d] - ov[d], vp.len[d], vp.cm[d]);
+        if (!(sf > R2)) {
+          covered = true;
+          return;
+        }
+      }
+      if (do_child && s2 < R2 && ccov != FULL) {
+        double f0[D], f1[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          f0[d] = far_term(qp[d] - ok0[d], vp.nl[d], vp.cm[d]);
+          f1[d] = far_term(qp[d] - ok1[d], vp.nl[d], vp.cm[d]);
+        }
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) {
+          double sf = 0.;
+#pragma unroll
+          for (int d = 0; d < D; d++) sf += vp.tab[ch][d] ? f1[d] : f0[d];
+          if (!(sf > R2)) ccov |= (1u << ch);
+        }
+      }
+    };
     // pass 0: coverage around cella (and children when cellb == cella); pass 1: children around cellb
     for (int pass = 0; pass < 2 && !covered; pass++) {
       const int64_t cell = pass == 0 ? cella : cellb;
@@ -1274,12 +1317,27 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
       if (!do_cov && !do_child) break;
       int cc[D];
       cell_coords<D>(g, cell, cc);
+      bool interior = true;  // the whole stencil lies in real cells: plain index offsets
+#pragma unroll
+      for (int d = 0; d < D; d++) interior = interior && cc[d] > g.gl && cc[d] < g.n[d] - g.gl - 1;
+      if (interior) {
+        for (int s = 0; s < g.nst && !covered; s++) {
+          const int src = (int)cell + g.slin[s];
+          const int nv = cs.cnt[src];
+          const Slot<D>* sl = cs.sph + (size_t)src * cs.cap;
+          for (int k = 0; k < nv && !covered; k++) {
+            const Slot<D> q = sl[k];
+            test(q.c, q.r, do_cov, do_child);
+          }
+        }
+        continue;
+      }
       for (int s = 0; s < g.nst && !covered; s++) {
         Nb<D> nb;
         if (!neighbour<D>(g, cc, s, nb)) continue;
         const int nv = cs.cnt[nb.src];
         const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
-        for (int k = 0; k < nv; k++) {
+        for (int k = 0; k < nv && !covered; k++) {
           const Slot<D> q = sl[k];
           double qp[D];
           if (nb.ghost) {
@@ -1288,40 +1346,7 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
 #pragma unroll
             for (int d = 0; d < D; d++) qp[d] = q.c[d];
           }
-          const double rr = q.r + vp.rmin;
-          const double R2 = rr * rr;
-          if (do_cov) {
-            double sf = 0.;
-#pragma unroll
-            for (int d = 0; d < D; d++) sf += far_term(qp[d] - ov[d], vp.len[d], vp.cm[d]);
-            if (!(sf > R2)) {
-              covered = true;
-              break;
-            }
-          }
-          if (do_child && ccov != FULL) {
-            double s2 = 0.;  // get_list_of_spheres pre-filter (:297-298)
-#pragma unroll
-            for (int d = 0; d < D; d++) {
-              const double t = qp[d] - cb[d];
-              s2 += t * t;
-            }
-            if (s2 < R2) {
-              double f0[D], f1[D];
-#pragma unroll
-              for (int d = 0; d < D; d++) {
-                f0[d] = far_term(qp[d] - ok0[d], vp.nl[d], vp.cm[d]);
-                f1[d] = far_term(qp[d] - ok1[d], vp.nl[d], vp.cm[d]);
-              }
-#pragma unroll
-              for (int ch = 0; ch < NCH; ch++) {
-                double sf = 0.;
-#pragma unroll
-                for (int d = 0; d < D; d++) sf += vp.tab[ch][d] ? f1[d] : f0[d];
-                if (!(sf > R2)) ccov |= (1u << ch);
-              }
-            }
-          }
+          test(qp, q.r, do_cov, do_child);
         }
       }
     }
@@ -1847,10 +1872,14 @@ class Engine final : public IEngine {
       }
     }
     void set_low(uint64_t v) { low.store(v, std::memory_order_release); }
+    double wait_s = 0;  // time consumers spent waiting for the generator
     void ensure(uint64_t upto) {
       if (upto > low.load(std::memory_order_relaxed) + kRing) throw Error("word window larger than the ring");
+      if (hi.load(std::memory_order_acquire) >= upto) return;
+      const auto t0 = std::chrono::steady_clock::now();
       while (hi.load(std::memory_order_acquire) < upto) {
       }
+      wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     uint32_t at(uint64_t pos) {
       if (hi.load(std::memory_order_acquire) <= pos) ensure(pos + 1);
@@ -1982,9 +2011,21 @@ class Engine final : public IEngine {
       for (int d = 0; d < D; d++) v |= ((int)vec[order[i]][d] + 1) << (2 * d);
       packed[i] = v;
     }
-    stab_.reserve(n3);
+    // linear index differences: cell index = c0 + n0 * (c1 + n1 * (c2 + ...))
+    std::vector<int> lin(n3);
+    for (int i = 0; i < n3; i++) {
+      int64_t off = 0, stride = 1;
+      for (int d = 0; d < D; d++) {
+        off += (int64_t)vec[order[i]][d] * stride;
+        stride *= geo_.n[d];
+      }
+      lin[i] = (int)off;
+    }
+    stab_.reserve(2 * n3);
     HIP_OK(hipMemcpy(stab_.p, packed.data(), n3 * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(stab_.p + n3, lin.data(), n3 * sizeof(int), hipMemcpyHostToDevice));
     geo_.stab = stab_.p;
+    geo_.slin = stab_.p + n3;
   }
 
   void set_classes(const std::vector<RadiusClass>& cls, double exclusion) {  // radius_generator.ixx:9-28
@@ -2215,6 +2256,10 @@ class Engine final : public IEngine {
     }
     if (distinct != prod) throw Error("internal: TabCorner rows are not a product set");
     vp.rmin = rmin;
+    vp.h2 = 0.;
+    for (int d = 0; d < D; d++) vp.h2 += (0.5 * vp.len[d]) * (0.5 * vp.len[d]);
+    for (int d = 0; d < D; d++)
+      if (vp.cm[d] != 3) vp.h2 = -1.;
     return vp;
   }
 
@@ -2297,7 +2342,7 @@ class Engine final : public IEngine {
   size_t hvpos_cap_[2] = {0, 0};
   DevBuf<int> next_, cslot_, ovhead_, slab_, pool_;
   DevBuf<uint8_t> owner_;
-  DevBuf<int> stab_;  // packed stencil offsets (Geo::stab)
+  DevBuf<int> stab_;  // packed stencil offsets (Geo::stab), then linear ones (Geo::slin)
 };
 
 // median_of_medians::find_pivot_for_first_elements on one rank (median_of_medians.ixx:8-178)
@@ -2670,6 +2715,7 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
   std::memset(st, 0, sizeof(*st));
   reset_dev_ring();  // no copy may read the host ring once the producer restarts
   hs_.start(seed);
+  hs_.wait_s = 0;
   struct Halt {
     HostRing& h;
     ~Halt() { h.halt(); }
@@ -2770,6 +2816,7 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     cur = nxt;
     cur_live = nxt_live;
   }
+  if (prof_) prof_add("host_wait_mt_words", hs_.wait_s, 0);
   HIP_OK(hipStreamSynchronize(st_));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

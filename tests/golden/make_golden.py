@@ -21,6 +21,12 @@ from _common import set_hash  # noqa: E402
 CASES = {
     # 3D monodisperse r=0.005 in the periodic unit cube, seed 0 (the bench workload)
     "cfg2_3d_r0.005_seed0": dict(dim=3, L=1.0, classes=[(0.005, 1.0, 0)], seed=0),
+    # 3D monodisperse r=0.002 in the periodic unit cube, seed 0 (~11.5M spheres)
+    "cfg5a_3d_r0.002_seed0": dict(dim=3, L=1.0, classes=[(0.002, 1.0, 0)], seed=0),
+    # 3D polydisperse: classes r=0.01/0.005/0.0025 at volume fraction 0.10 each,
+    # then fill with r=0.0025 (class order as given; equal radii keep their order)
+    "cfg3_3d_poly_seed0": dict(dim=3, L=1.0, classes=[(0.01, 0.10, 0), (0.005, 0.10, 1), (0.0025, 0.10, 2),
+                                                     (0.0025, 1.0, 3)], seed=0),
 }
 
 
@@ -35,12 +41,15 @@ def main(names):
         c = CASES[name]
         t = time.time()
         o = O.run(c["dim"], [0.0] * c["dim"], [c["L"]] * c["dim"], c["classes"], seed=c["seed"])
+        if "classes" in c:
+            c = dict(c, classes=[list(x) for x in c["classes"]])
         dt = time.time() - t
         tr = o["trace"]
         out = dict(case=c, n=int(o["n"]), sha256=set_hash(o["pos"], o["rad"]), words=o["words"],
                    draws=int(len(tr)), shots=int(tr[:, 2].sum()), refines=int(tr[:, 5].sum()),
                    rounds=int(tr[:, 4].sum()), nvox_max=int(max(tr[:, 1].max(), tr[:, 6].max())),
-                   end=o["msg"], oracle_seconds=round(dt, 2), toolchain=toolchain())
+                   end=o["msg"], oracle_seconds=round(dt, 2), toolchain=toolchain(),
+                   phases={str(int(p)): int((o["phase"] == p).sum()) for p in sorted(set(o["phase"].tolist()))})
         with open(os.path.join(HERE, name + ".json"), "w") as f:
             json.dump(out, f, indent=1)
         print(name, out)
