@@ -19,6 +19,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <map>
+#include <mutex>
 #include <deque>
 #include <thread>
 #include <chrono>
@@ -45,26 +47,75 @@ namespace rsa_amd {
       throw Error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + #expr);           \
   } while (0)
 
+// Process-wide cache of device allocations.  A released buffer goes back to
+// the cache once the device is idle (hipFree would wait for that too) and is
+// handed to a later request of at least half its size; hipMalloc runs only
+// when nothing fits (the whole cache is freed and the call retried when the
+// device is out of memory).  Engines created one after another thus reuse the
+// same HBM instead of mapping and unmapping it for every run.
+class DevicePool {
+ public:
+  static DevicePool& get() {
+    static DevicePool* pool = new DevicePool();  // never destroyed: outlives every engine
+    return *pool;
+  }
+  void* alloc(size_t bytes, size_t& got) {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      auto it = free_.lower_bound(bytes);
+      if (it != free_.end() && it->first / 2 <= bytes) {
+        got = it->first;
+        void* p = it->second;
+        free_.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) throw Error(std::string("HIP error ") + hipGetErrorString(e) + " at hipMalloc");
+    got = bytes;
+    return p;
+  }
+  void release(void* p, size_t bytes) {
+    (void)hipDeviceSynchronize();  // kernels queued earlier may still use it
+    std::lock_guard<std::mutex> lk(m_);
+    free_.emplace(bytes, p);
+  }
+  void trim() {
+    (void)hipDeviceSynchronize();
+    std::lock_guard<std::mutex> lk(m_);
+    for (auto& kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+  }
+
+ private:
+  std::mutex m_;
+  std::multimap<size_t, void*> free_;  // by size in bytes
+};
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
-  size_t cap = 0;
+  size_t cap = 0;  // elements
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  void reserve(size_t n) {  // contents are not preserved
+  ~DevBuf() { release(); }
+  void reserve(size_t n) {  // contents are not preserved; grows by at least 1.5x
     if (n <= cap && p) return;
-    if (p) HIP_OK(hipFree(p));
-    p = nullptr;
-    size_t c = std::max<size_t>(n, 1);
-    HIP_OK(hipMalloc(&p, c * sizeof(T)));
-    cap = c;
+    const size_t want = std::max<size_t>({n, cap + cap / 2, 1});
+    release();
+    size_t got = 0;
+    p = static_cast<T*>(DevicePool::get().alloc(want * sizeof(T), got));
+    cap = got / sizeof(T);
   }
   void release() {
-    if (p) HIP_OK(hipFree(p));
+    if (p) DevicePool::get().release(p, cap * sizeof(T));
     p = nullptr;
     cap = 0;
   }
@@ -2169,10 +2220,12 @@ class Engine final : public IEngine {
     }
   }
 
+  size_t cand_cap_ = 0;  // candidates every per-candidate buffer holds
   void ensure_cand(int n) {
-    if ((size_t)n <= ccell_.cap && ccell_.p) return;
+    if ((size_t)n <= cand_cap_) return;
     HIP_OK(hipStreamSynchronize(st_));  // queued kernels may still use the old buffers
     const size_t c = (size_t)std::max(n, 1024) * 5 / 4;
+    cand_cap_ = c;
     cpos_.reserve(c * D);
     crad_.reserve(c);
     cprio_.reserve(c);
@@ -2774,7 +2827,7 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
       // change the class or overshoot its target: shots(cur) + shots(nxt) < maximal.
       if (max_draws < 0 || st->draws + 2 <= max_draws) {
         nxt = host_timed("host_prepare_next", [&] { return prepare(cur.w_end(), cur.buf ^ 1); });
-        if (nxt.nshots <= 48000 && (size_t)nxt.nshots <= ccell_.cap && nshots + (uint64_t)nxt.nshots < maximal) {
+        if (nxt.nshots <= 48000 && (size_t)nxt.nshots <= cand_cap_ && nshots + (uint64_t)nxt.nshots < maximal) {
           enqueue_first(nxt, false, 0);  // the miss count restarts after each draw
           nxt_live = true;
         }
