@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <atomic>
 #include <map>
 #include <mutex>
@@ -683,6 +684,16 @@ struct LaneGroup {
 
 constexpr int CSLOT = 8;  // candidates of a cell kept in its fixed slots; more go to an overflow list
 constexpr int KMAX = 16;  // partners kept in a candidate's slab; longer lists go to a pool
+constexpr int PKEEP = 4;  // partners a lane keeps in registers between counting and writing
+constexpr int KREJ = 96;  // rejected voxel words passed with the launch; more: offsets read from host memory
+
+// The Lemire rejection chain of a draw as launch arguments: shot i's voxel word
+// sits at i * (2D + 1) + #{k : idx[k] <= i} (idx = shot index of each rejected
+// word, non-decreasing).  n < 0: read the offsets from the pinned array.
+struct RejList {
+  int n;
+  int idx[KREJ];
+};
 
 // Control block of the fast draw path (device side).
 struct DrawCtl {
@@ -703,6 +714,7 @@ struct DrawCtl {
   // closed, every kernel of that draw returns at once.
   int gate;
   int swept;     // seq of the last sweep that ran (k_update_commit of that round runs)
+  int nunres;    // candidates k_partner_lists left unresolved (listed in unres)
   int acc_draw;  // spheres accepted so far in the current draw
 };
 // What k_sweep_labels publishes to pinned host memory (seq last).
@@ -724,6 +736,7 @@ constexpr int kPubSlots = 8;
 template <int D>
 __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
                                                       int nshots, uint32_t N, const uint32_t* __restrict__ vpos,
+                                                      RejList rl,
                                                       int end_rel, VoxView<D> vx, ClassDev cls, Geo<D> g,
                                                       CellStore<D> cs, double* __restrict__ cpos,
                                                       double* __restrict__ crad, uint32_t* __restrict__ cprio,
@@ -733,12 +746,11 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
                                                       int* __restrict__ ovhead, int* __restrict__ next,
                                                       DrawCtl* __restrict__ ctl, int draw_id, int forced) {
   using LG = LaneGroup<D>;
-  if (!forced && ctl->gate != draw_id) return;  // whole grid: the gate does not change during this kernel
+  // The gate does not change during this kernel.  It is read first but tested
+  // only before the first store, so that its load overlaps the reads below
+  // (all of which stay in bounds whatever the gate says).
+  const int gate = forced ? draw_id : ctl->gate;
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gtid == 0) {
-    ctl->acc_draw = 0;
-    if (forced) ctl->gate = draw_id;
-  }
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
   const int sub = gtid % LG::LPC;
@@ -746,8 +758,17 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
   double p[D];
   int64_t cell = -1;
   bool ok = false;
+  uint32_t prio = 0;
   if (valid) {
-    const uint32_t v = vpos[i];
+    prio = ring_at(ring, mask, w0 + (uint64_t)end_rel + i) >> 1;
+    uint32_t v;
+    if (rl.n >= 0) {
+      int k = 0;
+      for (int t = 0; t < rl.n; t++) k += rl.idx[t] <= i;
+      v = (uint32_t)i * (uint32_t)(2 * D + 1) + (uint32_t)k;
+    } else {
+      v = vpos[i];
+    }
     const uint32_t gw = ring_at(ring, mask, w0 + v);
     const uint32_t id = (uint32_t)(((uint64_t)gw * (uint64_t)N) >> 32);
 #pragma unroll
@@ -791,10 +812,15 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
     }
   }
   const bool anyhit = (__ballot(hit) & LG::mask(lane)) != 0;
+  if (gate != draw_id) return;  // whole grid
+  if (gtid == 0) {
+    ctl->acc_draw = 0;
+    if (forced) ctl->gate = draw_id;
+  }
   if (valid && sub == 0) {
 #pragma unroll
     for (int d = 0; d < D; d++) cpos[(size_t)d * nshots + i] = p[d];
-    cprio[i] = ring_at(ring, mask, w0 + (uint64_t)end_rel + i) >> 1;
+    cprio[i] = prio;
     crad[i] = cls.radius;
     cphase[i] = cls.phase;
     if (!ok) {
@@ -824,57 +850,74 @@ __global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict_
                                               int* __restrict__ cellcc, const int* __restrict__ cslot,
                                               int* __restrict__ ovhead, const int* __restrict__ next,
                                               int* __restrict__ seg, int* __restrict__ len, int* __restrict__ ord,
-                                              int* __restrict__ cell2t, DrawCtl* __restrict__ ctl, int draw_id) {
-  if (ctl->gate != draw_id) return;
+                                              int* __restrict__ cell2t, DrawCtl* __restrict__ ctl, int draw_id,
+                                              int ncells) {
+  const int gate = ctl->gate;  // tested before the first store (loads overlap)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = i < nshots && owner[i];
-  const int c = on ? ccell[i] : 0;
+  // With a closed gate k_decode_filter did not run, and owner/ccell hold stale or
+  // never-written values: clamp the cell so that the read below stays in bounds.
+  const int c0 = on ? ccell[i] : 0;
+  const int c = (c0 >= 0 && c0 < ncells) ? c0 : 0;
   const int n = on ? cellcc[c] : 0;
-  // segment allocation: block prefix + one atomic per block
-  __shared__ int s_w[WG / 64];
-  __shared__ int s_base;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int incl = n;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int u = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += u;
-  }
-  if (lane == 63) s_w[w] = incl;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int tot = 0;
-    for (int k = 0; k < WG / 64; k++) {
-      const int v = s_w[k];
-      s_w[k] = tot;
-      tot += v;
-    }
-    s_base = tot ? atomicAdd(&ctl->seg_cursor, tot) : 0;
-  }
-  __syncthreads();
-  if (!on) return;
-  const int base = s_base + s_w[w] + incl - n;
+  if (gate != draw_id || !on) return;
+  // the owner's list: CSLOT entries at i * CSLOT, longer lists after all of those
+  const int base = n <= CSLOT ? i * CSLOT : nshots * CSLOT + atomicAdd(&ctl->seg_cursor, n);
   int* a = ord + base;
-  const int nf = n < CSLOT ? n : CSLOT;
-  for (int k = 0; k < nf; k++) a[k] = cslot[(size_t)c * CSLOT + k];
-  int k = nf;
-  for (int j = ovhead[c]; j != -1 && k < n; j = next[j]) a[k++] = j;
-  // shot order (Shell sort: lists are short except in clustered late draws)
-  for (int gap = n / 2; gap > 0; gap /= 2)
-    for (int q = gap; q < n; q++) {
-      const int v = a[q];
-      int j = q;
-      while (j >= gap && a[j - gap] > v) {
-        a[j] = a[j - gap];
-        j -= gap;
-      }
-      a[j] = v;
-    }
   int m = n;
-  for (int q = m - 1; q >= 0; q--) {
-    if (ctag[a[q]] == T_SKIP) {
-      a[q] = a[m - 1];
-      m--;
+  if (n <= CSLOT) {  // in registers: a sorting network, then the removal with selects
+    const int4* row = reinterpret_cast<const int4*>(cslot + (size_t)c * CSLOT);
+    const int4 r0 = row[0], r1 = row[1];
+    int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++)
+      if (k >= n) v[k] = INT_MAX;
+    // shot order: 19-comparator network for 8 keys (Batcher odd-even merge)
+    constexpr int NET[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
+#pragma unroll
+    for (int e = 0; e < 19; e++) {
+      const int x = v[NET[e][0]], y = v[NET[e][1]];
+      v[NET[e][0]] = min(x, y);
+      v[NET[e][1]] = max(x, y);
+    }
+    bool sk[CSLOT];
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++) sk[k] = k < n && ctag[v[k]] == T_SKIP;
+    // remove Skip from the back, the last entry taking the removed one's place
+#pragma unroll
+    for (int q = CSLOT - 1; q >= 0; q--) {
+      if (sk[q]) {
+        int x = v[q];
+#pragma unroll
+        for (int k = q + 1; k < CSLOT; k++) x = (m - 1 == k) ? v[k] : x;
+        v[q] = x;
+        m--;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++)
+      if (k < m) a[k] = v[k];
+  } else {
+    for (int k = 0; k < CSLOT; k++) a[k] = cslot[(size_t)c * CSLOT + k];
+    int k = CSLOT;
+    for (int j = ovhead[c]; j != -1 && k < n; j = next[j]) a[k++] = j;
+    // shot order (Shell sort: such long lists occur only in clustered late draws)
+    for (int gap = n / 2; gap > 0; gap /= 2)
+      for (int q = gap; q < n; q++) {
+        const int v = a[q];
+        int j = q;
+        while (j >= gap && a[j - gap] > v) {
+          a[j] = a[j - gap];
+          j -= gap;
+        }
+        a[j] = v;
+      }
+    for (int q = m - 1; q >= 0; q--) {
+      if (ctag[a[q]] == T_SKIP) {
+        a[q] = a[m - 1];
+        m--;
+      }
     }
   }
   seg[i] = base;
@@ -971,7 +1014,7 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
                                              double r, uint32_t pi, const double* cpos, const double* crad,
                                              const uint32_t* cprio, const uint8_t* ctag, const int* cell2t,
                                              const int* seg, const int* len, const int* ord, int* out,
-                                             bool& first_ghost, int* mismatch) {
+                                             bool& first_ghost, int* mismatch, int (*keep)[PKEEP] = nullptr) {
   using LG = LaneGroup<D>;
   int n = 0;
   first_ghost = false;
@@ -997,7 +1040,7 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
           const int64_t ic = cell_index<D>(g, qp);
           int64_t want = 0;
           for (int d = D - 1; d >= 0; d--) want = want * g.n[d] + (cc[d] + stencil_off<D>(g.stab[s], d));
-          if (ic != want) atomicAdd(mismatch, 1);
+          if (ic != want) (*mismatch)++;  // added by the caller once its gate is open
         }
       } else {
 #pragma unroll
@@ -1005,7 +1048,12 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
       }
       if (collide<D>(r, p, crad[j], qp)) {
         if (n == 0) first_ghost = nb.ghost;
-        if (out) out[n] = j | (nb.ghost ? (int)0x80000000 : 0);
+        const int e = j | (nb.ghost ? (int)0x80000000 : 0);
+        if (out) out[n] = e;
+        if (keep) {  // the first PKEEP partners in registers (constant indices)
+#pragma unroll
+          for (int t = 0; t < PKEEP; t++) (*keep)[t] = n == t ? e : (*keep)[t];
+        }
         n++;
       }
     }
@@ -1025,28 +1073,36 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
                                                  const int* __restrict__ seg, const int* __restrict__ len,
                                                  const int* __restrict__ ord, int* __restrict__ poff,
                                                  int* __restrict__ pcnt, int* __restrict__ slab,
-                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int gate_id) {
+                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int gate_id,
+                                                 int* __restrict__ unres) {
   using LG = LaneGroup<D>;
-  if (gate_id >= 0 && ctl->gate != gate_id) return;
+  const int gate = gate_id >= 0 ? ctl->gate : gate_id;  // tested before the first store (loads overlap)
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
   const int sub = gtid % LG::LPC;
-  const bool active = i < nshots && ctag[i] == T_UNDECIDED;
+  // read before the gate test: with a closed gate the candidate arrays are stale
+  // or never written, so a cell outside the grid disables the lane
+  const int ci = i < nshots ? ccell[i] : -1;
+  const bool active = i < nshots && ctag[i] == T_UNDECIDED && ci >= 0 && ci < g.ncells;
   int n = 0;
   bool fg = false;
   double p[D];
   double r = 0;
   uint32_t pi = 0;
   int cc[D];
+  int keep[PKEEP] = {};
+  int mism = 0;
   if (active) {
     cand_pos<D>(cpos, nshots, i, p);
     r = crad[i];
     pi = cprio[i];
     cell_coords<D>(g, ccell[i], cc);
     n = lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, nullptr, fg,
-                         &ctl->img_mismatch);
+                         &mism, &keep);
   }
+  if (gate != gate_id) return;  // whole grid
+  if (mism) atomicAdd(&ctl->img_mismatch, mism);
   int incl = n;  // inclusive scan over the lane group (stencil order)
 #pragma unroll
   for (int off = 1; off < LG::LPC; off <<= 1) {
@@ -1067,6 +1123,7 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
       lab[i] = T_CONFLICT;
     } else {
       lab[i] = T_UNRESOLVED;
+      unres[atomicAdd(&ctl->nunres, 1)] = i;
       pcnt[i] = total;
       if (total <= KMAX) {
         off = -1;
@@ -1084,7 +1141,11 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
   off = __shfl(off, (lane / LG::LPC) * LG::LPC, 64);
   if (pend && off != -2) {
     dst = off == -1 ? slab + (size_t)i * KMAX : pool + off;
-    if (n > 0) {
+    if (n > 0 && n <= PKEEP) {
+#pragma unroll
+      for (int t = 0; t < PKEEP; t++)
+        if (t < n) dst[incl - n + t] = keep[t];
+    } else if (n > PKEEP) {  // rare: walk the cells again, writing as it goes
       bool dummy;
       lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord,
                        dst + (incl - n), dummy, nullptr);
@@ -1105,19 +1166,74 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
                                                 const int* __restrict__ poff, const int* __restrict__ pcnt,
                                                 const int* __restrict__ slab, const int* __restrict__ pool,
                                                 DrawCtl* ctl, DrawPub* host, int seq, int gate_id, int draw_id,
-                                                double aimed, uint64_t miss_base) {
+                                                double aimed, uint64_t miss_base, const int* __restrict__ unres) {
   extern __shared__ uint8_t s_lab[];
   __shared__ int s_prog, s_left, s_acc, s_con, s_stuck;
-  if (gate_id >= 0 && ctl->gate != gate_id) return;
+  // everything below reads first and stores only once the gate is known open
+  const int gate = gate_id >= 0 ? ctl->gate : gate_id;
+  const bool ovf = ctl->pool_overflow != 0;
+  const int nu = ctl->nunres;
   const int tid = threadIdx.x;
   if (tid == 0) {
     s_acc = 0;
     s_con = 0;
     s_stuck = 0;
   }
-  const bool ovf = ctl->pool_overflow != 0;
   for (int q = tid; q < nshots; q += blockDim.x) s_lab[q] = ctag[q] == T_UNDECIDED ? lab[q] : T_DROPPED;
+  // each thread keeps up to two unresolved candidates and their first PKEEP
+  // partners in registers for all passes; further ones are re-read per pass
+  constexpr int NC = 2;
+  int cq[NC], cm[NC];
+  int ce[NC][PKEEP];
+  const int* cpl[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const int u = tid + c * (int)blockDim.x;
+    cq[c] = -1;
+    cm[c] = 0;
+    cpl[c] = nullptr;
+#pragma unroll
+    for (int t = 0; t < PKEEP; t++) ce[c][t] = 0;
+    if (!ovf && u < nu) {
+      const int q = unres[u];
+      const int m = pcnt[q];
+      const int o = poff[q];
+      const int* pl = o == -1 ? slab + (size_t)q * KMAX : pool + o;
+      cq[c] = q;
+      cm[c] = m;
+      cpl[c] = pl;
+#pragma unroll
+      for (int t = 0; t < PKEEP; t++)
+        if (t < m) ce[c][t] = pl[t];
+    }
+  }
   __syncthreads();
+  if (gate != gate_id) return;  // the whole workgroup
+  // one candidate's label from its partners (first non-Skip one decides), or
+  // "wait" while that partner is unresolved
+  auto resolve = [&](int m, const int* pl, const int* cache, uint8_t& res) -> bool {
+    res = T_UNDECIDED;
+    for (int k = 0; k < m; k++) {
+      int e;
+      if (k < PKEEP) {
+        e = cache[0];
+#pragma unroll
+        for (int t = 1; t < PKEEP; t++) e = k == t ? cache[t] : e;
+      } else {
+        e = pl[k];
+      }
+      if (e < 0) {
+        res = T_CONFLICT;
+        return true;
+      }
+      const uint8_t lj = s_lab[e];
+      if (lj == T_UNRESOLVED) return false;
+      if (lj == T_SKIP) continue;
+      res = (lj == T_CONFLICT) ? T_CONFLICT : T_SKIP;
+      return true;
+    }
+    return true;
+  };
   if (!ovf) {
     for (int pass = 0; pass < nshots + 2; pass++) {
       if (tid == 0) {
@@ -1126,34 +1242,35 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
       }
       __syncthreads();
       int prog = 0, left = 0;
-      for (int q = tid; q < nshots; q += blockDim.x) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int q = cq[c];
+        if (q < 0 || s_lab[q] != T_UNRESOLVED) continue;
+        uint8_t res;
+        if (resolve(cm[c], cpl[c], ce[c], res)) {
+          s_lab[q] = res;
+          lab[q] = res;
+          prog++;
+        } else {
+          left++;
+        }
+      }
+      for (int u = tid + NC * (int)blockDim.x; u < nu; u += blockDim.x) {
+        const int q = unres[u];
         if (s_lab[q] != T_UNRESOLVED) continue;
         const int m = pcnt[q];
         const int o = poff[q];
         const int* pl = o == -1 ? slab + (size_t)q * KMAX : pool + o;
-        uint8_t res = T_UNDECIDED;
-        bool wait = false;
-        for (int k = 0; k < m; k++) {
-          const int e = pl[k];
-          if (e < 0) {
-            res = T_CONFLICT;
-            break;
-          }
-          const uint8_t lj = s_lab[e];
-          if (lj == T_UNRESOLVED) {
-            wait = true;
-            break;
-          }
-          if (lj == T_SKIP) continue;
-          res = (lj == T_CONFLICT) ? T_CONFLICT : T_SKIP;
-          break;
-        }
-        if (wait) {
-          left++;
-        } else {
+        int cache[PKEEP];
+#pragma unroll
+        for (int t = 0; t < PKEEP; t++) cache[t] = t < m ? pl[t] : 0;
+        uint8_t res;
+        if (resolve(m, pl, cache, res)) {
           s_lab[q] = res;
           lab[q] = res;
           prog++;
+        } else {
+          left++;
         }
       }
       if (prog) atomicAdd(&s_prog, prog);
@@ -1204,6 +1321,7 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
   ctl->dropped = 0;
   ctl->img_mismatch = 0;
   ctl->seg_cursor = 0;
+  ctl->nunres = 0;
   if (!ovf) ctl->pool_cursor = 0;
 }
 
@@ -1221,10 +1339,11 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
                                                       int* __restrict__ ord, int* __restrict__ cell2t,
                                                       DrawCtl* __restrict__ ctl, int seq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ctl->swept != seq) return;  // its sweep did not run (closed gate)
-  if (i >= nshots || !owner[i]) return;
-  if (ctl->pool_overflow || ctl->label_stuck) return;
+  const int swept = ctl->swept;  // all reads are issued before the tests
+  const bool skip_round = ctl->pool_overflow || ctl->label_stuck;
   const bool commit = ctl->con == 0;
+  const bool on = i < nshots && owner[i];
+  if (swept != seq || !on || skip_round) return;  // its sweep did not run (closed gate), or no owner
   int* a = ord + seg[i];
   int n = len[i];
   for (int q = n - 1; q >= 0; q--) {
@@ -2237,7 +2356,7 @@ class Engine final : public IEngine {
     touched_.reserve(c);
     seg_.reserve(c);
     len_.reserve(c);
-    ord_.reserve(c);
+    ord_.reserve(c * (CSLOT + 1));  // k_lists: CSLOT per owner, then the longer lists
     poff_.reserve(c);
     pcnt_.reserve(c);
     ctag_.reserve(c);
@@ -2245,6 +2364,11 @@ class Engine final : public IEngine {
     next_.reserve(c);
     owner_.reserve(c);
     slab_.reserve(c * KMAX);
+    unres_.reserve(c);
+    // kernels of a draw whose gate stays closed read these before returning
+    HIP_OK(hipMemset(owner_.p, 0, c * sizeof(uint8_t)));
+    HIP_OK(hipMemset(ccell_.p, 0xff, c * sizeof(int)));
+    HIP_OK(hipMemset(ctag_.p, T_DROPPED, c * sizeof(uint8_t)));
     if (plist_.cap < 8 * c) plist_.reserve(8 * c);
   }
 
@@ -2374,6 +2498,7 @@ class Engine final : public IEngine {
     int buf = 0;             // pinned voxel-word buffer holding the chain
     int id = 0, seq = 0;     // draw id (gate value) and first sweep's seq, once queued
     uint64_t miss_base = 0;  // spheres placed before the draw since the miss-rate reference point
+    RejList rej{};           // the chain as launch arguments (n < 0: use the pinned offsets)
     uint64_t w_end() const { return w_shots + (uint64_t)end_rel + (uint64_t)nshots; }
   };
   void init_pub();
@@ -2383,7 +2508,7 @@ class Engine final : public IEngine {
   int draw_id_ = 0;
   int64_t single_draw_slow(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
                            uint64_t& w_next, rsa_gpu_stats* st);
-  int host_chain(int nshots, uint64_t w0, int b);
+  int host_chain(int nshots, uint64_t w0, int b, RejList* rej);
   Prep prepare(uint64_t w, int b);
   void wait_pub(int seq);
   DevBuf<DrawCtl> dctl_;
@@ -2393,7 +2518,7 @@ class Engine final : public IEngine {
   uint32_t* hvpos_[2] = {nullptr, nullptr};  // pinned voxel-word offsets of the shots (double buffer)
   const uint32_t* hvpos_dev_[2] = {nullptr, nullptr};  // their device addresses
   size_t hvpos_cap_[2] = {0, 0};
-  DevBuf<int> next_, cslot_, ovhead_, slab_, pool_;
+  DevBuf<int> next_, cslot_, ovhead_, slab_, pool_, unres_;
   DevBuf<uint8_t> owner_;
   DevBuf<int> stab_;  // packed stencil offsets (Geo::stab), then linear ones (Geo::slin)
 };
@@ -2452,7 +2577,7 @@ double Engine<D>::mom_pivot(std::vector<uint64_t> values, int64_t nb_first) {
 // uniform_int_dist.h:246-268): the host holds the word stream anyway (Poisson
 // draws from it), so it walks the rejections; the decode itself runs on the GPU.
 template <int D>
-int Engine<D>::host_chain(int nshots, uint64_t w0, int b) {
+int Engine<D>::host_chain(int nshots, uint64_t w0, int b, RejList* rej) {
   const uint32_t N = (uint32_t)nvox_;
   const uint32_t thr = (uint32_t)(0u - N) % N;
   const uint64_t stride = 2 * D + 1;
@@ -2476,6 +2601,12 @@ int Engine<D>::host_chain(int nshots, uint64_t w0, int b) {
       }
       if ((uint32_t)((uint64_t)buf[p & (kRing - 1)] * (uint64_t)N) >= thr) break;
       p++;
+      if (rej && rej->n >= 0) {
+        if (rej->n < KREJ)
+          rej->idx[rej->n++] = i;
+        else
+          rej->n = -1;
+      }
     }
     out[i] = (uint32_t)(p - w0);
     p += stride;
@@ -2512,7 +2643,8 @@ typename Engine<D>::Prep Engine<D>::prepare(uint64_t w, int b) {
     break;
   }
   p.w_shots = w;
-  p.end_rel = host_chain(p.nshots, w, b);
+  p.rej.n = 0;
+  p.end_rel = host_chain(p.nshots, w, b, &p.rej);
   return p;
 }
 
@@ -2556,11 +2688,12 @@ void Engine<D>::enqueue_round(const Prep& pr, int seq, int gate_id) {
   timed("k_partner_lists", (double)nshots * (8.0 * (D + 1) + 10.0), [&] {
     hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, cpos_.p, crad_.p, cprio_.p,
                        ccell_.p, ctag_.p, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, poff_.p, pcnt_.p, slab_.p,
-                       pool_.p, (int)pool_.cap, dctl_.p, gate_id);
+                       pool_.p, (int)pool_.cap, dctl_.p, gate_id, unres_.p);
   });
   timed("k_sweep_labels", (double)nshots * 4.0, [&] {
     hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(1024), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, poff_.p,
-                       pcnt_.p, slab_.p, pool_.p, dctl_.p, hpub_dev_, seq, gate_id, pr.id, aimed_, pr.miss_base);
+                       pcnt_.p, slab_.p, pool_.p, dctl_.p, hpub_dev_, seq, gate_id, pr.id, aimed_, pr.miss_base,
+                       unres_.p);
   });
   timed("k_update_commit", (double)nshots * (8.0 * (D + 1) + 12.0), [&] {
     hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
@@ -2580,7 +2713,7 @@ void Engine<D>::enqueue_first(Prep& pr, bool forced, uint64_t miss_base) {
   // k_decode_filter reads the shot words from the device ring and the chain
   // offsets from pinned host memory (written before this launch, untouched
   // until the draw's results have been published).
-  dev_words(w0, w0 + (uint64_t)pr.end_rel + (uint64_t)nshots);
+  host_timed("host_dev_words", [&] { dev_words(w0, w0 + (uint64_t)pr.end_rel + (uint64_t)nshots); });
   const uint32_t N = (uint32_t)nvox_;
   ClassDev cls{std::get<0>(classes_[cur_]), (uint32_t)std::get<2>(classes_[cur_])};
   using LG = LaneGroup<D>;
@@ -2588,13 +2721,13 @@ void Engine<D>::enqueue_first(Prep& pr, bool forced, uint64_t miss_base) {
   const unsigned nbs = nblocks(nshots);
   timed("k_decode_filter", (double)nshots * (4.0 * (2 * D + 2) + 4.0 * D + 8.0 * (D + 1) + 13.0), [&] {
     hipLaunchKernelGGL(k_decode_filter<D>, dim3(nbl), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N,
-                       hvpos_dev_[pr.buf], pr.end_rel, vview(), cls, geo_, cs(), cpos_.p, crad_.p, cprio_.p,
+                       hvpos_dev_[pr.buf], pr.rej, pr.end_rel, vview(), cls, geo_, cs(), cpos_.p, crad_.p, cprio_.p,
                        cphase_.p, ccell_.p, ctag_.p, owner_.p, cellcc_.p, cslot_.p, ovhead_.p, next_.p, dctl_.p, pr.id,
                        forced ? 1 : 0);
   });
   timed("k_lists", (double)nshots * 24.0, [&] {
     hipLaunchKernelGGL(k_lists, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, owner_.p, ctag_.p, cellcc_.p, cslot_.p,
-                       ovhead_.p, next_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, pr.id);
+                       ovhead_.p, next_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, pr.id, geo_.ncells);
   });
   enqueue_round(pr, pr.seq, pr.id);
   if (prof_)
