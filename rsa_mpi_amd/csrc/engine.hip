@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <condition_variable>
 #include <atomic>
 #include <map>
 #include <mutex>
@@ -97,6 +98,55 @@ class DevicePool {
  private:
   std::mutex m_;
   std::multimap<size_t, void*> free_;  // by size in bytes
+};
+
+// Process-wide cache of pinned host allocations, by flags and size.  A new
+// block is written once when allocated: the first CPU pass over fresh pinned
+// pages is several times slower than later ones, and the MT ring is filled by
+// a thread that must keep ahead of the draws.
+class PinnedPool {
+ public:
+  static PinnedPool& get() {
+    static PinnedPool* pool = new PinnedPool();  // never destroyed: outlives every engine
+    return *pool;
+  }
+  void* alloc(size_t bytes, unsigned flags) {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->flags == flags && it->bytes >= bytes && it->bytes / 2 <= bytes) {
+          void* p = it->p;
+          used_[p] = *it;
+          free_.erase(it);
+          return p;
+        }
+      }
+    }
+    void* p = nullptr;
+    HIP_OK(hipHostMalloc(&p, bytes, flags));
+    std::memset(p, 0, bytes);
+    std::lock_guard<std::mutex> lk(m_);
+    used_[p] = Block{p, bytes, flags};
+    return p;
+  }
+  void release(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(m_);
+    auto it = used_.find(p);
+    if (it == used_.end()) return;
+    free_.push_back(it->second);
+    used_.erase(it);
+  }
+
+ private:
+  struct Block {
+    void* p;
+    size_t bytes;
+    unsigned flags;
+  };
+  std::mutex m_;
+  std::vector<Block> free_;
+  std::map<void*, Block> used_;
 };
 
 template <class T>
@@ -677,8 +727,11 @@ struct LaneGroup {
   static constexpr int LPC = D == 2 ? 16 : (D == 3 ? 32 : 64);  // lanes per candidate
   static constexpr int CPL = (NST + LPC - 1) / LPC;              // stencil cells per lane
   __device__ static inline uint64_t mask(int lane) {
-    if (LPC == 64) return ~0ull;
-    return ((1ull << LPC) - 1ull) << ((lane / LPC) * LPC);
+    if constexpr (LPC == 64) {
+      return ~0ull;
+    } else {
+      return ((1ull << LPC) - 1ull) << ((lane / LPC) * LPC);
+    }
   }
 };
 
@@ -1770,18 +1823,24 @@ class Engine final : public IEngine {
       if (sup_[d] - inf_[d] < 2 * domain_rad) throw Error("The domain is not sufficiently large!");
     HIP_OK(hipStreamCreate(&st_));
     HIP_OK(hipStreamCreate(&cst_));
-    HIP_OK(hipHostMalloc((void**)&hctr_, sizeof(int) * C_NUM, hipHostMallocDefault));
+    hctr_ = static_cast<int*>(PinnedPool::get().alloc(sizeof(int) * C_NUM, hipHostMallocDefault));
     dctr_.reserve(C_NUM);
     ring_.reserve(kRing);
-    HIP_OK(hipHostMalloc((void**)&hs_.buf, sizeof(uint32_t) * kRing, hipHostMallocDefault));
+    hs_.buf = static_cast<uint32_t*>(PinnedPool::get().alloc(sizeof(uint32_t) * kRing, hipHostMallocDefault));
+    for (int b = 0; b < kPrepBufs; b++) {
+      hvpos_[b] = static_cast<uint32_t*>(PinnedPool::get().alloc((kFastShots + 1) * sizeof(uint32_t),
+                                                                 hipHostMallocCoherent | hipHostMallocMapped));
+      HIP_OK(hipHostGetDevicePointer((void**)&hvpos_dev_[b], hvpos_[b], 0));
+    }
     aimed_ = D <= 3 ? 0.95 : 1 - 0.05 * puissance<(D > 3 ? D - 3 : 0)>(0.1);
   }
   ~Engine() override {
+    prep_halt();
     hs_.halt();
     for (auto* h : hvpos_)
-      if (h) (void)hipHostFree(h);
-    if (hs_.buf) (void)hipHostFree(hs_.buf);
-    if (hpub_) (void)hipHostFree(hpub_);
+      PinnedPool::get().release(h);
+    PinnedPool::get().release(hs_.buf);
+    PinnedPool::get().release(hpub_);
     if (pe0_) (void)hipEventDestroy(pe0_);
     if (pe1_) (void)hipEventDestroy(pe1_);
     if (cst_) (void)hipStreamSynchronize(cst_);
@@ -1789,7 +1848,7 @@ class Engine final : public IEngine {
     for (auto e : free_ev_) (void)hipEventDestroy(e);
     if (cst_) (void)hipStreamDestroy(cst_);
     if (st_) (void)hipStreamDestroy(st_);
-    if (hctr_) (void)hipHostFree(hctr_);
+    PinnedPool::get().release(hctr_);
   }
 
   int dim() const override { return D; }
@@ -2042,14 +2101,14 @@ class Engine final : public IEngine {
       }
     }
     void set_low(uint64_t v) { low.store(v, std::memory_order_release); }
-    double wait_s = 0;  // time consumers spent waiting for the generator
+    std::atomic<int64_t> wait_ns{0};  // time consumers (main and helper threads) waited for the generator
     void ensure(uint64_t upto) {
       if (upto > low.load(std::memory_order_relaxed) + kRing) throw Error("word window larger than the ring");
       if (hi.load(std::memory_order_acquire) >= upto) return;
       const auto t0 = std::chrono::steady_clock::now();
       while (hi.load(std::memory_order_acquire) < upto) {
       }
-      wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      wait_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
     }
     uint32_t at(uint64_t pos) {
       if (hi.load(std::memory_order_acquire) <= pos) ensure(pos + 1);
@@ -2508,16 +2567,31 @@ class Engine final : public IEngine {
   int draw_id_ = 0;
   int64_t single_draw_slow(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
                            uint64_t& w_next, rsa_gpu_stats* st);
-  int host_chain(int nshots, uint64_t w0, int b, RejList* rej);
-  Prep prepare(uint64_t w, int b);
+  int host_chain(int nshots, uint64_t w0, uint32_t N, int b, RejList* rej);
+  Prep prepare(uint64_t w, int64_t nvox, int b);
+  // draw preparation on a helper thread (prep_start / prep_get / prep_halt)
+  static constexpr int kPrepAhead = 4, kPrepBufs = 8, kFastShots = 48000;
+  void prep_start();
+  void prep_halt();
+  Prep prep_get(uint64_t w, int64_t nv);
+  std::thread pth_;
+  std::mutex pm_;
+  std::condition_variable pcv_;
+  bool pstop_ = false, preq_set_ = false;
+  uint64_t pgen_ = 0, preq_w_ = 0, pnext_w_ = 0;  // pnext_w_: where the worker's next draw starts
+  int64_t preq_nv_ = -1;
+  int pbuf_next_ = 0;
+  std::deque<Prep> pready_;
+  std::string perr_;
   void wait_pub(int seq);
   DevBuf<DrawCtl> dctl_;
   DrawPub* hpub_ = nullptr;      // pinned, coherent: kPubSlots slots written by k_sweep_labels
   DrawPub* hpub_dev_ = nullptr;  // its device address
   int seq_ = 0;
-  uint32_t* hvpos_[2] = {nullptr, nullptr};  // pinned voxel-word offsets of the shots (double buffer)
-  const uint32_t* hvpos_dev_[2] = {nullptr, nullptr};  // their device addresses
-  size_t hvpos_cap_[2] = {0, 0};
+  // pinned voxel-word offsets of prepared draws (kFastShots + 1 entries each),
+  // used round-robin by the helper thread and never freed during a run
+  uint32_t* hvpos_[kPrepBufs] = {};
+  const uint32_t* hvpos_dev_[kPrepBufs] = {};
   DevBuf<int> next_, cslot_, ovhead_, slab_, pool_, unres_;
   DevBuf<uint8_t> owner_;
   DevBuf<int> stab_;  // packed stencil offsets (Geo::stab), then linear ones (Geo::slin)
@@ -2577,18 +2651,11 @@ double Engine<D>::mom_pivot(std::vector<uint64_t> values, int64_t nb_first) {
 // uniform_int_dist.h:246-268): the host holds the word stream anyway (Poisson
 // draws from it), so it walks the rejections; the decode itself runs on the GPU.
 template <int D>
-int Engine<D>::host_chain(int nshots, uint64_t w0, int b, RejList* rej) {
-  const uint32_t N = (uint32_t)nvox_;
+int Engine<D>::host_chain(int nshots, uint64_t w0, uint32_t N, int b, RejList* rej) {
   const uint32_t thr = (uint32_t)(0u - N) % N;
   const uint64_t stride = 2 * D + 1;
-  if ((size_t)nshots > hvpos_cap_[b]) {  // buffer b is idle (a queued draw may read the other one)
-    if (hvpos_[b]) HIP_OK(hipHostFree(hvpos_[b]));
-    hvpos_cap_[b] = (size_t)nshots * 5 / 4 + 1024;
-    HIP_OK(hipHostMalloc((void**)&hvpos_[b], hvpos_cap_[b] * sizeof(uint32_t),
-                         hipHostMallocCoherent | hipHostMallocMapped));
-    HIP_OK(hipHostGetDevicePointer((void**)&hvpos_dev_[b], hvpos_[b], 0));
-  }
-  uint32_t* out = hvpos_[b];
+  // offsets go to pinned buffer b when the draw takes the fast path
+  uint32_t* out = nshots <= kFastShots ? hvpos_[b] : nullptr;
   uint64_t avail = w0 + (uint64_t)nshots * stride + 64;  // rejections are rare: extend when passed
   hs_.ensure(avail);
   const uint32_t* buf = hs_.buf;
@@ -2608,7 +2675,7 @@ int Engine<D>::host_chain(int nshots, uint64_t w0, int b, RejList* rej) {
           rej->n = -1;
       }
     }
-    out[i] = (uint32_t)(p - w0);
+    if (out) out[i] = (uint32_t)(p - w0);
     p += stride;
   }
   return (int)(p - w0);
@@ -2618,12 +2685,12 @@ int Engine<D>::host_chain(int nshots, uint64_t w0, int b, RejList* rej) {
 // from position w until a draw has shots, then its rejection chain into pinned
 // buffer b.  A prepared draw stays valid while the voxel count is unchanged.
 template <int D>
-typename Engine<D>::Prep Engine<D>::prepare(uint64_t w, int b) {
+typename Engine<D>::Prep Engine<D>::prepare(uint64_t w, int64_t nvox, int b) {
   Prep p;
-  p.nv = nvox_;
+  p.nv = nvox;
   p.w_start = w;
   p.buf = b;
-  const int nv = (int)nvox_;
+  const int nv = (int)nvox;
   if (nv == 0) return p;
   const double intensity = 0.01 * nv;  // compute_intensity_poisson (:221-223)
   const double max_intensity = intensity;
@@ -2644,8 +2711,96 @@ typename Engine<D>::Prep Engine<D>::prepare(uint64_t w, int b) {
   }
   p.w_shots = w;
   p.rej.n = 0;
-  p.end_rel = host_chain(p.nshots, w, b, &p.rej);
+  p.end_rel = host_chain(p.nshots, w, (uint32_t)nvox, b, &p.rej);
   return p;
+}
+
+// The helper thread: prepares the draws that follow (w, nv) while the main
+// thread drives the GPU, up to kPrepAhead of them, each into the next of the
+// kPrepBufs pinned offset buffers.  A request for another (w, nv) -- after a
+// refinement -- discards what was prepared.
+template <int D>
+void Engine<D>::prep_start() {
+  prep_halt();
+  {
+    std::lock_guard<std::mutex> lk(pm_);
+    pstop_ = false;
+    pgen_ = 0;
+    preq_set_ = false;
+    pready_.clear();
+    perr_.clear();
+  }
+  pth_ = std::thread([this] {
+    std::unique_lock<std::mutex> lk(pm_);
+    uint64_t gen = 0, w = 0;
+    int64_t nv = -1;
+    for (;;) {
+      pcv_.wait(lk, [&] { return pstop_ || (preq_set_ && (pgen_ != gen || (int)pready_.size() < kPrepAhead)); });
+      if (pstop_) return;
+      if (pgen_ != gen) {
+        gen = pgen_;
+        w = preq_w_;
+        nv = preq_nv_;
+      }
+      const int b = pbuf_next_;
+      pbuf_next_ = (pbuf_next_ + 1) % kPrepBufs;
+      lk.unlock();
+      Prep pr;
+      std::string err;
+      try {
+        pr = prepare(w, nv, b);
+      } catch (const std::exception& e) {
+        err = e.what();
+      }
+      lk.lock();
+      if (!err.empty()) {
+        perr_ = err;
+        pcv_.notify_all();
+        return;
+      }
+      if (gen != pgen_) continue;  // superseded while it ran
+      pready_.push_back(pr);
+      w = pr.w_end();
+      pnext_w_ = w;
+      pcv_.notify_all();
+    }
+  });
+}
+
+template <int D>
+void Engine<D>::prep_halt() {
+  if (!pth_.joinable()) return;
+  {
+    std::lock_guard<std::mutex> lk(pm_);
+    pstop_ = true;
+  }
+  pcv_.notify_all();
+  pth_.join();
+}
+
+// The prepared draw starting at stream position w for voxel count nv: the
+// queue's front, or the one the worker is preparing, or a new request.
+template <int D>
+typename Engine<D>::Prep Engine<D>::prep_get(uint64_t w, int64_t nv) {
+  std::unique_lock<std::mutex> lk(pm_);
+  const bool front_ok = !pready_.empty() && pready_.front().w_start == w && pready_.front().nv == nv;
+  const bool coming = pready_.empty() && preq_set_ && pnext_w_ == w && preq_nv_ == nv;
+  if (!front_ok && !coming) {
+    pready_.clear();
+    pgen_++;
+    preq_w_ = w;
+    preq_nv_ = nv;
+    pnext_w_ = w;
+    preq_set_ = true;
+    pcv_.notify_all();
+  }
+  pcv_.wait(lk, [&] { return !pready_.empty() || !perr_.empty(); });
+  if (!perr_.empty()) throw Error(perr_);
+  Prep pr = pready_.front();
+  pready_.pop_front();
+  pcv_.notify_all();
+  if (pr.w_start != w || pr.nv != nv) throw Error("internal: prepared draw out of sequence");
+  return pr;
 }
 
 template <int D>
@@ -2673,7 +2828,8 @@ void Engine<D>::init_pub() {
   if (hpub_) return;
   dctl_.reserve(1);
   HIP_OK(hipMemset(dctl_.p, 0, sizeof(DrawCtl)));
-  HIP_OK(hipHostMalloc((void**)&hpub_, sizeof(DrawPub) * kPubSlots, hipHostMallocCoherent | hipHostMallocMapped));
+  hpub_ = static_cast<DrawPub*>(
+      PinnedPool::get().alloc(sizeof(DrawPub) * kPubSlots, hipHostMallocCoherent | hipHostMallocMapped));
   HIP_OK(hipHostGetDevicePointer((void**)&hpub_dev_, hpub_, 0));
   std::memset(hpub_, 0, sizeof(DrawPub) * kPubSlots);
   pool_.reserve(1 << 16);
@@ -2901,14 +3057,17 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
   std::memset(st, 0, sizeof(*st));
   reset_dev_ring();  // no copy may read the host ring once the producer restarts
   hs_.start(seed);
-  hs_.wait_s = 0;
+  hs_.wait_ns = 0;
+  prep_start();
   struct Halt {
-    HostRing& h;
-    ~Halt() { h.halt(); }
-  } halt_at_exit{hs_};
+    Engine& e;
+    ~Halt() {
+      e.prep_halt();
+      e.hs_.halt();
+    }
+  } halt_at_exit{*this};
   Prep cur;               // the next draw's inputs (valid while cur.nv == nvox_ and cur.w_start == w)
   bool cur_live = false;  // its first round is already queued (behind the previous draw)
-  int nextbuf = 0;
   uint64_t w = 0;
   init_voxels(r_max_ + gmin_);
   st->nvox_max = nvox_;
@@ -2932,9 +3091,8 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     hs_.set_low(w);
     if (cur.nv != nvox_ || cur.w_start != w) {
       if (cur_live) throw Error("internal: a queued draw lost its inputs");
-      cur = host_timed("host_prepare", [&] { return prepare(w, nextbuf); });
+      cur = host_timed("host_prepare", [&] { return prep_get(w, nvox_); });
     }
-    nextbuf = cur.buf ^ 1;
     st->loop_iterations += cur.zero_iters;  // the iterations whose Poisson draw gave no shot
     const uint64_t nshots = (uint64_t)cur.nshots;
     const uint64_t maximal = current_number();
@@ -2959,7 +3117,7 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
       // triggers a refinement (the device then keeps its gate closed); it cannot
       // change the class or overshoot its target: shots(cur) + shots(nxt) < maximal.
       if (max_draws < 0 || st->draws + 2 <= max_draws) {
-        nxt = host_timed("host_prepare_next", [&] { return prepare(cur.w_end(), cur.buf ^ 1); });
+        nxt = host_timed("host_prepare_next", [&] { return prep_get(cur.w_end(), nvox_); });
         if (nxt.nshots <= 48000 && (size_t)nxt.nshots <= cand_cap_ && nshots + (uint64_t)nxt.nshots < maximal) {
           enqueue_first(nxt, false, 0);  // the miss count restarts after each draw
           nxt_live = true;
@@ -3002,7 +3160,7 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     cur = nxt;
     cur_live = nxt_live;
   }
-  if (prof_) prof_add("host_wait_mt_words", hs_.wait_s, 0);
+  if (prof_) prof_add("host_wait_mt_words", 1e-9 * (double)hs_.wait_ns.load(), 0);
   HIP_OK(hipStreamSynchronize(st_));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
