@@ -267,8 +267,19 @@ __global__ void __launch_bounds__(1024) k_chain(const uint32_t* __restrict__ rin
 
 template <int D>
 struct VoxView {
+  // lattice coordinates, SoA: 32-bit while every coordinate fits, 64-bit after
+  // the refinement that would overflow them (the reference keeps int64)
   const int32_t* c[D];
+  const int64_t* w[D];
+  int wide;
   double origin[D], len[D];
+  __device__ __forceinline__ int64_t coord(int d, int64_t i) const { return wide ? w[d][i] : (int64_t)c[d][i]; }
+  __device__ __forceinline__ void set(int d, int64_t i, int64_t v) const {
+    if (wide)
+      const_cast<int64_t*>(w[d])[i] = v;
+    else
+      const_cast<int32_t*>(c[d])[i] = (int32_t)v;
+  }
 };
 
 struct ClassDev {
@@ -292,7 +303,7 @@ __global__ void k_decode(const uint32_t* __restrict__ ring, uint64_t mask, uint6
   const uint32_t id = (uint32_t)(((uint64_t)gw * (uint64_t)N) >> 32);
   double p[D];
   for (int d = 0; d < D; d++) {
-    const double o = vx.origin[d] + (double)vx.c[d][id] * vx.len[d];
+    const double o = vx.origin[d] + (double)vx.coord(d, id) * vx.len[d];
     const double u = canonical(ring_at(ring, mask, w0 + v + 1 + 2 * d), ring_at(ring, mask, w0 + v + 2 + 2 * d));
     p[d] = o + vx.len[d] * u;
     cpos[(size_t)d * nshots + i] = p[d];
@@ -826,7 +837,7 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
     const uint32_t id = (uint32_t)(((uint64_t)gw * (uint64_t)N) >> 32);
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      const double o = vx.origin[d] + (double)vx.c[d][id] * vx.len[d];
+      const double o = vx.origin[d] + (double)vx.coord(d, id) * vx.len[d];
       const double u = canonical(ring_at(ring, mask, w0 + v + 1 + 2 * d), ring_at(ring, mask, w0 + v + 2 + 2 * d));
       p[d] = o + vx.len[d] * u;
     }
@@ -1484,7 +1495,7 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
     double ov[D], ca[D], cb[D], ok0[D], ok1[D];
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      const int64_t c = vx.c[d][i];
+      const int64_t c = vx.coord(d, i);
       ov[d] = vp.origin[d] + (double)c * vp.len[d];  // orig_voxel (list_of_voxels.ixx:146-153)
       ca[d] = ov[d] + 0.5 * vp.len[d];                // is_covered's centre (:245)
       cb[d] = vp.origin[d] + ((double)c + 0.5) * vp.len[d];  // center() used by subdivide (:155-162)
@@ -1581,9 +1592,10 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
 }
 
 // single workgroup exclusive scan over block counts; writes the total to *total
+// (*total = -1 when the sum exceeds INT_MAX: the offsets would not fit)
 __global__ void __launch_bounds__(1024) k_scan_blocks(int* __restrict__ blk, int nb, int* __restrict__ total) {
   __shared__ int sh[1024];
-  int carry = 0;
+  long long carry = 0;
   for (int base = 0; base < nb; base += 1024) {
     const int t = base + threadIdx.x;
     const int v = t < nb ? blk[t] : 0;
@@ -1595,12 +1607,12 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(int* __restrict__ blk, int
       sh[threadIdx.x] += add;
       __syncthreads();
     }
-    if (t < nb) blk[t] = carry + sh[threadIdx.x] - v;
+    if (t < nb) blk[t] = (int)(carry + sh[threadIdx.x] - v);
     const int tot = sh[1023];
     __syncthreads();
     carry += tot;
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (threadIdx.x == 0) *total = carry > INT_MAX ? -1 : (int)carry;
 }
 
 // exclusive rank of element i among flagged elements: block offset + in-block prefix
@@ -1643,7 +1655,7 @@ __global__ void __launch_bounds__(WG) k_move(int64_t nvox, uint32_t* __restrict_
   const int keep = ctr[C_KEEP];
   if (valid && unc && i >= keep) {
     const int dst = holes[keep - U - 1];
-    for (int d = 0; d < D; d++) const_cast<int32_t*>(vx.c[d])[dst] = vx.c[d][i];
+    for (int d = 0; d < D; d++) vx.set(d, dst, vx.coord(d, i));
     code[dst] = code[i];
   }
 }
@@ -1665,8 +1677,8 @@ __global__ void __launch_bounds__(WG) k_child_count(int64_t n, const uint32_t* _
 // TabCorner order (list_of_voxels.ixx:76-99): coordinates 2c + tab[k]
 template <int D>
 __global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __restrict__ code,
-                                                const int* __restrict__ blk, VoxView<D> src, int32_t* dst0,
-                                                int32_t* dst1, int32_t* dst2, int32_t* dst3, VoxParams<D> vp) {
+                                                const int* __restrict__ blk, VoxView<D> src, VoxView<D> dst,
+                                                VoxParams<D> vp) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t m = i < n ? (code[i] & 0xffffu) : 0u;
   const int c = __popc(m);
@@ -1681,14 +1693,13 @@ __global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __res
   }
   if (i >= n || m == 0) return;
   int o = blk[blockIdx.x] + sh[threadIdx.x] - c;
-  int32_t* dst[4] = {dst0, dst1, dst2, dst3};
-  int32_t pc[D];
-  for (int d = 0; d < D; d++) pc[d] = src.c[d][i];
+  int64_t pc[D];
+  for (int d = 0; d < D; d++) pc[d] = src.coord(d, i);
   constexpr int NCH = ipow_c(2, D);
 #pragma unroll
   for (int k = 0; k < NCH; k++) {
     if (!(m & (1u << k))) continue;
-    for (int d = 0; d < D; d++) dst[d][o] = 2 * pc[d] + vp.tab[k][d];
+    for (int d = 0; d < D; d++) dst.set(d, o, 2 * pc[d] + vp.tab[k][d]);
     o++;
   }
 }
@@ -1983,10 +1994,16 @@ class Engine final : public IEngine {
       for (int d = 0; d < D; d++) len[d] = vlen_[d];
     if (coords == nullptr) return (uint64_t)nvox_;
     if ((uint64_t)nvox_ > cap) throw Error("voxels: output capacity too small");
-    std::vector<int32_t> tmp(nvox_);
     for (int d = 0; d < D; d++) {
-      if (nvox_) HIP_OK(hipMemcpy(tmp.data(), vc_[vcur_][d].p, nvox_ * sizeof(int32_t), hipMemcpyDeviceToHost));
-      for (int64_t i = 0; i < nvox_; i++) coords[i * D + d] = tmp[i];
+      if (wide_[vcur_]) {
+        std::vector<int64_t> tmp(nvox_);
+        if (nvox_) HIP_OK(hipMemcpy(tmp.data(), vw_[vcur_][d].p, nvox_ * sizeof(int64_t), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < nvox_; i++) coords[i * D + d] = tmp[i];
+      } else {
+        std::vector<int32_t> tmp(nvox_);
+        if (nvox_) HIP_OK(hipMemcpy(tmp.data(), vc_[vcur_][d].p, nvox_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < nvox_; i++) coords[i * D + d] = tmp[i];
+      }
     }
     return (uint64_t)nvox_;
   }
@@ -2043,6 +2060,8 @@ class Engine final : public IEngine {
 
   // voxels
   DevBuf<int32_t> vc_[2][D];
+  DevBuf<int64_t> vw_[2][D];  // the same lists once coordinates need 64 bits
+  bool wide_[2] = {false, false};
   int vcur_ = 0;
   int64_t nvox_ = 0;
   double vlen_[D] = {}, vorigin_[D] = {};
@@ -2445,6 +2464,7 @@ class Engine final : public IEngine {
     if (total > ((int64_t)1 << 31) - 1) throw Error("initial voxel lattice too large");
     nvox_ = total;
     vcur_ = 0;
+    wide_[0] = wide_[1] = false;
     for (int d = 0; d < D; d++) vc_[0][d].reserve(total);
     int32_t* c[4] = {nullptr, nullptr, nullptr, nullptr};
     for (int d = 0; d < D; d++) c[d] = vc_[0][d].p;
@@ -2456,13 +2476,16 @@ class Engine final : public IEngine {
   }
   int64_t max_coord_ = 0;
 
-  VoxView<D> vview() const {
+  VoxView<D> vview(int b = -1) const {
+    if (b < 0) b = vcur_;
     VoxView<D> v;
     for (int d = 0; d < D; d++) {
-      v.c[d] = vc_[vcur_][d].p;
+      v.c[d] = vc_[b][d].p;
+      v.w[d] = vw_[b][d].p;
       v.origin[d] = vorigin_[d];
       v.len[d] = vlen_[d];
     }
+    v.wide = wide_[b] ? 1 : 0;
     return v;
   }
 
@@ -2509,7 +2532,8 @@ class Engine final : public IEngine {
     const VoxParams<D> vp = vparams(rmin);
     VoxView<D> vx = vview();
     // compulsory traffic: lattice coordinates in, code out, every placed sphere and cell count once
-    const double cover_bytes = (double)nvox_ * (4.0 * D + 4.0) + (double)nb_fast_ * sizeof(Slot<D>) + geo_.ncells;
+    const double cover_bytes = (double)nvox_ * ((wide_[vcur_] ? 8.0 : 4.0) * D + 4.0) +
+                               (double)nb_fast_ * sizeof(Slot<D>) + geo_.ncells;
     timed("k_cover", cover_bytes, [&] {
       hipLaunchKernelGGL(k_cover<D>, dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, code_.p, blk_.p);
     }, 1);
@@ -2522,20 +2546,26 @@ class Engine final : public IEngine {
       nvox_ = 0;
       return;
     }
-    if (keep * (int64_t)VoxParams<D>::NCH > ((int64_t)1 << 31) - 1) throw Error("voxel list would exceed 2^31 entries");
     const unsigned nb2 = nblocks(keep);
     hipLaunchKernelGGL(k_child_count, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, blk_.p, (int)nb2, dctr_.p + C_NCHILD);
     read_ctr();
     const int64_t nchild = hctr_[C_NCHILD];
-    if (2 * max_coord_ > ((int64_t)1 << 31)) throw Error("voxel lattice coordinates exceed 31 bits");
+    if (nchild < 0) throw Error("voxel list would exceed 2^31 entries");
+    // children coordinates 2c + t: 64-bit lists from the refinement on where they pass INT32_MAX
+    if (max_coord_ > ((int64_t)1 << 61)) throw Error("voxel lattice coordinates exceed 62 bits");
     const int nxt = 1 - vcur_;
-    for (int d = 0; d < D; d++) vc_[nxt][d].reserve(std::max<int64_t>(nchild, 1));
-    int32_t* dst[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (int d = 0; d < D; d++) dst[d] = vc_[nxt][d].p;
-    timed("k_scatter", (double)keep * (4.0 * D + 4.0) + (double)nchild * 4.0 * D, [&] {
-      hipLaunchKernelGGL(k_scatter<D>, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p, vx, dst[0], dst[1],
-                         dst[2], dst[3], vp);
+    wide_[nxt] = wide_[vcur_] || 2 * max_coord_ + 1 > (int64_t)INT32_MAX;
+    for (int d = 0; d < D; d++) {
+      if (wide_[nxt])
+        vw_[nxt][d].reserve(std::max<int64_t>(nchild, 1));
+      else
+        vc_[nxt][d].reserve(std::max<int64_t>(nchild, 1));
+    }
+    const VoxView<D> dst = vview(nxt);
+    const double cw = wide_[vcur_] ? 8.0 : 4.0, nw = wide_[nxt] ? 8.0 : 4.0;
+    timed("k_scatter", (double)keep * (cw * D + 4.0) + (double)nchild * nw * D, [&] {
+      hipLaunchKernelGGL(k_scatter<D>, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p, vx, dst, vp);
     }, 1);
     HIP_OK(hipGetLastError());
     vcur_ = nxt;

@@ -144,3 +144,44 @@ def test_reiteration_bit_exact(dim, L, r0, reiter, seed):
 def periodic_overlaps_ok(g, dim, L):
     from _common import periodic_overlaps
     return periodic_overlaps(g["pos"], g["rad"], [L] * dim) == 0
+
+
+def _golden(name):
+    import json
+    import os
+    return json.load(open(os.path.join(os.path.dirname(__file__), "golden", name + ".json")))
+
+
+def _check_golden(g, gold, classes_phase=None):
+    st = g["stats"]
+    print(st)
+    assert len(g["rad"]) == gold["n"]
+    assert set_hash(g["pos"], g["rad"]) == gold["sha256"]
+    assert st["words"] == gold["words"]
+    assert st["draws"] == gold["draws"] and st["shots"] == gold["shots"] and st["refines"] == gold["refines"]
+    assert st["rounds"] == gold["rounds"]
+    assert st["nvox_max"] == gold["nvox_max"]
+    assert st["end_reason"] == 1 and st["nvox_final"] == 0
+    if "phases" in gold:
+        for ph, cnt in gold["phases"].items():
+            assert int((g["phase"] == int(ph)).sum()) == cnt
+
+
+def test_cfg3_polydisperse_matches_golden():
+    """3D polydisperse: classes r=0.01/0.005/0.0025 at volume fraction 0.10
+    each, then a fill class r=0.0025 (4.35M spheres; 1.15 billion voxels at the
+    peak), against the oracle's committed fixture."""
+    gold = _golden("cfg3_3d_poly_seed0")
+    cls = [tuple(c) for c in gold["case"]["classes"]]
+    g = gpu_run(3, 1.0, cls, seed=0)
+    _check_golden(g, gold)
+
+
+def test_cfg5a_r0002_matches_golden():
+    """3D monodisperse r=0.002 in the unit cube (11.46M spheres, 12144 draws)."""
+    from _common import uncovered_samples, volume_fraction
+    gold = _golden("cfg5a_3d_r0.002_seed0")
+    g = gpu_run(3, 1.0, [(0.002, 1.0, 0)], seed=0)
+    _check_golden(g, gold)
+    assert uncovered_samples(g["pos"], g["rad"], [1.0] * 3, 0.002, n=100000) == 0
+    assert 0.38 < volume_fraction(g["rad"], 3, 1.0) < 0.39
