@@ -1,0 +1,3 @@
+// Drop-in for the reference header of the same name: see rsa_mpi_amd.hpp.
+#pragma once
+#include "rsa_mpi_amd.hpp"

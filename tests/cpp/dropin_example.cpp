@@ -1,0 +1,82 @@
+// A caller of the reference's C++ API, unchanged but for the build line: the
+// calls mirror test/test_plugin_write_xyz.cpp:12-24, user_interface.hxx and
+// scripts/main_max_frac.cpp:39-52.  It writes the spheres it gets as binary
+// records (x.., r, phase) so that the tests can compare them bit for bit.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include <operator_algorithm.hxx>
+#include <rsa_decoration.hxx>
+#include <rsa_domain.hxx>
+#include <test.hxx>
+#include <user_interface.hxx>
+#include <write_xyz.hpp>
+
+template <int DIM>
+static void dump(const std::string& path, const std::vector<rsa_sphere<DIM>>& s) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) std::exit(2);
+  const uint64_t n = s.size();
+  std::fwrite(&n, sizeof n, 1, f);
+  for (const auto& x : s) {
+    double rec[DIM + 2];
+    for (int d = 0; d < DIM; d++) rec[d] = x.center[d];
+    rec[DIM] = x.radius;
+    rec[DIM + 1] = (double)x.phase;
+    std::fwrite(rec, sizeof rec, 1, f);
+  }
+  std::fclose(f);
+}
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: %s uniform3d|xyz3d|throw2d|reiter3d out [args]\n", argv[0]);
+    return 1;
+  }
+  rsa_mpi::init(&argc, &argv);
+  const std::string mode = argv[1], out = argv[2];
+  if (mode == "uniform3d") {  // algorithm::uniform_generate<3, 1>(domain, rad, 6000, 10, seed)
+    const double L = std::atof(argv[3]), rad = std::atof(argv[4]);
+    const size_t seed = std::strtoull(argv[5], nullptr, 10);
+    constexpr int DIM = 3;
+    std::array<double, DIM> inf = {0., 0., 0.}, sup = {L, L, L};
+    rsa_domain<DIM> domain(inf, sup, 1, rad);
+    auto algo = algorithm::uniform_generate<DIM, 1>(domain, rad, 6000, 10, seed);
+    dump<DIM>(out, domain.extract_spheres());
+    std::printf("spheres %llu volume %.17g\n", (unsigned long long)domain.get_grid().get_number_of_spheres(),
+                domain.compute_total_volume_of_spheres());
+  } else if (mode == "xyz3d") {  // test/test_plugin_write_xyz.cpp:12-24
+    const double rad = std::atof(argv[3]);
+    constexpr int DIM = 3;
+    std::array<double, DIM> inf = {0., 0., 0.};
+    std::array<double, DIM> sup = {std::atof(argv[4]), std::atof(argv[5]), std::atof(argv[6])};
+    rsa_domain<DIM> domain(inf, sup, 1, rad);
+    auto algo = algorithm::uniform_generate<DIM, 1>(domain, rad, 6000, 10, 0);
+    auto spheres = domain.get_grid().template extract_data<TypeCell::Real>();
+    write_xyz<DIM>(out, spheres, sup);
+  } else if (mode == "throw2d") {  // user_interface::throw_spheres<2>(L, classes, seed, false)
+    const double L = std::atof(argv[3]);
+    const size_t seed = std::strtoull(argv[4], nullptr, 10);
+    std::vector<std::tuple<double, double, int>> cls = {{0.02, 0.2, 0}, {0.01, 1.0, 1}};
+    auto s = user_interface::throw_spheres<2>({L, L}, cls, seed, false);
+    dump<2>(out, s);
+  } else if (mode == "reiter3d") {  // scripts/main_max_frac.cpp:39-52 through user_interface
+    const double L = std::atof(argv[3]);
+    const size_t seed = std::strtoull(argv[4], nullptr, 10);
+    constexpr int DIM = 3;
+    user_interface::rsa_domain<DIM> domain({0., 0., 0.}, {L, L, L}, 0.01);
+    user_interface::RadiusGenerator<DIM> g1({{0.01, 1.0, 0}}, domain.get_total_volume(), 0.);
+    user_interface::rsa_algo<DIM> algo(domain, g1, 6000);
+    algo.proceed(seed);
+    user_interface::RadiusGenerator<DIM> g2({{0.005, 1.0, 0}}, domain.get_total_volume(), 0.);
+    algo.reset_radius_generator(g2);
+    algo.proceed(seed);
+    dump<DIM>(out, domain.extract_spheres());
+  } else {
+    return 1;
+  }
+  rsa_mpi::finalize();
+  return 0;
+}
