@@ -36,6 +36,19 @@ def run_gpu_step(R, profile_level=1):
     return dom, st, dt
 
 
+def measured_traffic():
+    """HBM bytes per k_cover launch on this workload, from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes summarised in profiles/ (each counter scaled by what it reads for
+    a plain copy of known size); None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "k_cover_3d_r0.005.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(side=0.8):
     """The CPU oracle (restatement of the reference's single-rank path) on all
     host cores, on a bounded sample of the same workload: the same radius and
@@ -116,9 +129,10 @@ def main():
     if k and k["seconds"] > 0:
         achieved = k["bytes"] / k["seconds"] / 1e9
         roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(),
                         kernel="k_cover", launches=k["launches"] // args.steps,
-                        avg_launch_ms=round(1e3 * k["seconds"] / k["launches"], 3))
+                        avg_launch_ms=round(1e3 * k["seconds"] / k["launches"], 3),
+                        algorithmic_bytes_per_launch=round(k["bytes"] / k["launches"]))
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "spheres/s", "n_gpus": 1, "steps": args.steps,

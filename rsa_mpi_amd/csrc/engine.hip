@@ -1448,14 +1448,32 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
 
 // ============================================================== kernels: voxel refine
 
+// path::TabCorner (auxiliary/Path.ixx:85-116) as the reference builds it: the
+// children / corner order {00,11,01,10} (2D), {000,111,001,110,010,101,011,100}
+// (3D); for D >= 4 the reference's loop never sets component 0 (kept for parity).
+template <int D>
+struct TabCorner {
+  __host__ __device__ static constexpr int at(int k, int d) {
+    if constexpr (D == 2) {
+      constexpr int t[4][2] = {{0, 0}, {1, 1}, {0, 1}, {1, 0}};
+      return t[k][d];
+    } else if constexpr (D == 3) {
+      constexpr int t[8][3] = {{0, 0, 0}, {1, 1, 1}, {0, 0, 1}, {1, 1, 0},
+                               {0, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 0, 0}};
+      return t[k][d];
+    } else {
+      return d == 0 ? 0 : ((k >> d) & 1);
+    }
+  }
+  // whether component d takes both values over the table (the corner set is
+  // then the product over d of the values taken)
+  __host__ __device__ static constexpr bool both(int d) { return D <= 3 || d != 0; }
+};
+
 template <int D>
 struct VoxParams {
   static constexpr int NCH = ipow_c(2, D);
   double origin[D], len[D], nl[D];
-  signed char tab[NCH][D];
-  // cm[d]: bit t set when some TabCorner row has component d equal to t.  The
-  // corner set {tab[i] * len} is the product over d of these values.
-  unsigned char cm[D];
   double rmin;
   // h2 = sum_d (len_d / 2)^2 when every dimension has both corner values, else
   // a negative sentinel.  Then the farthest corner of a box is at squared
@@ -1472,36 +1490,63 @@ struct VoxParams {
 // product set, and IEEE addition and squaring are monotone (in |t| for the
 // square), so the largest corner sum is the same sum over the per-dimension
 // largest terms: testing that one sum gives the all-corner loop's answer bit
-// for bit.  far_term is that largest term of one dimension.
-__device__ __forceinline__ double far_term(double delta, double len, unsigned cm) {
-  const double a = 0.0 - delta;  // tab 0: 0 * len - delta
-  const double b = len - delta;  // tab 1: 1 * len - delta
-  const double aa = a * a, bb = b * b;
-  return cm == 1 ? aa : (cm == 2 ? bb : fmax(aa, bb));
+// for bit.  far_term is that largest term of one dimension (both: the
+// dimension takes tab values 0 and 1; otherwise only 0).
+template <bool BOTH>
+__device__ __forceinline__ double far_term(double delta, double len) {
+  const double aa = delta * delta;  // tab 0: (0 * len - delta)^2; 0 - delta = -delta exactly
+  if constexpr (!BOTH) {
+    return aa;
+  } else {
+    const double b = len - delta;  // tab 1: 1 * len - delta
+    const double bb = b * b;
+    return bb > aa ? bb : aa;  // squares are >= +0: the larger one, as fmax
+  }
 }
 
 // remove_covered's coverage flag (list_of_voxels.ixx:30-40, 240-259) and
 // subdivide_uncovered's children test (:56-100, 261-307) in one pass:
 // out[i] = 0 if voxel i is covered, else 0x10000 | (bit k set when child k in
 // TabCorner order stays uncovered).  blk[b] counts uncovered voxels per block.
-template <int D>
+template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView<D> vx, int64_t nvox,
                                               VoxParams<D> vp, uint32_t* __restrict__ out, int* __restrict__ blk) {
+  using TC = TabCorner<D>;
   constexpr int NCH = ipow_c(2, D);
+  constexpr int NST = ipow_c(3, D);
   constexpr uint32_t FULL = (NCH == 32) ? 0xffffffffu : ((1u << NCH) - 1u);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t code = 0;
-  if (i < nvox) {
-    double ov[D], ca[D], cb[D], ok0[D], ok1[D];
+  // linear offsets of the stencil cells (any order: coverage is an OR)
+  constexpr int NSTL = NST <= 32 ? NST : 1;
+  __shared__ int s_off[NSTL];
+  constexpr int KLIST = 16;  // listed spheres per lane before a flush
+  __shared__ uint32_t s_list[KLIST][WG];
+  if (NST <= 32 && threadIdx.x < NSTL) {
+    int o = 0, r = threadIdx.x, stride = 1;
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      const int64_t c = vx.coord(d, i);
-      ov[d] = vp.origin[d] + (double)c * vp.len[d];  // orig_voxel (list_of_voxels.ixx:146-153)
-      ca[d] = ov[d] + 0.5 * vp.len[d];                // is_covered's centre (:245)
-      cb[d] = vp.origin[d] + ((double)c + 0.5) * vp.len[d];  // center() used by subdivide (:155-162)
+      o += (r % 3 - 1) * stride;
+      r /= 3;
+      stride *= g.n[d];
+    }
+    s_off[threadIdx.x] = o;
+  }
+  __syncthreads();
+  uint32_t code = 0;
+  if (i < nvox) {
+    // ov (orig_voxel, list_of_voxels.ixx:146-153) = origin + c * len equals the
+    // tab-0 child origin origin + (2c) * (len / 2) bit for bit (same real
+    // product, len / 2 exact), so ok0 serves as both
+    double ca[D], cb[D], ok0[D], ok1[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int64_t c = WIDE ? vx.w[d][i] : (int64_t)vx.c[d][i];
       ok0[d] = vp.origin[d] + (double)(2 * c) * vp.nl[d];      // children origins (:82-86), tab 0
       ok1[d] = vp.origin[d] + (double)(2 * c + 1) * vp.nl[d];  // and tab 1
+      ca[d] = ok0[d] + 0.5 * vp.len[d];                         // is_covered's centre (:245)
+      cb[d] = vp.origin[d] + ((double)c + 0.5) * vp.len[d];  // center() used by subdivide (:155-162)
     }
+    const double* ov = ok0;
     const int64_t cella = cell_index<D>(g, ca);
     const int64_t cellb = cell_index<D>(g, cb);
     uint32_t ccov = 0;
@@ -1517,11 +1562,12 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
         const double t = qp[d] - cb[d];
         s2 += t * t;
       }
-      // necessary for coverage (vp.h2 above); the 1e-9 R^2 slack absorbs rounding
+      // necessary for coverage (VoxParams::h2); the 1e-9 R^2 slack absorbs rounding
       if (do_cov && (vp.h2 < 0. || s2 <= R2 - vp.h2 + 1e-9 * R2)) {
         double sf = 0.;
 #pragma unroll
-        for (int d = 0; d < D; d++) sf += far_term(qp[d] - ov[d], vp.len[d], vp.cm[d]);
+        for (int d = 0; d < D; d++) sf += TC::both(d) ? far_term<true>(qp[d] - ov[d], vp.len[d])
+                                                      : far_term<false>(qp[d] - ov[d], vp.len[d]);
         if (!(sf > R2)) {
           covered = true;
           return;
@@ -1531,14 +1577,14 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
         double f0[D], f1[D];
 #pragma unroll
         for (int d = 0; d < D; d++) {
-          f0[d] = far_term(qp[d] - ok0[d], vp.nl[d], vp.cm[d]);
-          f1[d] = far_term(qp[d] - ok1[d], vp.nl[d], vp.cm[d]);
+          f0[d] = TC::both(d) ? far_term<true>(qp[d] - ok0[d], vp.nl[d]) : far_term<false>(qp[d] - ok0[d], vp.nl[d]);
+          f1[d] = TC::both(d) ? far_term<true>(qp[d] - ok1[d], vp.nl[d]) : far_term<false>(qp[d] - ok1[d], vp.nl[d]);
         }
 #pragma unroll
         for (int ch = 0; ch < NCH; ch++) {
           double sf = 0.;
 #pragma unroll
-          for (int d = 0; d < D; d++) sf += vp.tab[ch][d] ? f1[d] : f0[d];
+          for (int d = 0; d < D; d++) sf += TC::at(ch, d) ? f1[d] : f0[d];
           if (!(sf > R2)) ccov |= (1u << ch);
         }
       }
@@ -1554,9 +1600,57 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
       bool interior = true;  // the whole stencil lies in real cells: plain index offsets
 #pragma unroll
       for (int d = 0; d < D; d++) interior = interior && cc[d] > g.gl && cc[d] < g.n[d] - g.gl - 1;
+      if (interior && NST <= 32) {
+        // Two phases against divergence: first every sphere of the stencil gets
+        // the cheap distance test and the ones either test can use are listed
+        // (per lane, in LDS); then the full tests run over the short lists.  A
+        // sphere is listed when it passes the subdivide pre-filter or the
+        // coverage pre-check, exactly the gates of the full test.
+        int nl = 0;
+        auto flush = [&]() {
+          for (int t = 0; t < nl && !covered; t++) {
+            const Slot<D> q = cs.sph[s_list[t][threadIdx.x]];
+            test(q.c, q.r, do_cov, do_child);
+          }
+          nl = 0;
+        };
+        // software pipeline: a cell's count and first slot (always allocated)
+        // are read together, one cell ahead of the tests
+        int src = (int)cell + s_off[0];
+        int nvn = cs.cnt[src];
+        Slot<D> q0n = cs.sph[(uint32_t)(src * cs.cap)];
+        for (int st = 0; st < NST && !covered; st++) {
+          const uint32_t base = (uint32_t)(src * cs.cap);
+          const int nv = nvn;
+          const Slot<D> q0 = q0n;
+          if (st + 1 < NST) {
+            src = (int)cell + s_off[st + 1];
+            nvn = cs.cnt[src];
+            q0n = cs.sph[(uint32_t)(src * cs.cap)];
+          }
+          for (int k = 0; k < nv; k++) {
+            const Slot<D> q = k == 0 ? q0 : cs.sph[base + k];
+            const double rr = q.r + vp.rmin;
+            const double R2 = rr * rr;
+            double s2 = 0.;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+              const double t = q.c[d] - cb[d];
+              s2 += t * t;
+            }
+            const bool use = (do_child && s2 < R2) || (do_cov && (vp.h2 < 0. || s2 <= R2 - vp.h2 + 1e-9 * R2));
+            if (use) {
+              if (nl == KLIST) flush();
+              s_list[nl++][threadIdx.x] = base + k;
+            }
+          }
+        }
+        flush();
+        continue;
+      }
       if (interior) {
-        for (int s = 0; s < g.nst && !covered; s++) {
-          const int src = (int)cell + g.slin[s];
+        for (int st = 0; st < g.nst && !covered; st++) {
+          const int src = (int)cell + g.slin[st];
           const int nv = cs.cnt[src];
           const Slot<D>* sl = cs.sph + (size_t)src * cs.cap;
           for (int k = 0; k < nv && !covered; k++) {
@@ -1566,9 +1660,9 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
         }
         continue;
       }
-      for (int s = 0; s < g.nst && !covered; s++) {
+      for (int st = 0; st < g.nst && !covered; st++) {
         Nb<D> nb;
-        if (!neighbour<D>(g, cc, s, nb)) continue;
+        if (!neighbour<D>(g, cc, st, nb)) continue;
         const int nv = cs.cnt[nb.src];
         const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
         for (int k = 0; k < nv && !covered; k++) {
@@ -1699,7 +1793,7 @@ __global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __res
 #pragma unroll
   for (int k = 0; k < NCH; k++) {
     if (!(m & (1u << k))) continue;
-    for (int d = 0; d < D; d++) dst.set(d, o, 2 * pc[d] + vp.tab[k][d]);
+    for (int d = 0; d < D; d++) dst.set(d, o, 2 * pc[d] + TabCorner<D>::at(k, d));
     o++;
   }
 }
@@ -2496,16 +2590,19 @@ class Engine final : public IEngine {
       vp.len[d] = vlen_[d];
       vp.nl[d] = 0.5 * vlen_[d];  // new_voxel_lengths (list_of_voxels.ixx:62)
     }
-    for (int d = 0; d < D; d++) vp.cm[d] = 0;
+    // the kernels' compile-time table must be the host's, and the farthest-corner
+    // test in k_cover needs its rows to form a product set
+    unsigned cm[D] = {};
     for (int k = 0; k < VoxParams<D>::NCH; k++)
       for (int d = 0; d < D; d++) {
-        if (tab_[k][d] != 0 && tab_[k][d] != 1) throw Error("internal: TabCorner entries must be 0 or 1");
-        vp.tab[k][d] = (signed char)tab_[k][d];
-        vp.cm[d] |= (unsigned char)(1u << tab_[k][d]);
+        if (tab_[k][d] != TabCorner<D>::at(k, d)) throw Error("internal: TabCorner table mismatch");
+        cm[d] |= 1u << tab_[k][d];
       }
-    // the farthest-corner test in k_cover needs the corners to form a product set
     int prod = 1;
-    for (int d = 0; d < D; d++) prod *= (vp.cm[d] == 3 ? 2 : 1);
+    for (int d = 0; d < D; d++) {
+      if ((cm[d] == 3) != TabCorner<D>::both(d)) throw Error("internal: TabCorner table mismatch");
+      prod *= (cm[d] == 3 ? 2 : 1);
+    }
     std::vector<int> seen(1 << D, 0);
     int distinct = 0;
     for (int k = 0; k < VoxParams<D>::NCH; k++) {
@@ -2518,7 +2615,7 @@ class Engine final : public IEngine {
     vp.h2 = 0.;
     for (int d = 0; d < D; d++) vp.h2 += (0.5 * vp.len[d]) * (0.5 * vp.len[d]);
     for (int d = 0; d < D; d++)
-      if (vp.cm[d] != 3) vp.h2 = -1.;
+      if (cm[d] != 3) vp.h2 = -1.;
     return vp;
   }
 
@@ -2535,7 +2632,10 @@ class Engine final : public IEngine {
     const double cover_bytes = (double)nvox_ * ((wide_[vcur_] ? 8.0 : 4.0) * D + 4.0) +
                                (double)nb_fast_ * sizeof(Slot<D>) + geo_.ncells;
     timed("k_cover", cover_bytes, [&] {
-      hipLaunchKernelGGL(k_cover<D>, dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, code_.p, blk_.p);
+      if (wide_[vcur_])
+        hipLaunchKernelGGL((k_cover<D, true>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, code_.p, blk_.p);
+      else
+        hipLaunchKernelGGL((k_cover<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, code_.p, blk_.p);
     }, 1);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, blk_.p, (int)nb, dctr_.p + C_KEEP);
     hipLaunchKernelGGL(k_holes, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
