@@ -85,6 +85,19 @@ __device__ __host__ __forceinline__ int64_t cell_index(const Geo<D>& g, const do
   return (int64_t)ret;
 }
 
+// compute_cell_idx's per-dimension terms for a point known to lie in the grid
+// (every term in [0, n[d]), so the 32-bit truncation equals the 64-bit one):
+// the coordinates c and the linear index (which fits 32 bits as ncells does)
+template <int D>
+__device__ __host__ __forceinline__ int cell_of_point(const Geo<D>& g, const double* p, int* c) {
+  int ret = 0;
+  for (int d = D - 1; d >= 0; d--) {
+    c[d] = (int)((p[d] - g.cshift[d]) * g.cinv[d]);
+    ret = ret * g.n[d] + c[d];
+  }
+  return ret;
+}
+
 template <int D>
 __device__ __host__ __forceinline__ void cell_coords(const Geo<D>& g, int64_t idx, int* c) {
   for (int d = 0; d < D; d++) {

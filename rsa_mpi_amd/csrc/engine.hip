@@ -1547,8 +1547,10 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
       cb[d] = vp.origin[d] + ((double)c + 0.5) * vp.len[d];  // center() used by subdivide (:155-162)
     }
     const double* ov = ok0;
-    const int64_t cella = cell_index<D>(g, ca);
-    const int64_t cellb = cell_index<D>(g, cb);
+    // voxel centres lie inside the real cells
+    int cca[D], ccb[D];
+    const int cella = cell_of_point<D>(g, ca, cca);
+    const int cellb = cell_of_point<D>(g, cb, ccb);
     uint32_t ccov = 0;
     bool covered = false;
     // One sphere (centre qp, radius qr) against the voxel and its children.
@@ -1591,12 +1593,13 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
     };
     // pass 0: coverage around cella (and children when cellb == cella); pass 1: children around cellb
     for (int pass = 0; pass < 2 && !covered; pass++) {
-      const int64_t cell = pass == 0 ? cella : cellb;
+      const int cell = pass == 0 ? cella : cellb;
       const bool do_cov = pass == 0;
       const bool do_child = (pass == 0) ? (cellb == cella) : (cellb != cella);
       if (!do_cov && !do_child) break;
       int cc[D];
-      cell_coords<D>(g, cell, cc);
+#pragma unroll
+      for (int d = 0; d < D; d++) cc[d] = pass == 0 ? cca[d] : ccb[d];
       bool interior = true;  // the whole stencil lies in real cells: plain index offsets
 #pragma unroll
       for (int d = 0; d < D; d++) interior = interior && cc[d] > g.gl && cc[d] < g.n[d] - g.gl - 1;
