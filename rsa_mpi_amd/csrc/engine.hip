@@ -190,6 +190,7 @@ enum Ctr : int {
   C_END_REL,      // word offset after the last shot's coordinates
   C_KEEP,         // uncovered voxels
   C_NCHILD,       // children kept by subdivide
+  C_GROUPED,      // voxels binned by cell (k_cover's processing order)
   C_NUM
 };
 
@@ -1510,12 +1511,15 @@ __device__ __forceinline__ double far_term(double delta, double len) {
 // TabCorner order stays uncovered).  blk[b] counts uncovered voxels per block.
 template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView<D> vx, int64_t nvox,
-                                              VoxParams<D> vp, uint32_t* __restrict__ out, int* __restrict__ blk) {
+                                              VoxParams<D> vp, const uint32_t* __restrict__ perm,
+                                              uint32_t* __restrict__ out, int* __restrict__ blk) {
   using TC = TabCorner<D>;
   constexpr int NCH = ipow_c(2, D);
   constexpr int NST = ipow_c(3, D);
   constexpr uint32_t FULL = (NCH == 32) ? 0xffffffffu : ((1u << NCH) - 1u);
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // thread t takes voxel perm[t] (the cell-grouped order) or, without perm, voxel t
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = perm == nullptr ? t : (t < nvox ? (int64_t)perm[t] : nvox);
   // linear offsets of the stencil cells (any order: coverage is an OR)
   constexpr int NSTL = NST <= 32 ? NST : 1;
   __shared__ int s_off[NSTL];
@@ -1684,8 +1688,119 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
     code = covered ? 0u : (0x10000u | (~ccov & FULL & 0xffffu));
     out[i] = code;
   }
-  const int u = __syncthreads_count(code != 0);
+  if (perm == nullptr) {  // uniform: per-block counts of uncovered voxels
+    const int u = __syncthreads_count(code != 0);
+    if (threadIdx.x == 0) blk[blockIdx.x] = u;
+  }
+}
+
+// k_cover's processing order: the voxels grouped by the cell of their centre
+// (a counting sort), so that the lanes of a wave walk the same stencil cells.
+// The flags do not depend on the order; the grouping only makes the walk
+// uniform across lanes and the cell reads shared.
+template <int D, bool WIDE>
+__global__ void __launch_bounds__(WG) k_bin(Geo<D> g, VoxView<D> vx, int64_t nvox, VoxParams<D> vp,
+                                            int* __restrict__ ccount, int* __restrict__ vcell,
+                                            int* __restrict__ vrank) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < nvox;
+  int key = -1 - lane;  // distinct for lanes past the end
+  if (valid) {
+    double p[D];
+    int cc[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int64_t c = WIDE ? vx.w[d][i] : (int64_t)vx.c[d][i];
+      p[d] = vp.origin[d] + ((double)c + 0.5) * vp.len[d];
+    }
+    const int cell = cell_of_point<D>(g, p, cc);
+    key = (cell >= 0 && cell < g.ncells) ? cell : 0;  // a key only: any value is correct
+  }
+  // one atomic per run of equal keys in adjacent lanes (siblings share a cell)
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const uint64_t heads = __ballot(head);
+  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int start = 63 - __clzll(heads & le);
+  const uint64_t above = heads & ~le;
+  int base = 0;
+  if (head && valid) {
+    const int end = above ? __ffsll((unsigned long long)above) - 1 : 64;
+    base = atomicAdd(&ccount[key], end - lane);
+  }
+  base = __shfl(base, start, 64);
+  if (valid) {
+    vcell[i] = key;
+    vrank[i] = base + lane - start;
+  }
+}
+
+__global__ void __launch_bounds__(WG) k_place(int64_t nvox, const int* __restrict__ vcell,
+                                              const int* __restrict__ vrank, const int* __restrict__ cstart,
+                                              uint32_t* __restrict__ perm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvox) return;
+  perm[cstart[vcell[i]] + vrank[i]] = (uint32_t)i;
+}
+
+// uncovered voxels per block of WG (the blocks of the list order)
+__global__ void __launch_bounds__(WG) k_block_count(int64_t nvox, const uint32_t* __restrict__ code,
+                                                    int* __restrict__ blk) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int u = __syncthreads_count(i < nvox && code[i] != 0);
   if (threadIdx.x == 0) blk[blockIdx.x] = u;
+}
+
+// Device-wide exclusive scan in three launches: tile sums, a scan of the
+// sums (k_scan_blocks), then each tile's scan plus its offset.
+constexpr int SCAN_T = 4096;  // elements per tile: 1024 threads x 4
+
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+__global__ void __launch_bounds__(1024) k_tile_sums(const int* __restrict__ a, int64_t n, int* __restrict__ sums) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_T + threadIdx.x * 4;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) s += base + k < n ? a[base + k] : 0;
+  __shared__ int sh[16];
+  s = wave_incl_scan(s);
+  if ((threadIdx.x & 63) == 63) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < 16; w++) tot += sh[w];
+    sums[blockIdx.x] = tot;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_tile_scan(int* __restrict__ a, int64_t n, const int* __restrict__ sums) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_T + threadIdx.x * 4;
+  int v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    v[k] = base + k < n ? a[base + k] : 0;
+    s += v[k];
+  }
+  __shared__ int sh[16];
+  const int x = wave_incl_scan(s);
+  if ((threadIdx.x & 63) == 63) sh[threadIdx.x >> 6] = x;
+  __syncthreads();
+  int run = sums[blockIdx.x] + x - s;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); w++) run += sh[w];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (base + k < n) a[base + k] = run;
+    run += v[k];
+  }
 }
 
 // single workgroup exclusive scan over block counts; writes the total to *total
@@ -2171,8 +2286,8 @@ class Engine final : public IEngine {
   DevBuf<uint8_t> ctag_, lab_;
 
   // refine scratch
-  DevBuf<uint32_t> code_;
-  DevBuf<int> blk_, holes_;
+  DevBuf<uint32_t> code_, perm_;
+  DevBuf<int> blk_, holes_, ccount_, vcell_, vrank_, tsum_;
 
   // word stream: generated on the host into a pinned ring (the MT19937
   // recurrence is sequential), copied asynchronously into the device ring
@@ -2622,6 +2737,16 @@ class Engine final : public IEngine {
     return vp;
   }
 
+  // exclusive scan of a[0, n) in place; the total (-1 above INT_MAX) to *total
+  void scan_excl(int* a, int64_t n, int* total) {
+    const unsigned nt = nblocks(n, SCAN_T);
+    tsum_.reserve(nt);
+    hipLaunchKernelGGL(k_tile_sums, dim3(nt), dim3(1024), 0, st_, a, n, tsum_.p);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, tsum_.p, (int)nt, total);
+    hipLaunchKernelGGL(k_tile_scan, dim3(nt), dim3(1024), 0, st_, a, n, tsum_.p);
+  }
+  static constexpr int64_t kGroupMin = int64_t(1) << 20;
+
   // remove_covered + subdivide_uncovered (operator_algorithm.ixx:190-197)
   void refine(double rmin) {
     if (nvox_ == 0) return;
@@ -2634,13 +2759,38 @@ class Engine final : public IEngine {
     // compulsory traffic: lattice coordinates in, code out, every placed sphere and cell count once
     const double cover_bytes = (double)nvox_ * ((wide_[vcur_] ? 8.0 : 4.0) * D + 4.0) +
                                (double)nb_fast_ * sizeof(Slot<D>) + geo_.ncells;
+    // long lists are walked in cell-grouped order
+    const bool grouped = nvox_ >= kGroupMin;
+    if (grouped) {
+      perm_.reserve(nvox_);
+      vcell_.reserve(nvox_);
+      vrank_.reserve(nvox_);
+      ccount_.reserve(geo_.ncells);
+      HIP_OK(hipMemsetAsync(ccount_.p, 0, sizeof(int) * (size_t)geo_.ncells, st_));
+      timed("k_bin", (double)nvox_ * ((wide_[vcur_] ? 8.0 : 4.0) * D + 8.0), [&] {
+        if (wide_[vcur_])
+          hipLaunchKernelGGL((k_bin<D, true>), dim3(nb), dim3(WG), 0, st_, geo_, vx, nvox_, vp, ccount_.p, vcell_.p,
+                             vrank_.p);
+        else
+          hipLaunchKernelGGL((k_bin<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, vx, nvox_, vp, ccount_.p, vcell_.p,
+                             vrank_.p);
+      }, 1);
+      scan_excl(ccount_.p, geo_.ncells, dctr_.p + C_GROUPED);
+      timed("k_place", (double)nvox_ * 12.0, [&] {
+        hipLaunchKernelGGL(k_place, dim3(nb), dim3(WG), 0, st_, nvox_, vcell_.p, vrank_.p, ccount_.p, perm_.p);
+      }, 1);
+    }
+    const uint32_t* perm = grouped ? perm_.p : nullptr;
     timed("k_cover", cover_bytes, [&] {
       if (wide_[vcur_])
-        hipLaunchKernelGGL((k_cover<D, true>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, code_.p, blk_.p);
+        hipLaunchKernelGGL((k_cover<D, true>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, perm, code_.p,
+                           blk_.p);
       else
-        hipLaunchKernelGGL((k_cover<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, code_.p, blk_.p);
+        hipLaunchKernelGGL((k_cover<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, perm, code_.p,
+                           blk_.p);
     }, 1);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, blk_.p, (int)nb, dctr_.p + C_KEEP);
+    if (grouped) hipLaunchKernelGGL(k_block_count, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p);
+    scan_excl(blk_.p, nb, dctr_.p + C_KEEP);
     hipLaunchKernelGGL(k_holes, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
     hipLaunchKernelGGL(k_move<D>, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p, vx);
     read_ctr();
@@ -2651,7 +2801,7 @@ class Engine final : public IEngine {
     }
     const unsigned nb2 = nblocks(keep);
     hipLaunchKernelGGL(k_child_count, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, blk_.p, (int)nb2, dctr_.p + C_NCHILD);
+    scan_excl(blk_.p, nb2, dctr_.p + C_NCHILD);
     read_ctr();
     const int64_t nchild = hctr_[C_NCHILD];
     if (nchild < 0) throw Error("voxel list would exceed 2^31 entries");
