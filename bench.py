@@ -13,6 +13,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -65,6 +67,32 @@ def cpu_baseline(side=0.8):
                        f"({side ** 3:.0%} of the workload volume) to maximal packing: {o['n']} spheres in {dt:.2f} s")
 
 
+def batched(R, workers):
+    """The batched workload: 256 independent 3D realizations (box edge L ~ U[0.2, 1],
+    r ~ U[0.004, 0.008], seeds 0..255, each to maximal packing), timed once with
+    `workers` realizations at a time and once one after another (the plain form)."""
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "cfg4_batch.json")))
+    real = fx["realizations"]
+    L = [c["L"] for c in real]
+    r = [c["r"] for c in real]
+    s = [c["seed"] for c in real]
+    R.batch_run(3, L[:8], r[:8], s[:8], workers=workers)  # warm-up: contexts, pools, threads
+    want = np.array([c["n"] for c in fx["oracle"]], dtype=np.uint64)
+    res = {}
+    for name, w in (("batched", workers), ("plain", 1)):
+        t0 = time.perf_counter()
+        out = R.batch_run(3, L, r, s, workers=w)
+        dt = time.perf_counter() - t0
+        res[name] = dict(seconds=round(dt, 3), value=round(float(out["counts"].sum()) / dt, 1),
+                         counts_match_oracle=bool(np.array_equal(out["counts"], want)))
+    return {"workload": "256 independent 3D monodisperse realizations, periodic box [0,L]^3, L~U[0.2,1], "
+                        "r~U[0.004,0.008], seeds 0..255, each to maximal packing",
+            "realizations": len(real), "spheres": int(want.sum()), "workers": workers, "unit": "spheres/s",
+            "value": res["batched"]["value"], "plain_value": res["plain"]["value"],
+            "speedup_vs_plain": round(res["batched"]["value"] / res["plain"]["value"], 3), **{
+                k + "_" + kk: v for k, d in res.items() for kk, v in d.items() if kk != "value"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +100,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-side", type=float, default=0.8)
+    ap.add_argument("--batch-workers", type=int, default=4,
+                    help="realizations run at once in the batched measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (reference reads for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -149,6 +180,7 @@ def main():
         "profile": {kk: dict(launches=v["launches"] // args.steps, ms=round(1e3 * v["seconds"] / args.steps, 3))
                     for kk, v in prof.items()},
         "kernels_all_bracketed": dict(step_ms=round(1e3 * dt2, 1), **detail),
+        "batched": None if args.no_batch else batched(R, args.batch_workers),
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_side),
     }
     print(json.dumps(out), flush=True)

@@ -92,7 +92,8 @@ int rsa_gpu_add_spheres(rsa_gpu_ctx* ctx, int64_t n, const double* pos, const do
 int rsa_gpu_count(rsa_gpu_ctx* ctx, uint64_t* n);
 
 /* rsa_grid<DIM>::extract_spheres / extract_data<Real> order (rsa_grid.ixx:73-98,
- * 139-160): real cells in index order, then in-cell order.  pos is cap*dim. */
+ * 139-160): real cells in index order, then in-cell order.  pos is cap*dim;
+ * phase may be NULL.  With pos NULL only *n is set. */
 int rsa_gpu_extract(rsa_gpu_ctx* ctx, double* pos, double* radius, uint64_t* phase, uint64_t cap,
                     uint64_t* n);
 
@@ -118,6 +119,23 @@ int rsa_gpu_kernel_profile(rsa_gpu_ctx* ctx, int k, const char** name, int64_t* 
 /* Raw MT19937 words of std::mt19937 seeded with seed_seq{rank, seed}
  * (rsa_random.hxx:37-42), from the library's own generator. */
 int rsa_gpu_mt_words(uint64_t seed, uint64_t rank, uint64_t n, uint32_t* out);
+
+/* Independent realizations (the batched form).  Realization k is
+ * uniform_generate<DIM, 1>(domain, radius[k], nbshots, 10, seed[k])
+ * (operator_algorithm.ixx:275-292) on the periodic domain [0, side[k]]^dim
+ * with ghost layer 1 and domain radius radius[k], run to maximal packing: what
+ * the reference runs once per process.  `workers` contexts run at once, each
+ * with its own streams and host threads, taking realizations from a shared
+ * counter; results do not depend on workers.  count[k] receives the spheres of
+ * realization k, volume[k] (volume may be NULL) their total volume.  With pos
+ * non-NULL, every realization's spheres are extracted (extract order),
+ * realization after realization in index order (pos is cap*dim, radius_out
+ * cap); their total must fit cap.  seconds (may be NULL) receives the batch's
+ * wall time.  No counterpart in the reference, which runs one realization per
+ * process. */
+int rsa_gpu_batch_run(int dim, int n, const double* side, const double* radius, const uint64_t* seed,
+                      int nbshots, int workers, uint64_t* count, double* volume, double* pos,
+                      double* radius_out, uint64_t cap, double* seconds);
 
 #ifdef __cplusplus
 }

@@ -225,6 +225,43 @@ def throw_spheres(L, desired_radius_volumeFraction_phase, seed, write_paraview=F
     return domain.extract_spheres()
 
 
+def batch_run(dim, sides, radii, seeds, nbshots=6000, workers=4, extract=False):
+    """Independent realizations, realization k being uniform_generate<dim, 1> on
+    the periodic box [0, sides[k]]^dim with radius radii[k] and seed seeds[k]
+    (operator_algorithm.ixx:275-292), `workers` of them at once on the GPU.
+    Returns a dict: counts, volumes, seconds and, with extract, per realization
+    the (pos, rad) arrays in extraction order."""
+    sides = f64(sides)
+    radii = f64(radii)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    n = len(sides)
+    if len(radii) != n or len(seeds) != n:
+        raise ValueError("sides, radii and seeds must have the same length")
+    counts = np.zeros(n, dtype=np.uint64)
+    volumes = np.zeros(n, dtype=np.float64)
+    secs = np.zeros(1, dtype=np.float64)
+    if extract:
+        # upper bound on the spheres of a box: its volume over the volume of the
+        # cube inscribed in one sphere
+        cap = int(sum(float(s) ** dim / (2 * float(r) / np.sqrt(dim)) ** dim for s, r in zip(sides, radii))) + 16
+        pos = np.zeros(cap * dim, dtype=np.float64)
+        rad = np.zeros(cap, dtype=np.float64)
+        pp, rp = dptr(pos), dptr(rad)
+    else:
+        cap, pp, rp = 0, None, None
+    check(lib().rsa_gpu_batch_run(dim, n, dptr(sides), dptr(radii), seeds.ctypes.data_as(_lib._U64), int(nbshots),
+                                  int(workers), counts.ctypes.data_as(_lib._U64), dptr(volumes), pp, rp, cap,
+                                  dptr(secs)))
+    out = {"counts": counts, "volumes": volumes, "seconds": float(secs[0])}
+    if extract:
+        res, off = [], 0
+        for c in counts.astype(np.int64):
+            res.append((pos[off * dim:(off + c) * dim].reshape(c, dim).copy(), rad[off:off + c].copy()))
+            off += c
+        out["spheres"] = res
+    return out
+
+
 def _make_dim_aliases():
     g = globals()
     for d in (2, 3, 4):

@@ -55,6 +55,31 @@ namespace rsa_amd {
 // when nothing fits (the whole cache is freed and the call retried when the
 // device is out of memory).  Engines created one after another thus reuse the
 // same HBM instead of mapping and unmapping it for every run.
+// The streams on which a buffer released by this thread may still be in use:
+// an engine names its own for the duration of its calls (StreamScope), so a
+// release waits for that engine's work only and engines on other threads run
+// on.  Without a scope a release waits for the whole device.
+struct RelStreams {
+  bool set = false;
+  hipStream_t s[2] = {nullptr, nullptr};  // both null with set: nothing in flight
+};
+inline thread_local RelStreams tl_rel;
+
+class StreamScope {
+ public:
+  StreamScope(hipStream_t a, hipStream_t b) : prev_(tl_rel) {
+    tl_rel.set = true;
+    tl_rel.s[0] = a;
+    tl_rel.s[1] = b;
+  }
+  ~StreamScope() { tl_rel = prev_; }
+  StreamScope(const StreamScope&) = delete;
+  StreamScope& operator=(const StreamScope&) = delete;
+
+ private:
+  RelStreams prev_;
+};
+
 class DevicePool {
  public:
   static DevicePool& get() {
@@ -84,7 +109,12 @@ class DevicePool {
     return p;
   }
   void release(void* p, size_t bytes) {
-    (void)hipDeviceSynchronize();  // kernels queued earlier may still use it
+    if (!tl_rel.set) {
+      (void)hipDeviceSynchronize();  // kernels queued earlier may still use it
+    } else {
+      for (hipStream_t st : tl_rel.s)
+        if (st) (void)hipStreamSynchronize(st);
+    }
     std::lock_guard<std::mutex> lk(m_);
     free_.emplace(bytes, p);
   }
@@ -1856,8 +1886,15 @@ __global__ void __launch_bounds__(WG) k_holes(int64_t nvox, const uint32_t* __re
   if (valid && !unc && i < keep) holes[i - U] = (int)i;
 }
 
+// Step 2: the uncovered voxels at or above keep move into the holes.  Their
+// codes go to `moved` (at the hole's index), not into `code`: the workgroup
+// that straddles keep reads code[] below keep for its ranks while other
+// workgroups fill holes, so code[] must not change during this kernel.
+// Afterwards the code of voxel i < keep is code[i], or moved[i] where code[i]
+// is 0 (every hole is filled).
 template <int D>
-__global__ void __launch_bounds__(WG) k_move(int64_t nvox, uint32_t* __restrict__ code, const int* __restrict__ blk,
+__global__ void __launch_bounds__(WG) k_move(int64_t nvox, const uint32_t* __restrict__ code,
+                                             uint32_t* __restrict__ moved, const int* __restrict__ blk,
                                              const int* __restrict__ ctr, const int* __restrict__ holes,
                                              VoxView<D> vx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1868,13 +1905,19 @@ __global__ void __launch_bounds__(WG) k_move(int64_t nvox, uint32_t* __restrict_
   if (valid && unc && i >= keep) {
     const int dst = holes[keep - U - 1];
     for (int d = 0; d < D; d++) vx.set(d, dst, vx.coord(d, i));
-    code[dst] = code[i];
+    moved[dst] = code[i];
   }
 }
 
-__global__ void __launch_bounds__(WG) k_child_count(int64_t n, const uint32_t* __restrict__ code, int* __restrict__ blk) {
+__device__ __forceinline__ uint32_t kept_code(const uint32_t* code, const uint32_t* moved, int64_t i) {
+  const uint32_t c = code[i];
+  return c != 0 ? c : moved[i];
+}
+
+__global__ void __launch_bounds__(WG) k_child_count(int64_t n, const uint32_t* __restrict__ code,
+                                                    const uint32_t* __restrict__ moved, int* __restrict__ blk) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = i < n ? __popc(code[i] & 0xffffu) : 0;
+  const int c = i < n ? __popc(kept_code(code, moved, i) & 0xffffu) : 0;
   __shared__ int sh[WG];
   sh[threadIdx.x] = c;
   __syncthreads();
@@ -1889,10 +1932,10 @@ __global__ void __launch_bounds__(WG) k_child_count(int64_t n, const uint32_t* _
 // TabCorner order (list_of_voxels.ixx:76-99): coordinates 2c + tab[k]
 template <int D>
 __global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __restrict__ code,
-                                                const int* __restrict__ blk, VoxView<D> src, VoxView<D> dst,
-                                                VoxParams<D> vp) {
+                                                const uint32_t* __restrict__ moved, const int* __restrict__ blk,
+                                                VoxView<D> src, VoxView<D> dst, VoxParams<D> vp) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t m = i < n ? (code[i] & 0xffffu) : 0u;
+  const uint32_t m = i < n ? (kept_code(code, moved, i) & 0xffffu) : 0u;
   const int c = __popc(m);
   __shared__ int sh[WG];
   sh[threadIdx.x] = c;
@@ -2044,8 +2087,10 @@ class Engine final : public IEngine {
     define_grid(domain_rad);  // the domain's own grid (rsa_domain.ixx:26); replaced by algo_init
     for (int d = 1; d < D; d++)
       if (sup_[d] - inf_[d] < 2 * domain_rad) throw Error("The domain is not sufficiently large!");
-    HIP_OK(hipStreamCreate(&st_));
-    HIP_OK(hipStreamCreate(&cst_));
+    // non-blocking: no implicit ordering with the null stream (every copy and
+    // fill of the engine is on its own streams), so engines run side by side
+    HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&cst_, hipStreamNonBlocking));
     hctr_ = static_cast<int*>(PinnedPool::get().alloc(sizeof(int) * C_NUM, hipHostMallocDefault));
     dctr_.reserve(C_NUM);
     ring_.reserve(kRing);
@@ -2058,6 +2103,10 @@ class Engine final : public IEngine {
     aimed_ = D <= 3 ? 0.95 : 1 - 0.05 * puissance<(D > 3 ? D - 3 : 0)>(0.1);
   }
   ~Engine() override {
+    // work still queued (speculative draws) ends before any buffer is handed on
+    if (st_) (void)hipStreamSynchronize(st_);
+    if (cst_) (void)hipStreamSynchronize(cst_);
+    rs_.arm();  // the device buffers (members) are then released without further waits
     prep_halt();
     hs_.halt();
     for (auto* h : hvpos_)
@@ -2083,6 +2132,7 @@ class Engine final : public IEngine {
 
   // RadiusGenerator (radius_generator.ixx:9-28) then rsa_algo ctor (operator_algorithm.ixx:15-30)
   void algo_init(const std::vector<RadiusClass>& cls, double exclusion, int nbshots) override {
+    StreamScope scope(st_, cst_);
     set_classes(cls, exclusion);
     nbshots_ = nbshots;
     r_max_ = gmax_;
@@ -2099,6 +2149,7 @@ class Engine final : public IEngine {
   }
 
   void reset_classes(const std::vector<RadiusClass>& cls, double exclusion) override {  // :238-250
+    StreamScope scope(st_, cst_);
     if (!algo_ready_) throw Error("rsa_algo not initialised");
     const auto saved = classes_;
     const double old_max = gmax_, old_min = gmin_;
@@ -2120,10 +2171,12 @@ class Engine final : public IEngine {
   void set_miss_rate(double m) override { aimed_ = m; }
 
   void add_spheres(int64_t n, const double* pos, const double* rad, const uint64_t* phase) override {
+    StreamScope scope(st_, cst_);
     if (!algo_ready_) throw Error("rsa_algo not initialised");
     if (n <= 0) return;
     std::vector<uint8_t> cnt(geo_.ncells);
-    HIP_OK(hipMemcpy(cnt.data(), cells_cnt_.p, geo_.ncells, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpyAsync(cnt.data(), cells_cnt_.p, geo_.ncells, hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipStreamSynchronize(st_));
     std::vector<int> cell(n), slot(n);
     for (int64_t i = 0; i < n; i++) {  // build_grid(..., Validated) appends in input order
       const int64_t c = cell_index<D>(geo_, pos + i * D);
@@ -2142,12 +2195,12 @@ class Engine final : public IEngine {
     dph.reserve(n);
     std::vector<uint32_t> ph(n);
     for (int64_t i = 0; i < n; i++) ph[i] = (uint32_t)phase[i];
-    HIP_OK(hipMemcpy(dcell.p, cell.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(dslot.p, slot.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(dpos.p, pos, n * D * sizeof(double), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(drd.p, rad, n * sizeof(double), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(dph.p, ph.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(cells_cnt_.p, cnt.data(), geo_.ncells, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpyAsync(dcell.p, cell.data(), n * sizeof(int), hipMemcpyHostToDevice, st_));
+    HIP_OK(hipMemcpyAsync(dslot.p, slot.data(), n * sizeof(int), hipMemcpyHostToDevice, st_));
+    HIP_OK(hipMemcpyAsync(dpos.p, pos, n * D * sizeof(double), hipMemcpyHostToDevice, st_));
+    HIP_OK(hipMemcpyAsync(drd.p, rad, n * sizeof(double), hipMemcpyHostToDevice, st_));
+    HIP_OK(hipMemcpyAsync(dph.p, ph.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st_));
+    HIP_OK(hipMemcpyAsync(cells_cnt_.p, cnt.data(), geo_.ncells, hipMemcpyHostToDevice, st_));
     hipLaunchKernelGGL(k_insert<D>, dim3(nblocks(n)), dim3(WG), 0, st_, cs(), n, dcell.p, dslot.p, dpos.p, drd.p, dph.p);
     HIP_OK(hipStreamSynchronize(st_));
   }
@@ -2155,6 +2208,7 @@ class Engine final : public IEngine {
   uint64_t count() const override { return nb_fast_; }
 
   uint64_t extract(double* pos, double* rad, uint64_t* phase, uint64_t cap) override {
+    StreamScope scope(st_, cst_);
     if (!algo_ready_) return 0;
     const unsigned nb = nblocks(geo_.ncells);
     DevBuf<int> blk;
@@ -2173,15 +2227,21 @@ class Engine final : public IEngine {
     dph.reserve(total);
     hipLaunchKernelGGL(k_extract<D>, dim3(nb), dim3(WG), 0, st_, geo_, cs(), blk.p, dp.p, dr.p, dph.p);
     HIP_OK(hipStreamSynchronize(st_));
-    HIP_OK(hipMemcpy(pos, dp.p, (size_t)total * D * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(rad, dr.p, (size_t)total * sizeof(double), hipMemcpyDeviceToHost));
-    std::vector<uint32_t> ph(total);
-    HIP_OK(hipMemcpy(ph.data(), dph.p, (size_t)total * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    for (int i = 0; i < total; i++) phase[i] = ph[i];
+    HIP_OK(hipMemcpyAsync(pos, dp.p, (size_t)total * D * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipStreamSynchronize(st_));
+    HIP_OK(hipMemcpyAsync(rad, dr.p, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipStreamSynchronize(st_));
+    if (phase) {
+      std::vector<uint32_t> ph(total);
+      HIP_OK(hipMemcpyAsync(ph.data(), dph.p, (size_t)total * sizeof(uint32_t), hipMemcpyDeviceToHost, st_));
+      HIP_OK(hipStreamSynchronize(st_));
+      for (int i = 0; i < total; i++) phase[i] = ph[i];
+    }
     return (uint64_t)total;
   }
 
   double volume_of_spheres() override {  // rsa_grid.ixx:61-71: per real cell, cells summed in index order
+    StreamScope scope(st_, cst_);
     const uint64_t n = extract(nullptr, nullptr, nullptr, 0);
     std::vector<double> p(n * D), r(n);
     std::vector<uint64_t> ph(n);
@@ -2190,7 +2250,8 @@ class Engine final : public IEngine {
     double res = 0.;
     // the reference sums per cell then adds the cell sums; redo that grouping
     std::vector<uint8_t> cnt(geo_.ncells);
-    HIP_OK(hipMemcpy(cnt.data(), cells_cnt_.p, geo_.ncells, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpyAsync(cnt.data(), cells_cnt_.p, geo_.ncells, hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipStreamSynchronize(st_));
     uint64_t k = 0;
     for (int c = 0; c < geo_.ncells; c++) {
       if (!cnt[c]) continue;
@@ -2202,6 +2263,7 @@ class Engine final : public IEngine {
   }
 
   uint64_t voxels(int64_t* coords, double* len, uint64_t cap) override {
+    StreamScope scope(st_, cst_);
     if (len)
       for (int d = 0; d < D; d++) len[d] = vlen_[d];
     if (coords == nullptr) return (uint64_t)nvox_;
@@ -2209,11 +2271,17 @@ class Engine final : public IEngine {
     for (int d = 0; d < D; d++) {
       if (wide_[vcur_]) {
         std::vector<int64_t> tmp(nvox_);
-        if (nvox_) HIP_OK(hipMemcpy(tmp.data(), vw_[vcur_][d].p, nvox_ * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (nvox_) {
+          HIP_OK(hipMemcpyAsync(tmp.data(), vw_[vcur_][d].p, nvox_ * sizeof(int64_t), hipMemcpyDeviceToHost, st_));
+          HIP_OK(hipStreamSynchronize(st_));
+        }
         for (int64_t i = 0; i < nvox_; i++) coords[i * D + d] = tmp[i];
       } else {
         std::vector<int32_t> tmp(nvox_);
-        if (nvox_) HIP_OK(hipMemcpy(tmp.data(), vc_[vcur_][d].p, nvox_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (nvox_) {
+          HIP_OK(hipMemcpyAsync(tmp.data(), vc_[vcur_][d].p, nvox_ * sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+          HIP_OK(hipStreamSynchronize(st_));
+        }
         for (int64_t i = 0; i < nvox_; i++) coords[i * D + d] = tmp[i];
       }
     }
@@ -2244,6 +2312,21 @@ class Engine final : public IEngine {
  private:
   // ---------------------------------------------------------------- state
   static constexpr uint64_t kRing = 1ull << 23;  // words in the device ring (32 MiB)
+  // Declared first, so destroyed last: from arm() (in the destructor, once the
+  // streams are idle) until every member is gone, releases do not wait.
+  struct QuietRelease {
+    RelStreams prev;
+    bool armed = false;
+    void arm() {
+      prev = tl_rel;
+      armed = true;
+      tl_rel.set = true;
+      tl_rel.s[0] = tl_rel.s[1] = nullptr;
+    }
+    ~QuietRelease() {
+      if (armed) tl_rel = prev;
+    }
+  } rs_;
   double ginf_a[D], gsup_a[D], inf_[D], sup_[D], L_[D], cl_[D];
   double drad_ = 0.05;
   int gl_ = 1;
@@ -2286,7 +2369,7 @@ class Engine final : public IEngine {
   DevBuf<uint8_t> ctag_, lab_;
 
   // refine scratch
-  DevBuf<uint32_t> code_, perm_;
+  DevBuf<uint32_t> code_, moved_, perm_;
   DevBuf<int> blk_, holes_, ccount_, vcell_, vrank_, tsum_;
 
   // word stream: generated on the host into a pinned ring (the MT19937
@@ -2482,8 +2565,9 @@ class Engine final : public IEngine {
       lin[i] = (int)off;
     }
     stab_.reserve(2 * n3);
-    HIP_OK(hipMemcpy(stab_.p, packed.data(), n3 * sizeof(int), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(stab_.p + n3, lin.data(), n3 * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpyAsync(stab_.p, packed.data(), n3 * sizeof(int), hipMemcpyHostToDevice, st_));
+    HIP_OK(hipMemcpyAsync(stab_.p + n3, lin.data(), n3 * sizeof(int), hipMemcpyHostToDevice, st_));
+    HIP_OK(hipStreamSynchronize(st_));
     geo_.stab = stab_.p;
     geo_.slin = stab_.p + n3;
   }
@@ -2547,11 +2631,11 @@ class Engine final : public IEngine {
     cell2t_.reserve(geo_.ncells);
     cellcc_.reserve(geo_.ncells);
     ovhead_.reserve(geo_.ncells);
-    HIP_OK(hipMemset(ovhead_.p, 0xff, (size_t)geo_.ncells * sizeof(int)));
+    HIP_OK(hipMemsetAsync(ovhead_.p, 0xff, (size_t)geo_.ncells * sizeof(int), st_));
     cslot_.reserve((size_t)geo_.ncells * CSLOT);
-    HIP_OK(hipMemset(cells_cnt_.p, 0, geo_.ncells));
-    HIP_OK(hipMemset(cell2t_.p, 0xff, (size_t)geo_.ncells * sizeof(int)));
-    HIP_OK(hipMemset(cellcc_.p, 0, (size_t)geo_.ncells * sizeof(int)));
+    HIP_OK(hipMemsetAsync(cells_cnt_.p, 0, geo_.ncells, st_));
+    HIP_OK(hipMemsetAsync(cell2t_.p, 0xff, (size_t)geo_.ncells * sizeof(int), st_));
+    HIP_OK(hipMemsetAsync(cellcc_.p, 0, (size_t)geo_.ncells * sizeof(int), st_));
     nb_fast_ = 0;
   }
 
@@ -2656,9 +2740,9 @@ class Engine final : public IEngine {
     slab_.reserve(c * KMAX);
     unres_.reserve(c);
     // kernels of a draw whose gate stays closed read these before returning
-    HIP_OK(hipMemset(owner_.p, 0, c * sizeof(uint8_t)));
-    HIP_OK(hipMemset(ccell_.p, 0xff, c * sizeof(int)));
-    HIP_OK(hipMemset(ctag_.p, T_DROPPED, c * sizeof(uint8_t)));
+    HIP_OK(hipMemsetAsync(owner_.p, 0, c * sizeof(uint8_t), st_));
+    HIP_OK(hipMemsetAsync(ccell_.p, 0xff, c * sizeof(int), st_));
+    HIP_OK(hipMemsetAsync(ctag_.p, T_DROPPED, c * sizeof(uint8_t), st_));
     if (plist_.cap < 8 * c) plist_.reserve(8 * c);
   }
 
@@ -2752,6 +2836,7 @@ class Engine final : public IEngine {
     if (nvox_ == 0) return;
     const unsigned nb = nblocks(nvox_);
     code_.reserve(nvox_);
+    moved_.reserve(nvox_);
     blk_.reserve(nb);
     holes_.reserve(nvox_);
     const VoxParams<D> vp = vparams(rmin);
@@ -2792,7 +2877,8 @@ class Engine final : public IEngine {
     if (grouped) hipLaunchKernelGGL(k_block_count, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p);
     scan_excl(blk_.p, nb, dctr_.p + C_KEEP);
     hipLaunchKernelGGL(k_holes, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
-    hipLaunchKernelGGL(k_move<D>, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p, vx);
+    hipLaunchKernelGGL(k_move<D>, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, moved_.p, blk_.p, dctr_.p, holes_.p,
+                       vx);
     read_ctr();
     const int64_t keep = hctr_[C_KEEP];
     if (keep == 0) {  // subdivide_uncovered returns early on an empty list (list_of_voxels.ixx:60)
@@ -2800,7 +2886,7 @@ class Engine final : public IEngine {
       return;
     }
     const unsigned nb2 = nblocks(keep);
-    hipLaunchKernelGGL(k_child_count, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p);
+    hipLaunchKernelGGL(k_child_count, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, moved_.p, blk_.p);
     scan_excl(blk_.p, nb2, dctr_.p + C_NCHILD);
     read_ctr();
     const int64_t nchild = hctr_[C_NCHILD];
@@ -2818,7 +2904,7 @@ class Engine final : public IEngine {
     const VoxView<D> dst = vview(nxt);
     const double cw = wide_[vcur_] ? 8.0 : 4.0, nw = wide_[nxt] ? 8.0 : 4.0;
     timed("k_scatter", (double)keep * (cw * D + 4.0) + (double)nchild * nw * D, [&] {
-      hipLaunchKernelGGL(k_scatter<D>, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, blk_.p, vx, dst, vp);
+      hipLaunchKernelGGL(k_scatter<D>, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, moved_.p, blk_.p, vx, dst, vp);
     }, 1);
     HIP_OK(hipGetLastError());
     vcur_ = nxt;
@@ -3110,7 +3196,7 @@ template <int D>
 void Engine<D>::init_pub() {
   if (hpub_) return;
   dctl_.reserve(1);
-  HIP_OK(hipMemset(dctl_.p, 0, sizeof(DrawCtl)));
+  HIP_OK(hipMemsetAsync(dctl_.p, 0, sizeof(DrawCtl), st_));
   hpub_ = static_cast<DrawPub*>(
       PinnedPool::get().alloc(sizeof(DrawPub) * kPubSlots, hipHostMallocCoherent | hipHostMallocMapped));
   HIP_OK(hipHostGetDevicePointer((void**)&hpub_dev_, hpub_, 0));
@@ -3301,7 +3387,8 @@ int64_t Engine<D>::single_draw_slow(int nshots, uint64_t maximal, bool may_outre
       read_ctr();
       std::vector<uint32_t> pr(hctr_[C_CURSOR]);
       if (!pr.empty())
-        HIP_OK(hipMemcpy(pr.data(), undec_.p, pr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpyAsync(pr.data(), undec_.p, pr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st_));
+        HIP_OK(hipStreamSynchronize(st_));
       std::vector<uint64_t> prio(pr.begin(), pr.end());
       const double pivot = mom_pivot(prio, (int64_t)maximal);
       int64_t fin = 0;
@@ -3336,6 +3423,7 @@ int64_t Engine<D>::single_draw_slow(int nshots, uint64_t maximal, bool may_outre
 template <int D>
 void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
   if (!algo_ready_) throw Error("rsa_algo not initialised");
+  StreamScope scope(st_, cst_);
   const auto t0 = std::chrono::steady_clock::now();
   std::memset(st, 0, sizeof(*st));
   reset_dev_ring();  // no copy may read the host ring once the producer restarts

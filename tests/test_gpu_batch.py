@@ -1,0 +1,66 @@
+"""The batched form: 256 independent 3D realizations (box edge L ~ U[0.2, 1],
+radius r ~ U[0.004, 0.008], seeds 0..255; tests/golden/cfg4_batch.json) run
+several at a time on the GPU.  Each realization is uniform_generate<3, 1> on
+[0, L]^3 (operator_algorithm.ixx:275-292); its placed set must be the oracle's,
+whatever the number of concurrent workers."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from _common import set_hash
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "cfg4_batch.json")
+
+
+def load():
+    return json.load(open(FIX))
+
+
+def test_batch_fixture_regenerates_from_meta_seed():
+    """The realization list is numpy PCG64(20261203): L ~ U[0.2, 1.0], then r ~ U[0.004, 0.008]."""
+    fx = load()
+    g = np.random.Generator(np.random.PCG64(fx["meta_seed"]))
+    L = g.uniform(0.2, 1.0, size=256)
+    r = g.uniform(0.004, 0.008, size=256)
+    assert [c["L"] for c in fx["realizations"]] == L.tolist()
+    assert [c["r"] for c in fx["realizations"]] == r.tolist()
+    assert [c["seed"] for c in fx["realizations"]] == list(range(256))
+    assert [c["index"] for c in fx["oracle"]] == list(range(256))
+    assert all(c["end"].endswith("fully packed") for c in fx["oracle"])
+
+
+def test_batch_rejects_bad_arguments():
+    import rsa_mpi_amd as R
+    with pytest.raises(RuntimeError, match="dim must be"):
+        R.batch_run(5, [1.0], [0.01], [0])
+
+
+@pytest.mark.gpu
+def test_cfg4_batch_counts_match_oracle():
+    """All 256 realizations at full size, 4 at a time: counts equal the oracle's."""
+    import rsa_mpi_amd as R
+    fx = load()
+    real = fx["realizations"]
+    out = R.batch_run(3, [c["L"] for c in real], [c["r"] for c in real], [c["seed"] for c in real], workers=4)
+    want = np.array([c["n"] for c in fx["oracle"]], dtype=np.uint64)
+    bad = np.nonzero(out["counts"] != want)[0]
+    assert len(bad) == 0, (bad[:10].tolist(), out["counts"][bad[:10]].tolist(), want[bad[:10]].tolist())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("workers", [1, 3, 8])
+def test_cfg4_batch_sets_match_oracle(workers):
+    """Every 8th realization extracted: the placed sets are the oracle's bit for bit."""
+    import rsa_mpi_amd as R
+    fx = load()
+    idx = list(range(workers % 8, 256, 8))
+    real = [fx["realizations"][k] for k in idx]
+    out = R.batch_run(3, [c["L"] for c in real], [c["r"] for c in real], [c["seed"] for c in real],
+                      workers=workers, extract=True)
+    for k, (pos, rad) in zip(idx, out["spheres"]):
+        o = fx["oracle"][k]
+        assert len(rad) == o["n"], k
+        assert set_hash(pos, rad) == o["sha256"], k
