@@ -70,6 +70,20 @@ int rsa_gpu_algo_init(rsa_gpu_ctx* ctx, int n, const double* radius, const uint6
 int rsa_gpu_algo_init_fractions(rsa_gpu_ctx* ctx, int n, const double* radius, const double* fraction,
                                 const int* phase, double volume, double exclusion, int nbshots_singledraw);
 
+/* A custom radius law: radius[i] = law(u[i]) for i < n, u uniform in [0, 1).
+ * It must be a pure function of u (prepared draws that a refinement discards
+ * are evaluated again). */
+typedef void (*rsa_gpu_radius_law)(const double* u, double* radius, int64_t n, void* user);
+
+/* rsa_algo constructor with RadiusGenerator(min_radius, max_radius,
+ * RandomRadiusGenerator*, count, exclusion) (radius_generator.hxx:62-70,
+ * radius_generator.ixx:38-48): `count` spheres whose radii are law(u), u drawn
+ * from 2 words per shot after the priorities (radius_generator.hxx:25-33), all
+ * of phase `phase`.  The law runs on a host thread of the library, once per
+ * draw for all its shots; its radii must lie in [min_radius, max_radius]. */
+int rsa_gpu_algo_init_law(rsa_gpu_ctx* ctx, double min_radius, double max_radius, rsa_gpu_radius_law law,
+                          void* user, uint64_t phase, uint64_t count, double exclusion, int nbshots_singledraw);
+
 /* rsa_algo<DIM>::reset_radius_generator (operator_algorithm.ixx:238-250): new
  * classes, placed spheres kept. */
 int rsa_gpu_reset_classes(rsa_gpu_ctx* ctx, int n, const double* radius, const uint64_t* count,

@@ -18,10 +18,13 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <filesystem>
 #include <fstream>
 #include <functional>
 #include <iostream>
 #include <memory>
+#include <random>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -89,6 +92,14 @@ class RandomRadiusGenerator {
  public:
   RandomRadiusGenerator(std::function<double(double)> transform_, uint64_t phase_)
       : transform{std::move(transform_)}, phase{phase_} {}
+  //! radius_generator.hxx:25-33: (phases, radii) of a_size shots, u uniform in [0, 1)
+  std::tuple<std::vector<int>, std::vector<double>> operator()(size_t a_size, std::mt19937& random_generator) const {
+    std::uniform_real_distribution<> ulaw(0., 1.);
+    std::tuple<std::vector<int>, std::vector<double>> result(std::vector<int>(a_size, (int)phase),
+                                                             std::vector<double>(a_size, 0.));
+    for (size_t i = 0; i < a_size; i++) std::get<1>(result)[i] = transform(ulaw(random_generator));
+    return result;
+  }
   std::function<double(double)> transform;
   uint64_t phase;
 };
@@ -117,11 +128,17 @@ class RadiusGenerator {
     }
     normalise();
   }
-  // Custom radius laws (radius_generator.hxx:70-76) are evaluated per candidate
-  // on the host by the reference; the GPU path has no host callback per shot.
-  RadiusGenerator(double, double, const RANDOM_RADIUS_GENERATOR*, size_t, double = 0) {
-    throw std::runtime_error("custom radius laws are not available on the GPU path");
-  }
+  //! custom law (radius_generator.ixx:38-48): one class (0, n, 0) whose radii are
+  //! law->transform(u); the library evaluates the law on a host thread, once per draw
+  RadiusGenerator(double min_radius, double max_radius, const RANDOM_RADIUS_GENERATOR* randomRadiusGenerator,
+                  size_t desired_nb_spheres, double exclusion_distance = 0)
+      : classes_{std::make_tuple(0., (uint64_t)desired_nb_spheres, 0)},
+        sorted_(classes_),
+        exclusion_(exclusion_distance),
+        max_radius_(max_radius),
+        min_radius_(min_radius),
+        law_(randomRadiusGenerator) {}
+  const RANDOM_RADIUS_GENERATOR* custom_law() const { return law_; }
 
   bool is_there_still_radii() const { return cur_ < classes_.size(); }
   uint64_t get_current_number() const { return get<1>(sorted_[cur_]); }
@@ -150,6 +167,7 @@ class RadiusGenerator {
   }
   vector<tuple<double, uint64_t, int>> classes_, sorted_;
   double exclusion_ = 0, max_radius_ = 0, min_radius_ = 0;
+  const RANDOM_RADIUS_GENERATOR* law_ = nullptr;
   size_t cur_ = 0;
 };
 
@@ -369,7 +387,20 @@ class rsa_algo {
   const rsa_gpu_stats& last_stats() const { return last_stats_; }
 
  private:
+  // the C-ABI law: the generator's transform applied to each u
+  static void law_trampoline(const double* u, double* radius, int64_t n, void* user) {
+    const auto* g = static_cast<const RandomRadiusGenerator*>(user);
+    for (int64_t i = 0; i < n; i++) radius[i] = g->transform(u[i]);
+  }
   void install(bool first) {
+    if (const RandomRadiusGenerator* law = m_radius_generator.custom_law()) {
+      if (!first) throw std::runtime_error("reset_radius_generator with a custom radius law is not supported");
+      rsa_mpi_amd_detail::check(rsa_gpu_algo_init_law(
+          m_domain.ctx(), m_radius_generator.get_min_radius(), m_radius_generator.get_max_radius(), &law_trampoline,
+          const_cast<RandomRadiusGenerator*>(law), law->phase, m_radius_generator.get_current_number(),
+          m_radius_generator.exclusion_distance(), m_nbshots_singledraw));
+      return;
+    }
     const auto& cls = m_radius_generator.raw_classes();
     std::vector<double> r;
     std::vector<uint64_t> n;
@@ -433,6 +464,101 @@ void init(Args...) {
 inline void finalize() { message("=========== end ==========="); }
 }  // namespace rsa_mpi
 
+// ------------------------------------------------------------ operator_paraview.hxx:5-37
+// VTK XML PolyData of the real spheres (operator_paraview.ixx:12-142): per call
+// `step`, a_name/rsa_mpi_<step>.pvtp naming one piece per rank (one here) and
+// a_name/rsa_mpi_<step>/rsa_mpi_<step>_0.vtp with point data Id (phase), Tag
+// (TypeTag::Validated) and Radius, and the centres as points, as ASCII with the
+// stream's default precision.  The reference reads three coordinates whatever
+// DIM is; here D = 2 writes z = 0.
+namespace rsa_paraview {
+inline int& output_step() {
+  static int step = 0;
+  return step;
+}
+
+inline void write_pvtp(const std::string& basedir, const std::string& basename, size_t number_of_files) {
+  const std::string name = basedir + "/" + basename + ".pvtp";
+  std::ofstream out(name);
+  if (!out) {
+    std::cerr << "Erreur : impossible de créer le fichier de sortie suivant: " << name << std::endl;
+    return;
+  }
+  out << " <VTKFile type=\"PPolyData\"> " << std::endl;
+  out << "   <PPolyData GhostLevel=\"0\">" << std::endl;
+  out << "    <PPointData Scalar=\"id, tag\">" << std::endl;
+  out << "      <PDataArray type=\"Int64\" Name=\"Id\"/>" << std::endl;
+  out << "      <PDataArray type=\"Int32\" Name=\"Tag\"/>" << std::endl;
+  out << "      <PDataArray type=\"Float64\" Name=\"Radius\"/>" << std::endl;
+  out << "    </PPointData>" << std::endl;
+  out << "     <PPoints>" << std::endl;
+  out << "       <PDataArray type=\"Float64\" NumberOfComponents=\"3\"/>" << std::endl;
+  out << "     </PPoints> " << std::endl;
+  for (size_t i = 0; i < number_of_files; i++)
+    out << "     <Piece Source=\"" << basename << "/" << basename << "_" << i << ".vtp\"/>" << std::endl;
+  out << "   </PPolyData>" << std::endl;
+  out << " </VTKFile>" << std::endl;
+}
+
+template <int DIM>
+void write_vtp(const std::string& name, const std::vector<rsa_sphere<DIM>>& spheres) {
+  std::ofstream out(name + ".vtp");
+  if (!out) {
+    std::cerr << "Erreur : impossible de créer le fichier de sortie suivant: " << name << ".vtp" << std::endl;
+    return;
+  }
+  std::stringstream pts, ids, tags, rads;
+  for (const auto& s : spheres) {
+    for (int d = 0; d < 3; d++) pts << (d < DIM ? s.center[d] : 0.) << " ";
+    ids << s.phase << " ";
+    tags << int(TypeTag::Validated) << " ";
+    rads << s.radius << " ";
+  }
+  out << "<VTKFile type=\"PolyData\">" << std::endl;
+  out << " <PolyData>" << std::endl;
+  out << "   <Piece NumberOfPoints=\"" << spheres.size() << "\" NumberOfPolys=\"" << 0 << "\">" << std::endl;
+  out << "   <PointData>" << std::endl;
+  out << "     <DataArray type=\"Int64\" Name=\"Id\" format=\"ascii\">" << std::endl;
+  out << ids.str() << std::endl;
+  out << "     </DataArray>" << std::endl;
+  out << "     <DataArray type=\"Int32\" Name=\"Tag\" format=\"ascii\">" << std::endl;
+  out << tags.str() << std::endl;
+  out << "     </DataArray>" << std::endl;
+  out << "     <DataArray type=\"Float64\" Name=\"Radius\" format=\"ascii\">" << std::endl;
+  out << rads.str() << std::endl;
+  out << "     </DataArray>" << std::endl;
+  out << "   </PointData>" << std::endl;
+  out << "   <Points>" << std::endl;
+  out << "     <DataArray type=\"Float64\" Name=\"\"  NumberOfComponents=\"3\" format=\"ascii\">" << std::endl;
+  out << pts.str() << std::endl;
+  out << "     </DataArray>" << std::endl;
+  out << "   </Points>" << std::endl;
+  out << "  </Piece>" << std::endl;
+  out << " </PolyData>" << std::endl;
+  out << "</VTKFile>" << std::endl;
+}
+
+//! operator_paraview.ixx:98-136 on one rank
+template <int DIM>
+void paraview(std::string a_name, const rsa_grid<DIM>& a_grid) {
+  const int step = output_step();
+  const std::string basename = "rsa_mpi";
+  const std::string stepname = basename + "_" + std::to_string(step);
+  const std::string directory = a_name + "/" + stepname;
+  std::filesystem::create_directory(a_name);
+  std::filesystem::create_directory(directory);
+  write_pvtp(a_name, stepname, 1);
+  write_vtp<DIM>(directory + "/" + stepname + "_0", a_grid.extract_spheres());
+  output_step()++;
+}
+
+//! operator_paraview.ixx:138-142
+template <int DIM>
+void paraview(const rsa_domain<DIM>& a_domain, std::string a_name = "ParaviewOutput") {
+  paraview<DIM>(a_name, a_domain.get_grid());
+}
+}  // namespace rsa_paraview
+
 // ------------------------------------------------------------ test.hxx:13-32
 //! The reference passes volume 1 (not prod(L)) to the radius generator; kept.
 template <int DIM, int method>
@@ -446,7 +572,10 @@ std::vector<rsa_sphere<DIM>> throw_spheres(sac_de_billes::Point<DIM> L,
   rsa_domain<DIM> domain(inf, sup, 1, radius_generator.get_max_radius());
   domain.domain_log();
   algorithm::uniform_generate<DIM, method>(domain, radius_generator, 6000, 10, seed);
-  (void)write_paraview;  // Paraview output is not part of the GPU path
+  if (write_paraview) {
+    rsa_mpi::message("====> write mpi output file <=====");
+    rsa_paraview::paraview(domain);
+  }
   return domain.extract_spheres();
 }
 
@@ -463,6 +592,8 @@ class rsa_domain {
   double compute_total_volume_of_spheres() const { return rsa_domain_ptr_->compute_total_volume_of_spheres(); }
   void domain_log() const { rsa_domain_ptr_->domain_log(); }
   std::vector<rsa_sphere<DIM>> extract_spheres() const { return rsa_domain_ptr_->extract_spheres(); }
+  //! user_interface.hxx:40
+  void paraview() const { rsa_paraview::paraview(*rsa_domain_ptr_); }
   const ::rsa_domain<DIM>& get_pointed_to() const { return *rsa_domain_ptr_; }
   ::rsa_domain<DIM>& get_pointed_to() { return *rsa_domain_ptr_; }
 

@@ -15,8 +15,8 @@ import numpy as np
 from . import _lib
 from ._lib import Stats, check, dptr, f64, lib, mt_words  # noqa: F401
 
-__all__ = ["Domain", "RadiusGenerator", "Algo", "Stats", "mt_words", "uniform_generate", "throw_spheres",
-           "set_nb_threads"]
+__all__ = ["Domain", "RadiusGenerator", "LawRadiusGenerator", "Algo", "Stats", "mt_words", "uniform_generate",
+           "throw_spheres", "batch_run", "set_nb_threads"]
 
 
 def set_nb_threads(n):
@@ -179,14 +179,47 @@ class RadiusGenerator:
                 np.ascontiguousarray(self.phase, dtype=np.int32))
 
 
+class LawRadiusGenerator:
+    """RadiusGenerator<DIM>(min_radius, max_radius, RandomRadiusGenerator*, n, exclusion)
+    (radius_generator.hxx:62-70, radius_generator.ixx:38-48): `count` spheres whose
+    radii are transform(u), u uniform in [0, 1) (radius_generator.hxx:12-38).
+    `transform` maps a NumPy array of u to an array of radii in [min_radius, max_radius]
+    and must be a pure function of u; it is called from a library thread, once per draw."""
+
+    def __init__(self, min_radius, max_radius, transform, count, phase=0, exclusion=0.0):
+        self.min_radius = float(min_radius)
+        self.max_radius = float(max_radius)
+        self.transform = transform
+        self.count = int(count)
+        self.phase = int(phase)
+        self.exclusion = float(exclusion)
+
+    def _callback(self):
+        tf = self.transform
+
+        def law(u, r, n, user):
+            uu = np.ctypeslib.as_array(u, shape=(n,))
+            out = np.ctypeslib.as_array(r, shape=(n,))
+            out[:] = np.asarray(tf(uu), dtype=np.float64)
+        return _lib.RADIUS_LAW(law)
+
+
 class Algo:
     """algorithm::rsa_algo<DIM> (operator_algorithm.hxx:18-109), voxel method."""
 
     def __init__(self, domain, radius_generator, nbshots_singledraw=6000, n_draw=10):
         self.domain = domain
-        r, c, p = radius_generator._arrays(domain.dim)
-        check(lib().rsa_gpu_algo_init(domain._ctx, len(r), dptr(r), c.ctypes.data_as(_lib._U64),
-                                      p.ctypes.data_as(_lib._I), radius_generator.exclusion, int(nbshots_singledraw)))
+        if isinstance(radius_generator, LawRadiusGenerator):
+            g = radius_generator
+            self._law = g._callback()  # kept alive with the algorithm
+            domain._law = self._law   # and with the domain, whose context calls it
+            check(lib().rsa_gpu_algo_init_law(domain._ctx, g.min_radius, g.max_radius, self._law, None,
+                                              g.phase, g.count, g.exclusion, int(nbshots_singledraw)))
+        else:
+            r, c, p = radius_generator._arrays(domain.dim)
+            check(lib().rsa_gpu_algo_init(domain._ctx, len(r), dptr(r), c.ctypes.data_as(_lib._U64),
+                                          p.ctypes.data_as(_lib._I), radius_generator.exclusion,
+                                          int(nbshots_singledraw)))
         self.last_stats = None
 
     def proceed(self, seed, max_draws=-1):

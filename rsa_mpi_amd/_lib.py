@@ -17,8 +17,12 @@ EXPORTED = (
     "rsa_gpu_algo_init", "rsa_gpu_algo_init_fractions", "rsa_gpu_reset_classes",
     "rsa_gpu_set_miss_rate", "rsa_gpu_proceed", "rsa_gpu_add_spheres", "rsa_gpu_count",
     "rsa_gpu_extract", "rsa_gpu_volume_of_spheres", "rsa_gpu_voxels", "rsa_gpu_mt_words",
-    "rsa_gpu_profile_enable", "rsa_gpu_kernel_profile", "rsa_gpu_batch_run",
+    "rsa_gpu_profile_enable", "rsa_gpu_kernel_profile", "rsa_gpu_batch_run", "rsa_gpu_algo_init_law",
 )
+
+# rsa_gpu_radius_law: void (*)(const double* u, double* radius, int64_t n, void* user)
+RADIUS_LAW = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
+                              ctypes.c_void_p)
 
 
 class Stats(ctypes.Structure):
@@ -69,6 +73,8 @@ def lib():
     L.rsa_gpu_mt_words.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _U32]
     L.rsa_gpu_profile_enable.argtypes = [_P, ctypes.c_int]
     L.rsa_gpu_kernel_profile.argtypes = [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _I64, _D, _D]
+    L.rsa_gpu_algo_init_law.argtypes = [_P, ctypes.c_double, ctypes.c_double, RADIUS_LAW, ctypes.c_void_p,
+                                        ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, ctypes.c_int]
     L.rsa_gpu_batch_run.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _U64, ctypes.c_int, ctypes.c_int, _U64, _D,
                                     _D, _D, ctypes.c_uint64, _D]
     _lib = L

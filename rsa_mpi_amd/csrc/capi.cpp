@@ -120,6 +120,13 @@ int rsa_gpu_algo_init_fractions(rsa_gpu_ctx* ctx, int n, const double* radius, c
   });
 }
 
+int rsa_gpu_algo_init_law(rsa_gpu_ctx* ctx, double min_radius, double max_radius, rsa_gpu_radius_law law,
+                          void* user, uint64_t phase, uint64_t count, double exclusion, int nbshots_singledraw) {
+  if (need_ctx(ctx)) return 1;
+  return guarded(
+      [&] { ctx->eng->algo_init_law(min_radius, max_radius, law, user, phase, count, exclusion, nbshots_singledraw); });
+}
+
 int rsa_gpu_reset_classes(rsa_gpu_ctx* ctx, int n, const double* radius, const uint64_t* count, const int* phase,
                           double exclusion) {
   if (need_ctx(ctx)) return 1;

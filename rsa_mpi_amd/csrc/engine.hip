@@ -323,7 +323,7 @@ struct ClassDev {
 // each centre (build_grid, rsa_grid.ixx:182-187).
 template <int D>
 __global__ void k_decode(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0, int nshots, uint32_t N,
-                         const uint32_t* __restrict__ vpos, VoxView<D> vx, ClassDev cls,
+                         const uint32_t* __restrict__ vpos, const double* __restrict__ hrad, VoxView<D> vx, ClassDev cls,
                          Geo<D> g, double* __restrict__ cpos, double* __restrict__ crad,
                          uint32_t* __restrict__ cprio, uint32_t* __restrict__ cphase, int* __restrict__ ccell,
                          uint8_t* __restrict__ ctag, int* ctr) {
@@ -341,7 +341,7 @@ __global__ void k_decode(const uint32_t* __restrict__ ring, uint64_t mask, uint6
   }
   const int end_rel = ctr[C_END_REL];
   cprio[i] = ring_at(ring, mask, w0 + (uint64_t)end_rel + i) >> 1;
-  crad[i] = cls.radius;
+  crad[i] = hrad ? hrad[i] : cls.radius;
   cphase[i] = cls.phase;
   const int64_t cell = cell_index<D>(g, p);
   bool ok = cell >= 0 && cell < g.ncells;
@@ -831,7 +831,7 @@ constexpr int kPubSlots = 8;
 template <int D>
 __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
                                                       int nshots, uint32_t N, const uint32_t* __restrict__ vpos,
-                                                      RejList rl,
+                                                      const double* __restrict__ hrad, RejList rl,
                                                       int end_rel, VoxView<D> vx, ClassDev cls, Geo<D> g,
                                                       CellStore<D> cs, double* __restrict__ cpos,
                                                       double* __restrict__ crad, uint32_t* __restrict__ cprio,
@@ -854,7 +854,10 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
   int64_t cell = -1;
   bool ok = false;
   uint32_t prio = 0;
+  // radius: the class radius, or the shot's value of a custom law (host-evaluated)
+  double r = cls.radius;
   if (valid) {
+    if (hrad) r = hrad[i];
     prio = ring_at(ring, mask, w0 + (uint64_t)end_rel + i) >> 1;
     uint32_t v;
     if (rl.n >= 0) {
@@ -882,7 +885,6 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
   }
   bool hit = false;
   if (valid && ok) {
-    const double r = cls.radius;
     int cc[D];
     cell_coords<D>(g, cell, cc);
 #pragma unroll
@@ -916,7 +918,7 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
 #pragma unroll
     for (int d = 0; d < D; d++) cpos[(size_t)d * nshots + i] = p[d];
     cprio[i] = prio;
-    crad[i] = cls.radius;
+    crad[i] = r;
     cphase[i] = cls.phase;
     if (!ok) {
       ccell[i] = -1;
@@ -2099,6 +2101,9 @@ class Engine final : public IEngine {
       hvpos_[b] = static_cast<uint32_t*>(PinnedPool::get().alloc((kFastShots + 1) * sizeof(uint32_t),
                                                                  hipHostMallocCoherent | hipHostMallocMapped));
       HIP_OK(hipHostGetDevicePointer((void**)&hvpos_dev_[b], hvpos_[b], 0));
+      hrad_[b] = static_cast<double*>(PinnedPool::get().alloc((kFastShots + 1) * sizeof(double),
+                                                              hipHostMallocCoherent | hipHostMallocMapped));
+      HIP_OK(hipHostGetDevicePointer((void**)&hrad_dev_[b], hrad_[b], 0));
     }
     aimed_ = D <= 3 ? 0.95 : 1 - 0.05 * puissance<(D > 3 ? D - 3 : 0)>(0.1);
   }
@@ -2111,6 +2116,9 @@ class Engine final : public IEngine {
     hs_.halt();
     for (auto* h : hvpos_)
       PinnedPool::get().release(h);
+    for (auto* h : hrad_)
+      PinnedPool::get().release(h);
+    PinnedPool::get().release(hrad_big_);
     PinnedPool::get().release(hs_.buf);
     PinnedPool::get().release(hpub_);
     if (pe0_) (void)hipEventDestroy(pe0_);
@@ -2141,6 +2149,25 @@ class Engine final : public IEngine {
     if (gmax_ > drad_)
       throw Error("Impossible to have the maximal radius of radius generator larger than domain implicit radius");
     // cells beyond the ghost-band width would need images further than one layer
+    for (int d = 0; d < D; d++)
+      if (2 * drad_ > cl_[d])
+        throw Error("domain radius too large for the cell grid: the ghost band must fit in one cell layer");
+    alloc_cells();
+    algo_ready_ = true;
+  }
+
+  // rsa_algo constructor with a custom-law RadiusGenerator (radius_generator.ixx:38-48)
+  void algo_init_law(double rmin, double rmax, rsa_gpu_radius_law law, void* user, uint64_t phase, uint64_t count,
+                     double exclusion, int nbshots) override {
+    StreamScope scope(st_, cst_);
+    (void)exclusion;  // kept by the reference's generator, not added to the law's radii
+    set_law(rmin, rmax, law, user, phase, count);
+    nbshots_ = nbshots;
+    r_max_ = gmax_;
+    aimed_ = D <= 3 ? 0.95 : 1 - 0.05 * puissance<(D > 3 ? D - 3 : 0)>(0.1);
+    define_grid(gmax_);
+    if (gmax_ > drad_)
+      throw Error("Impossible to have the maximal radius of radius generator larger than domain implicit radius");
     for (int d = 0; d < D; d++)
       if (2 * drad_ > cl_[d])
         throw Error("domain radius too large for the cell grid: the ghost band must fit in one cell layer");
@@ -2574,6 +2601,7 @@ class Engine final : public IEngine {
 
   void set_classes(const std::vector<RadiusClass>& cls, double exclusion) {  // radius_generator.ixx:9-28
     if (cls.empty()) throw Error("Empty radius generator!");
+    custom_ = false;
     classes_.clear();
     for (const auto& c : cls) classes_.emplace_back(c.radius, c.count, c.phase);
     std::sort(classes_.begin(), classes_.end(), [](auto& a, auto& b) { return std::get<0>(a) > std::get<0>(b); });
@@ -2581,6 +2609,26 @@ class Engine final : public IEngine {
     gmax_ = std::get<0>(classes_[0]);
     gmin_ = std::get<0>(classes_[classes_.size() - 1]);
     cur_ = 0;
+  }
+
+  // RadiusGenerator(min, max, RandomRadiusGenerator*, n, exclusion) (radius_generator.ixx:38-48): one
+  // class (0, n, 0) whose radii come from the law; the exclusion distance is kept but not added
+  void set_law(double rmin, double rmax, rsa_gpu_radius_law law, void* user, uint64_t phase, uint64_t count) {
+    if (!law) throw Error("null radius law");
+    if (!(rmin > 0) || rmax < rmin) throw Error("radius law bounds must satisfy 0 < min_radius <= max_radius");
+    classes_.clear();
+    classes_.emplace_back(0., count, 0);
+    gmax_ = rmax;
+    gmin_ = rmin;
+    cur_ = 0;
+    custom_ = true;
+    law_ = law;
+    law_user_ = user;
+    law_phase_ = (uint32_t)phase;
+  }
+  ClassDev class_dev() const {
+    if (custom_) return ClassDev{0., law_phase_};
+    return ClassDev{std::get<0>(classes_[cur_]), (uint32_t)std::get<2>(classes_[cur_])};
   }
 
   // Spheres per cell for radii in [rmin, rmax].  Disjoint balls of radius rmin
@@ -2927,14 +2975,17 @@ class Engine final : public IEngine {
     int id = 0, seq = 0;     // draw id (gate value) and first sweep's seq, once queued
     uint64_t miss_base = 0;  // spheres placed before the draw since the miss-rate reference point
     RejList rej{};           // the chain as launch arguments (n < 0: use the pinned offsets)
-    uint64_t w_end() const { return w_shots + (uint64_t)end_rel + (uint64_t)nshots; }
+    bool law = false;        // custom radius law: 2 words per shot follow the priorities
+    uint64_t w_end() const { return w_shots + (uint64_t)end_rel + (uint64_t)nshots * (law ? 3 : 1); }
   };
+  void law_radii(const Prep& pr, double* out);
   void init_pub();
   void enqueue_round(const Prep& pr, int seq, int gate_id);
   void enqueue_first(Prep& pr, bool forced, uint64_t miss_base);
   int64_t finish_draw(const Prep& pr, int64_t& rounds, bool* next_live, bool& gate_open, rsa_gpu_stats* st);
   int draw_id_ = 0;
-  int64_t single_draw_slow(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
+  int64_t single_draw_slow(int nshots, const double* hrad, uint64_t maximal, bool may_outreach, uint64_t w0,
+                           int64_t& rounds,
                            uint64_t& w_next, rsa_gpu_stats* st);
   int host_chain(int nshots, uint64_t w0, uint32_t N, int b, RejList* rej);
   Prep prepare(uint64_t w, int64_t nvox, int b);
@@ -2961,6 +3012,16 @@ class Engine final : public IEngine {
   // used round-robin by the helper thread and never freed during a run
   uint32_t* hvpos_[kPrepBufs] = {};
   const uint32_t* hvpos_dev_[kPrepBufs] = {};
+  // custom radius law: per-shot radii of prepared draws (pinned, read by k_decode_filter)
+  double* hrad_[kPrepBufs] = {};
+  const double* hrad_dev_[kPrepBufs] = {};
+  double* hrad_big_ = nullptr;  // radii of host-driven draws with more shots
+  const double* hrad_big_dev_ = nullptr;
+  size_t hrad_big_cap_ = 0;
+  bool custom_ = false;
+  rsa_gpu_radius_law law_ = nullptr;
+  void* law_user_ = nullptr;
+  uint32_t law_phase_ = 0;
   DevBuf<int> next_, cslot_, ovhead_, slab_, pool_, unres_;
   DevBuf<uint8_t> owner_;
   DevBuf<int> stab_;  // packed stencil offsets (Geo::stab), then linear ones (Geo::slin)
@@ -3050,6 +3111,23 @@ int Engine<D>::host_chain(int nshots, uint64_t w0, uint32_t N, int b, RejList* r
   return (int)(p - w0);
 }
 
+// Radii of a draw's shots under a custom law: law(u) with u = generate_canonical
+// over the 2 words per shot that follow the priorities (uniform_real_distribution
+// on [0, 1), random.tcc:3346-3379; radius_generator.hxx:25-33).
+template <int D>
+void Engine<D>::law_radii(const Prep& pr, double* out) {
+  thread_local std::vector<double> u;
+  u.resize((size_t)pr.nshots);
+  const uint64_t w = pr.w_shots + (uint64_t)pr.end_rel + (uint64_t)pr.nshots;
+  hs_.ensure(w + 2 * (uint64_t)pr.nshots);
+  for (int i = 0; i < pr.nshots; i++) u[i] = canonical(hs_.at(w + 2 * (uint64_t)i), hs_.at(w + 2 * (uint64_t)i + 1));
+  law_(u.data(), out, pr.nshots, law_user_);
+  // the cell grid is sized for max_radius and the voxel test uses min_radius
+  for (int i = 0; i < pr.nshots; i++)
+    if (!(out[i] >= gmin_ && out[i] <= gmax_))
+      throw Error("radius law returned " + std::to_string(out[i]) + ", outside [min_radius, max_radius]");
+}
+
 // Poisson shot count(s) (compute_nb_shots_voxel, operator_algorithm.ixx:208-223)
 // from position w until a draw has shots, then its rejection chain into pinned
 // buffer b.  A prepared draw stays valid while the voxel count is unchanged.
@@ -3081,6 +3159,8 @@ typename Engine<D>::Prep Engine<D>::prepare(uint64_t w, int64_t nvox, int b) {
   p.w_shots = w;
   p.rej.n = 0;
   p.end_rel = host_chain(p.nshots, w, (uint32_t)nvox, b, &p.rej);
+  p.law = custom_;
+  if (p.law && p.nshots <= kFastShots) law_radii(p, hrad_[b]);  // larger draws: in single_draw_slow
   return p;
 }
 
@@ -3240,13 +3320,14 @@ void Engine<D>::enqueue_first(Prep& pr, bool forced, uint64_t miss_base) {
   // until the draw's results have been published).
   host_timed("host_dev_words", [&] { dev_words(w0, w0 + (uint64_t)pr.end_rel + (uint64_t)nshots); });
   const uint32_t N = (uint32_t)nvox_;
-  ClassDev cls{std::get<0>(classes_[cur_]), (uint32_t)std::get<2>(classes_[cur_])};
+  const ClassDev cls = class_dev();
   using LG = LaneGroup<D>;
   const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
   const unsigned nbs = nblocks(nshots);
   timed("k_decode_filter", (double)nshots * (4.0 * (2 * D + 2) + 4.0 * D + 8.0 * (D + 1) + 13.0), [&] {
     hipLaunchKernelGGL(k_decode_filter<D>, dim3(nbl), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N,
-                       hvpos_dev_[pr.buf], pr.rej, pr.end_rel, vview(), cls, geo_, cs(), cpos_.p, crad_.p, cprio_.p,
+                       hvpos_dev_[pr.buf], pr.law ? hrad_dev_[pr.buf] : nullptr, pr.rej, pr.end_rel, vview(), cls,
+                       geo_, cs(), cpos_.p, crad_.p, cprio_.p,
                        cphase_.p, ccell_.p, ctag_.p, owner_.p, cellcc_.p, cslot_.p, ovhead_.p, next_.p, dctl_.p, pr.id,
                        forced ? 1 : 0);
   });
@@ -3310,7 +3391,8 @@ int64_t Engine<D>::finish_draw(const Prep& pr, int64_t& rounds, bool* next_live,
 // Host-driven form of a draw; used when the draw may overshoot the class target
 // (retain_only_first_elements needs the host-side pivot between sweep and update).
 template <int D>
-int64_t Engine<D>::single_draw_slow(int nshots, uint64_t maximal, bool may_outreach, uint64_t w0, int64_t& rounds,
+int64_t Engine<D>::single_draw_slow(int nshots, const double* hrad, uint64_t maximal, bool may_outreach, uint64_t w0,
+                                    int64_t& rounds,
                                uint64_t& w_next, rsa_gpu_stats* st) {
   ensure_cand(nshots);
   const uint32_t N = (uint32_t)nvox_;
@@ -3332,8 +3414,8 @@ int64_t Engine<D>::single_draw_slow(int nshots, uint64_t maximal, bool may_outre
   }
   const int end_rel = hctr_[C_END_REL];
   dev_words(w0, w0 + (uint64_t)end_rel + (uint64_t)nshots);
-  ClassDev cls{std::get<0>(classes_[cur_]), (uint32_t)std::get<2>(classes_[cur_])};
-  hipLaunchKernelGGL(k_decode<D>, dim3(nbs), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N, vpos_.p,
+  const ClassDev cls = class_dev();
+  hipLaunchKernelGGL(k_decode<D>, dim3(nbs), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N, vpos_.p, hrad,
                      vview(), cls, geo_, cpos_.p, crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p, dctr_.p);
   hipLaunchKernelGGL(k_bin_count, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, cellcc_.p, carr_.p, cell2t_.p,
                      touched_.p, dctr_.p);
@@ -3476,7 +3558,23 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     if (may_outreach || nshots > 48000) {
       if (cur_live) throw Error("internal: a queued draw needs the host-driven form");
       uint64_t w_next = 0;
-      nb_new = (uint64_t)single_draw_slow(cur.nshots, maximal, may_outreach, cur.w_shots, rounds, w_next, st);
+      const double* hrad = nullptr;
+      if (cur.law) {
+        if (cur.nshots <= kFastShots) {
+          hrad = hrad_dev_[cur.buf];
+        } else {  // radii of a draw too large for the prepared buffers
+          if (hrad_big_cap_ < (size_t)cur.nshots) {
+            PinnedPool::get().release(hrad_big_);
+            hrad_big_cap_ = (size_t)cur.nshots;
+            hrad_big_ = static_cast<double*>(PinnedPool::get().alloc(hrad_big_cap_ * sizeof(double),
+                                                                     hipHostMallocCoherent | hipHostMallocMapped));
+            HIP_OK(hipHostGetDevicePointer((void**)&hrad_big_dev_, hrad_big_, 0));
+          }
+          law_radii(cur, hrad_big_);
+          hrad = hrad_big_dev_;
+        }
+      }
+      nb_new = (uint64_t)single_draw_slow(cur.nshots, hrad, maximal, may_outreach, cur.w_shots, rounds, w_next, st);
     } else {
       fast = true;
       init_pub();

@@ -1,6 +1,6 @@
 // A caller of the reference's C++ API, unchanged but for the build line: the
-// calls mirror test/test_plugin_write_xyz.cpp:12-24, user_interface.hxx and
-// scripts/main_max_frac.cpp:39-52.  It writes the spheres it gets as binary
+// calls mirror test/test_plugin_write_xyz.cpp:12-24, user_interface.hxx,
+// scripts/main_max_frac.cpp:39-52 and test/test_random_radius_generator.cpp:5-32.  It writes the spheres it gets as binary
 // records (x.., r, phase) so that the tests can compare them bit for bit.
 #include <cstdio>
 #include <cstdlib>
@@ -8,6 +8,8 @@
 #include <string>
 
 #include <operator_algorithm.hxx>
+#include <operator_paraview.hxx>
+#include <radius_generator.hxx>
 #include <rsa_decoration.hxx>
 #include <rsa_domain.hxx>
 #include <test.hxx>
@@ -32,7 +34,7 @@ static void dump(const std::string& path, const std::vector<rsa_sphere<DIM>>& s)
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s uniform3d|xyz3d|throw2d|reiter3d out [args]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s uniform3d|xyz3d|throw2d|reiter3d|law3d|pv out [args]\n", argv[0]);
     return 1;
   }
   rsa_mpi::init(&argc, &argv);
@@ -74,6 +76,36 @@ int main(int argc, char** argv) {
     algo.reset_radius_generator(g2);
     algo.proceed(seed);
     dump<DIM>(out, domain.extract_spheres());
+  } else if (mode == "law3d") {  // test/test_random_radius_generator.cpp:8-30
+    constexpr int DIM = 3;
+    const double L = std::atof(argv[3]), r_min = std::atof(argv[4]), r_max = std::atof(argv[5]);
+    const uint64_t nb_spheres_max = std::strtoull(argv[6], nullptr, 10);
+    const size_t seed = std::strtoull(argv[7], nullptr, 10);
+    std::array<double, DIM> domain_inf = create_array<DIM>(0.);
+    std::array<double, DIM> domain_sup = create_array<DIM>(L);
+    uint64_t phase = 0;
+    auto nonlinear_transform = [r_min, r_max](double x) { return r_min + (r_max - r_min) * x; };
+    sac_de_billes::RandomRadiusGenerator random_radius_generator(nonlinear_transform, phase);
+    sac_de_billes::RadiusGenerator<DIM> radius_generator(r_min, r_max, &random_radius_generator, nb_spheres_max, 0.);
+    rsa_domain<DIM> domain(domain_inf, domain_sup, 1, radius_generator.get_max_radius());
+    algorithm::uniform_generate<DIM, 1, false>(domain, radius_generator, 6000, 10, seed);
+    dump<DIM>(out, domain.extract_spheres());
+  } else if (mode == "pv") {  // uniform_generate then rsa_paraview::paraview(domain, out) (test.hxx:45-50)
+    const int dim = std::atoi(argv[3]);
+    const double L = std::atof(argv[4]), rad = std::atof(argv[5]);
+    const size_t seed = std::strtoull(argv[6], nullptr, 10);
+    if (dim == 2) {
+      rsa_domain<2> domain({0., 0.}, {L, L}, 1, rad);
+      auto algo = algorithm::uniform_generate<2, 1>(domain, rad, 6000, 10, seed);
+      rsa_paraview::paraview(domain, out);
+      rsa_paraview::paraview(domain, out);  // a second call writes the next step
+      dump<2>(out + "/spheres.bin", domain.extract_spheres());
+    } else {
+      rsa_domain<3> domain({0., 0., 0.}, {L, L, L}, 1, rad);
+      auto algo = algorithm::uniform_generate<3, 1>(domain, rad, 6000, 10, seed);
+      rsa_paraview::paraview(domain, out);
+      dump<3>(out + "/spheres.bin", domain.extract_spheres());
+    }
   } else {
     return 1;
   }

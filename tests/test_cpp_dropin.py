@@ -84,3 +84,48 @@ def test_write_xyz_plugin(tmp_path):
     assert lines[1] == " ".join("%g" % v for v in sup)
     want = ["0 " + " ".join("%g" % v for v in p) for p in o["pos"]]
     assert lines[2:] == want
+
+
+@pytest.mark.gpu
+def test_random_radius_generator_law_3d(tmp_path):
+    """test/test_random_radius_generator.cpp: RadiusGenerator(r_min, r_max, &RandomRadiusGenerator(uniform
+    transform), 1e16) in the unit cube, seed 0; radii r_min + (r_max - r_min) u from 2 words per shot."""
+    out, _ = run_bin(tmp_path, "law3d", 1.0, 0.05, 0.055, 10 ** 16, 0)
+    pos, rad, ph = read_records(out, 3)
+    o = O.run(3, [0.0] * 3, [1.0] * 3, [(0.05, 1e16, 0), (0.055, 0.0, 0)], seed=0, mode=2)
+    same_as_oracle(pos, rad, ph, o)
+    assert len(np.unique(rad)) > len(rad) // 2  # the radii are drawn, not a class value
+
+
+def parse_vtp(path):
+    """The point data arrays and points of an ASCII VTK PolyData file."""
+    import xml.etree.ElementTree as ET
+    piece = ET.parse(path).getroot().find("PolyData").find("Piece")
+    arrays = {a.get("Name"): a.text.split() for a in piece.find("PointData").findall("DataArray")}
+    pts = piece.find("Points").find("DataArray").text.split()
+    return int(piece.get("NumberOfPoints")), arrays, pts
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim", [2, 3])
+def test_paraview_output(tmp_path, dim):
+    """rsa_paraview::paraview (operator_paraview.ixx:98-142): <name>/rsa_mpi_<step>.pvtp naming the piece
+    rsa_mpi_<step>/rsa_mpi_<step>_0.vtp, whose points, Id (phase), Tag (Validated = 0) and Radius are the
+    extracted spheres in default stream format; each call writes the next step."""
+    out = str(tmp_path / "pv")
+    r = subprocess.run([BIN, "pv", out, str(dim), "0.2", "0.01", "1"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    pos, rad, ph = read_records(os.path.join(out, "spheres.bin"), dim)
+    pvtp = open(os.path.join(out, "rsa_mpi_0.pvtp")).read()
+    assert '<Piece Source="rsa_mpi_0/rsa_mpi_0_0.vtp"/>' in pvtp
+    n, arrays, pts = parse_vtp(os.path.join(out, "rsa_mpi_0", "rsa_mpi_0_0.vtp"))
+    assert n == len(rad)
+    want_pts = []
+    for p in pos:
+        want_pts += ["%g" % v for v in p] + (["0"] if dim == 2 else [])
+    assert pts == want_pts
+    assert arrays["Radius"] == ["%g" % v for v in rad]
+    assert arrays["Id"] == [str(v) for v in ph]
+    assert arrays["Tag"] == ["0"] * n
+    if dim == 2:  # the second call in the same process
+        assert os.path.exists(os.path.join(out, "rsa_mpi_1", "rsa_mpi_1_0.vtp"))

@@ -185,3 +185,20 @@ def test_cfg5a_r0002_matches_golden():
     _check_golden(g, gold)
     assert uncovered_samples(g["pos"], g["rad"], [1.0] * 3, 0.002, n=100000) == 0
     assert 0.38 < volume_fraction(g["rad"], 3, 1.0) < 0.39
+
+
+@pytest.mark.gpu
+def test_custom_radius_law_2d_python():
+    """LawRadiusGenerator (RadiusGenerator(min, max, RandomRadiusGenerator*, n)) through the Python facade:
+    a uniform law on [0.008, 0.012] in the unit square, 1000 spheres wanted (the run ends on the class count)."""
+    import rsa_mpi_amd as R
+    dom = R.Domain([0.0, 0.0], [1.0, 1.0], 0.012)
+    gen = R.LawRadiusGenerator(0.008, 0.012, lambda u: 0.008 + (0.012 - 0.008) * u, 1000, phase=3)
+    algo = R.Algo(dom, gen, 6000)
+    st = algo.proceed(5)
+    pos, rad, ph = dom.extract_arrays()
+    o = O.run(2, [0.0, 0.0], [1.0, 1.0], [(0.008, 1000.0, 3), (0.012, 0.0, 3)], seed=5, mode=2)
+    assert len(rad) == o["n"] == 1000
+    assert set_hash(pos, rad) == set_hash(o["pos"], o["rad"])
+    assert (ph == 3).all()
+    assert st["end_reason"] == 2
