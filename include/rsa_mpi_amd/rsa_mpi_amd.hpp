@@ -464,6 +464,59 @@ void init(Args...) {
 inline void finalize() { message("=========== end ==========="); }
 }  // namespace rsa_mpi
 
+// ------------------------------------------------------------ lossless binary dump
+//! The spheres of a storage at full FP64 precision (the XYZ text above keeps 6
+//! digits).  No counterpart in the reference.  Little-endian layout: the 8
+//! bytes "RSAMPIB1", uint32 DIM, uint32 0, uint64 count, then per sphere DIM
+//! float64 centre coordinates, float64 radius, uint64 phase.
+template <int DIM>
+void write_binary(std::string a_name, const rsa_data_storage<DIM>& spheres) {
+  std::ofstream out(a_name, std::ios::binary);
+  if (!out) throw std::runtime_error("write_binary: cannot open " + a_name);
+  const char magic[8] = {'R', 'S', 'A', 'M', 'P', 'I', 'B', '1'};
+  const uint32_t head[2] = {(uint32_t)DIM, 0u};
+  const uint64_t n = spheres.size();
+  out.write(magic, 8);
+  out.write(reinterpret_cast<const char*>(head), sizeof head);
+  out.write(reinterpret_cast<const char*>(&n), sizeof n);
+  for (size_t i = 0; i < spheres.size(); i++) {
+    double rec[DIM + 1];
+    for (int d = 0; d < DIM; d++) rec[d] = spheres.get_center((int)i)[d];
+    rec[DIM] = spheres.get_rad((int)i);
+    const uint64_t ph = spheres.get_phase((int)i);
+    out.write(reinterpret_cast<const char*>(rec), sizeof rec);
+    out.write(reinterpret_cast<const char*>(&ph), sizeof ph);
+  }
+  if (!out) throw std::runtime_error("write_binary: write failed for " + a_name);
+}
+
+//! The storage write_binary wrote (priorities are not kept: 0).
+template <int DIM>
+rsa_data_storage<DIM> read_binary(std::string a_name) {
+  std::ifstream in(a_name, std::ios::binary);
+  if (!in) throw std::runtime_error("read_binary: cannot open " + a_name);
+  char magic[8];
+  uint32_t head[2];
+  uint64_t n = 0;
+  in.read(magic, 8);
+  in.read(reinterpret_cast<char*>(head), sizeof head);
+  in.read(reinterpret_cast<char*>(&n), sizeof n);
+  if (!in || std::string(magic, 8) != "RSAMPIB1" || head[0] != (uint32_t)DIM)
+    throw std::runtime_error("read_binary: not a " + std::to_string(DIM) + "D sphere dump: " + a_name);
+  rsa_data_storage<DIM> s;
+  for (uint64_t i = 0; i < n; i++) {
+    double rec[DIM + 1];
+    uint64_t ph = 0;
+    in.read(reinterpret_cast<char*>(rec), sizeof rec);
+    in.read(reinterpret_cast<char*>(&ph), sizeof ph);
+    if (!in) throw std::runtime_error("read_binary: truncated file " + a_name);
+    vec_d<DIM> c;
+    for (int d = 0; d < DIM; d++) c[d] = rec[d];
+    s.add_sphere(0, ph, rec[DIM], c);
+  }
+  return s;
+}
+
 // ------------------------------------------------------------ operator_paraview.hxx:5-37
 // VTK XML PolyData of the real spheres (operator_paraview.ixx:12-142): per call
 // `step`, a_name/rsa_mpi_<step>.pvtp naming one piece per rank (one here) and

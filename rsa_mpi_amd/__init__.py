@@ -16,7 +16,7 @@ from . import _lib
 from ._lib import Stats, check, dptr, f64, lib, mt_words  # noqa: F401
 
 __all__ = ["Domain", "RadiusGenerator", "LawRadiusGenerator", "Algo", "Stats", "mt_words", "uniform_generate",
-           "throw_spheres", "batch_run", "set_nb_threads"]
+           "throw_spheres", "batch_run", "write_binary", "read_binary", "set_nb_threads"]
 
 
 def set_nb_threads(n):
@@ -256,6 +256,36 @@ def throw_spheres(L, desired_radius_volumeFraction_phase, seed, write_paraview=F
     domain = Domain([0.0] * dim, [float(v) for v in L], max(gen.radius))
     uniform_generate(domain, gen, 6000, 10, seed)
     return domain.extract_spheres()
+
+
+_BIN_MAGIC = b"RSAMPIB1"
+
+
+def write_binary(path, pos, rad, phase):
+    """Lossless dump of spheres (the layout of write_binary in the C++ drop-in): "RSAMPIB1",
+    uint32 dim, uint32 0, uint64 n, then per sphere dim float64 coordinates, float64 radius,
+    uint64 phase, little-endian."""
+    pos = np.asarray(pos, dtype=np.float64)
+    n, dim = pos.shape
+    rec = np.zeros(n, dtype=[("c", "<f8", (dim,)), ("r", "<f8"), ("p", "<u8")])
+    rec["c"] = pos
+    rec["r"] = np.asarray(rad, dtype=np.float64)
+    rec["p"] = np.asarray(phase, dtype=np.uint64)
+    with open(path, "wb") as f:
+        f.write(_BIN_MAGIC + np.array([dim, 0], dtype="<u4").tobytes() + np.array([n], dtype="<u8").tobytes())
+        f.write(rec.tobytes())
+
+
+def read_binary(path):
+    """(pos, rad, phase) of a file write_binary (C++ or Python) wrote."""
+    with open(path, "rb") as f:
+        head = f.read(24)
+        if len(head) != 24 or head[:8] != _BIN_MAGIC:
+            raise ValueError(f"{path} is not a sphere dump")
+        dim = int(np.frombuffer(head[8:12], dtype="<u4")[0])
+        n = int(np.frombuffer(head[16:24], dtype="<u8")[0])
+        rec = np.frombuffer(f.read(), dtype=[("c", "<f8", (dim,)), ("r", "<f8"), ("p", "<u8")], count=n)
+    return rec["c"].copy(), rec["r"].copy(), rec["p"].copy()
 
 
 def batch_run(dim, sides, radii, seeds, nbshots=6000, workers=4, extract=False):

@@ -127,3 +127,40 @@ def test_farthest_corner_needs_the_product_set():
     R2 = 1.0
     assert corner_loop(delta, length, tab, R2) is True
     assert farthest_corner(delta, length, tab, R2) is False
+
+
+# ---------------------------------------------------------------- lossless binary dump
+
+def test_binary_dump_round_trip_cpp_python(tmp_path):
+    """write_binary (C++ drop-in) is read bit for bit by rsa_mpi_amd.read_binary, and the Python
+    writer's file is read back by the C++ read_binary."""
+    import subprocess
+    import rsa_mpi_amd as R
+    exe = str(tmp_path / "dump")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include", "rsa_mpi_amd"),
+                        os.path.join(ROOT, "tests", "cpp", "binary_dump.cpp"), "-L",
+                        os.path.join(ROOT, "rsa_mpi_amd", "lib"), "-lrsa_mpi_amd",
+                        "-Wl,-rpath," + os.path.join(ROOT, "rsa_mpi_amd", "lib"), "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    f3 = str(tmp_path / "a.bin")
+    assert subprocess.run([exe, "w", f3]).returncode == 0
+    pos, rad, ph = R.read_binary(f3)
+    assert pos.tolist() == [[0.1, 0.2, 0.30000000000000004], [1e-300, 0.9999999999999999, 0.5],
+                            [0.125, 0.25, 0.375]]
+    assert rad.tolist() == [0.1, 5e-324, 0.0025]
+    assert ph.tolist() == [0, 3, 2 ** 64 - 1]
+    f2 = str(tmp_path / "b.bin")
+    rng = np.random.default_rng(0)
+    p2 = rng.random((5, 2))
+    r2 = rng.random(5) * 0.01
+    ph2 = np.arange(5, dtype=np.uint64) * 11
+    R.write_binary(f2, p2, r2, ph2)
+    out = subprocess.run([exe, "r", f2], capture_output=True, text=True).stdout.split("\n")
+    assert int(out[0]) == 5
+    for k in range(5):
+        x, y, rr, p = out[1 + k].split()
+        assert (float(x), float(y), float(rr), int(p)) == (p2[k, 0], p2[k, 1], r2[k], int(ph2[k]))
+    with pytest.raises(ValueError):
+        open(str(tmp_path / "c.bin"), "wb").write(b"not a dump at all, 24+ bytes")
+        R.read_binary(str(tmp_path / "c.bin"))
