@@ -1555,17 +1555,33 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
   // linear offsets of the stencil cells (any order: coverage is an OR)
   constexpr int NSTL = NST <= 32 ? NST : 1;
   __shared__ int s_off[NSTL];
+  __shared__ uint8_t s_last[NSTL];  // 1 where the next stencil cell is farther from the centre cell
   constexpr int KLIST = 16;  // listed spheres per lane before a flush
   __shared__ uint32_t s_list[KLIST][WG];
   if (NST <= 32 && threadIdx.x < NSTL) {
-    int o = 0, r = threadIdx.x, stride = 1;
+    // centre cell first, then by the number of non-zero offsets (faces, edges,
+    // corners): a covering sphere is met early and a covered voxel stops there
+    auto nz = [](int r) {
+      int n = 0;
+      for (int d = 0; d < D; d++, r /= 3) n += (r % 3 != 1);
+      return n;
+    };
+    const int t = threadIdx.x, nt = nz(t);
+    int pos = 0, same_after = 0;
+    for (int u = 0; u < NST; u++) {
+      const int nu = nz(u);
+      pos += (nu < nt) || (nu == nt && u < t);
+      same_after += (nu == nt && u > t);
+    }
+    int o = 0, r = t, stride = 1;
 #pragma unroll
     for (int d = 0; d < D; d++) {
       o += (r % 3 - 1) * stride;
       r /= 3;
       stride *= g.n[d];
     }
-    s_off[threadIdx.x] = o;
+    s_off[pos] = o;
+    s_last[pos] = same_after == 0;
   }
   __syncthreads();
   uint32_t code = 0;
@@ -1683,6 +1699,7 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
               s_list[nl++][threadIdx.x] = base + k;
             }
           }
+          if (s_last[st]) flush();  // end of a distance shell: test what was listed
         }
         flush();
         continue;
