@@ -38,6 +38,17 @@ def run_gpu_step(R, profile_level=1):
     return dom, st, dt
 
 
+def profile_figure(key):
+    """A `key: value` line of profiles/k_cover_3d_r0.005.md (rocprofv3 PMC summary), or None."""
+    path = os.path.join(ROOT, "profiles", "k_cover_3d_r0.005.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def measured_traffic():
     """HBM bytes per k_cover launch on this workload, from the rocprofv3 FETCH_SIZE and
     WRITE_SIZE passes summarised in profiles/ (each counter scaled by what it reads for
@@ -164,6 +175,14 @@ def main():
                         kernel="k_cover", launches=k["launches"] // args.steps,
                         avg_launch_ms=round(1e3 * k["seconds"] / k["launches"], 3),
                         algorithmic_bytes_per_launch=round(k["bytes"] / k["launches"]))
+        # k_cover issues FP64 VALU work: its VALU instruction count (SQ_INSTS_VALU, from the
+        # committed PMC summary) at 4 cycles per wave64 instruction on 1024 SIMDs at 2.4 GHz,
+        # over the measured kernel time
+        valu = profile_figure("valu_wave_instructions_per_run")
+        if valu:
+            busy_s = valu * 4 / (1024 * 2.4e9)
+            roofline["valu"] = dict(wave_instructions_per_run=valu, issue_seconds_per_run=round(busy_s, 6),
+                                    busy_frac=round(busy_s / (k["seconds"] / args.steps), 3))
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "spheres/s", "n_gpus": 1, "steps": args.steps,
