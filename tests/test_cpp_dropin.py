@@ -129,3 +129,14 @@ def test_paraview_output(tmp_path, dim):
     assert arrays["Tag"] == ["0"] * n
     if dim == 2:  # the second call in the same process
         assert os.path.exists(os.path.join(out, "rsa_mpi_1", "rsa_mpi_1_0.vtp"))
+
+
+@pytest.mark.gpu
+def test_write_xyz_plugin_reference_args(tmp_path):
+    """test/CMakeLists.txt:66-67 runs test_plugin_write_xyz with 0.5 10 10 10."""
+    out, _ = run_bin(tmp_path, "xyz3d", 0.5, 10, 10, 10)
+    o = O.run(3, [0.0] * 3, [10.0] * 3, [(0.5, 1.0, 0)], seed=0)
+    lines = open(out).read().splitlines()
+    assert lines[0] == str(o["n"])
+    assert lines[1] == "10 10 10"
+    assert lines[2:] == ["0 " + " ".join("%g" % v for v in p) for p in o["pos"]]

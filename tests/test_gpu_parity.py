@@ -223,3 +223,46 @@ def test_cfg5b_4d_r002_matches_golden():
     gold = _golden("cfg5b_4d_r0.02_seed0")
     g = gpu_run(4, 1.0, [(0.02, 1.0, 0)], seed=0)
     _check_golden(g, gold)
+
+
+# ------------------------------------------------------------ the reference's own test programs
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim", [2, 3, 4])
+def test_reference_full_voxel(dim):
+    """test/test_{2D,3D,4D}_full_voxel.cpp: test_ND<DIM, 1>(0.05) (test.hxx:53-67), unit box, seed 0."""
+    cls = [(0.05, 1.0, 0)]
+    g = gpu_run(dim, 1.0, cls, seed=0)
+    o = orc_run(dim, 1.0, cls, seed=0)
+    assert_same(g, o)
+    assert g["stats"]["words"] == o["words"]
+
+
+@pytest.mark.gpu
+def test_reference_3d_radius():
+    """test/test_3D_radius.cpp: test_ND_radius<3, 1>({(0.25, 0.1, 0), (0.125, 0.1, 1)}) with volume 1
+    (test.hxx:34-51): two cells per dimension for the largest radius."""
+    cls = [(0.25, 0.1, 0), (0.125, 0.1, 1)]
+    g = gpu_run(3, 1.0, cls, seed=0)
+    o = orc_run(3, 1.0, cls, seed=0)
+    assert_same(g, o)
+
+
+@pytest.mark.gpu
+def test_reference_user_interface_cpp():
+    """test/test_python.cpp: user_interface domain [0, 3.5]^3, r=0.4 at volume fraction 1, seed 0."""
+    cls = [(0.4, 1.0, 0)]
+    g = gpu_run(3, 3.5, cls, seed=0)
+    o = orc_run(3, 3.5, cls, seed=0)
+    assert_same(g, o)
+
+
+@pytest.mark.gpu
+def test_reference_bug_2d_matches_golden():
+    """test.hxx:104-109 test_bug_2D: throw_spheres<2, 1>({1, 1}, {(0.0005, 1, 0)}, 0), written in the
+    reference to provoke subdivisions past 32-bit voxel coordinates (696 552 disks, 23 refinements)."""
+    from _common import uncovered_samples
+    gold = _golden("bug2d_2d_r0.0005_seed0")
+    g = gpu_run(2, 1.0, [(0.0005, 1.0, 0)], seed=0)
+    _check_golden(g, gold)
+    assert uncovered_samples(g["pos"], g["rad"], [1.0] * 2, 0.0005, n=100000) == 0
