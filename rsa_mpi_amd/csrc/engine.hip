@@ -1933,10 +1933,12 @@ __device__ __forceinline__ uint32_t kept_code(const uint32_t* code, const uint32
   return c != 0 ? c : moved[i];
 }
 
-__global__ void __launch_bounds__(WG) k_child_count(int64_t n, const uint32_t* __restrict__ code,
+__global__ void __launch_bounds__(WG) k_child_count(int64_t n, const int* __restrict__ ctr,
+                                                    const uint32_t* __restrict__ code,
                                                     const uint32_t* __restrict__ moved, int* __restrict__ blk) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = i < n ? __popc(kept_code(code, moved, i) & 0xffffu) : 0;
+  const int64_t keep = ctr[C_KEEP];
+  const int c = i < n && i < keep ? __popc(kept_code(code, moved, i) & 0xffffu) : 0;
   __shared__ int sh[WG];
   sh[threadIdx.x] = c;
   __syncthreads();
@@ -2944,6 +2946,10 @@ class Engine final : public IEngine {
     hipLaunchKernelGGL(k_holes, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
     hipLaunchKernelGGL(k_move<D>, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, moved_.p, blk_.p, dctr_.p, holes_.p,
                        vx);
+    // children per block of the kept prefix, counted against the device's keep
+    // (blocks past it count 0), so that one host round trip returns both totals
+    hipLaunchKernelGGL(k_child_count, dim3(nb), dim3(WG), 0, st_, nvox_, dctr_.p, code_.p, moved_.p, blk_.p);
+    scan_excl(blk_.p, nb, dctr_.p + C_NCHILD);
     read_ctr();
     const int64_t keep = hctr_[C_KEEP];
     if (keep == 0) {  // subdivide_uncovered returns early on an empty list (list_of_voxels.ixx:60)
@@ -2951,9 +2957,6 @@ class Engine final : public IEngine {
       return;
     }
     const unsigned nb2 = nblocks(keep);
-    hipLaunchKernelGGL(k_child_count, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, moved_.p, blk_.p);
-    scan_excl(blk_.p, nb2, dctr_.p + C_NCHILD);
-    read_ctr();
     const int64_t nchild = hctr_[C_NCHILD];
     if (nchild < 0) throw Error("voxel list would exceed 2^31 entries");
     // children coordinates 2c + t: 64-bit lists from the refinement on where they pass INT32_MAX
