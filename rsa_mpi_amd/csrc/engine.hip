@@ -1683,8 +1683,7 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
             nvn = cs.cnt[src];
             q0n = cs.sph[(uint32_t)(src * cs.cap)];
           }
-          for (int k = 0; k < nv; k++) {
-            const Slot<D> q = k == 0 ? q0 : cs.sph[base + k];
+          auto consider = [&](const Slot<D>& q, uint32_t idx) {
             const double rr = q.r + vp.rmin;
             const double R2 = rr * rr;
             double s2 = 0.;
@@ -1696,9 +1695,12 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
             const bool use = (do_child && s2 < R2) || (do_cov && (vp.h2 < 0. || s2 <= R2 - vp.h2 + 1e-9 * R2));
             if (use) {
               if (nl == KLIST) flush();
-              s_list[nl++][threadIdx.x] = base + k;
+              s_list[nl++][threadIdx.x] = idx;
             }
-          }
+          };
+          // the first slot came with the count; the others are plain loads
+          if (nv > 0) consider(q0, base);
+          for (int k = 1; k < nv; k++) consider(cs.sph[base + k], base + k);
           if (s_last[st]) flush();  // end of a distance shell: test what was listed
         }
         flush();
