@@ -129,8 +129,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 5}))
         return
     if args.gpus != 1:
-        sys.exit("bench.py measures one GPU only: this build runs one rank's whole domain on a single MI355X "
-                 "(multi-GPU sharding is a design in DESIGN.md, not a measured path).")
+        sys.exit("bench.py measures one GPU only: the library runs one rank's whole domain on a single MI355X, "
+                 "and sharding a realization over several GPUs is designed but not implemented.")
 
     import rsa_mpi_amd as R
     R.lib()

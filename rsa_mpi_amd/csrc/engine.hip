@@ -1656,11 +1656,13 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
 #pragma unroll
       for (int d = 0; d < D; d++) interior = interior && cc[d] > g.gl && cc[d] < g.n[d] - g.gl - 1;
       if (interior && NST <= 32) {
-        // Two phases against divergence: first every sphere of the stencil gets
-        // the cheap distance test and the ones either test can use are listed
-        // (per lane, in LDS); then the full tests run over the short lists.  A
-        // sphere is listed when it passes the subdivide pre-filter or the
-        // coverage pre-check, exactly the gates of the full test.
+        // Two phases against divergence: every sphere of the stencil gets the
+        // cheap distance test and the ones either test can use are listed (per
+        // lane, in LDS); the full tests run over the short lists at the end of
+        // each distance shell (and when a list is full), so that a voxel covered
+        // by a sphere of the nearer cells stops there.  A sphere is listed when
+        // it passes the subdivide pre-filter or the coverage pre-check, exactly
+        // the gates of the full test.
         int nl = 0;
         auto flush = [&]() {
           for (int t = 0; t < nl && !covered; t++) {
