@@ -244,9 +244,11 @@ class rsa_grid {
     for (const auto& s : extract_spheres()) out.add_sphere(0, s);
     return out;
   }
+  //! spheres stored in the real cells (rsa_grid.ixx:73-98 counts them per cell)
   template <TypeCell typeCell = TypeCell::Real>
   uint64_t get_number_of_spheres() const;
-  uint64_t get_number_of_spheres_fast() const { return get_number_of_spheres(); }
+  //! rsa_grid.hxx:64 / rsa_grid.ixx:102-121: the counter add_to_sample keeps (rsa_domain::add leaves it)
+  uint64_t get_number_of_spheres_fast() const;
   double local_volume_of_spheres() const;
 
  private:
@@ -339,6 +341,13 @@ template <int DIM>
 template <TypeCell typeCell>
 uint64_t rsa_grid<DIM>::get_number_of_spheres() const {
   static_assert(typeCell == TypeCell::Real, "only real cells are kept on the device");
+  uint64_t n = 0;
+  rsa_mpi_amd_detail::check(rsa_gpu_extract(dom_->ctx(), nullptr, nullptr, nullptr, 0, &n));
+  return n;
+}
+
+template <int DIM>
+uint64_t rsa_grid<DIM>::get_number_of_spheres_fast() const {
   uint64_t n = 0;
   rsa_mpi_amd_detail::check(rsa_gpu_count(dom_->ctx(), &n));
   return n;

@@ -226,7 +226,14 @@ enum Ctr : int {
 
 // ============================================================== kernels: word stream
 
-__device__ __forceinline__ uint32_t ring_at(const uint32_t* ring, uint64_t mask, uint64_t pos) { return ring[pos & mask]; }
+// Kernels never read device memory that a copy engine wrote through a plain
+// load: an XCD's L2 may still hold an older line of it (the ring's slots are
+// rewritten every kRing words, and pool memory changes hands).  Ring words are
+// read with system-scope loads, which see what the copy wrote; the other
+// host-provided tables are written by kernels (k_set_table, k_insert).
+__device__ __forceinline__ uint32_t ring_at(const uint32_t* ring, uint64_t mask, uint64_t pos) {
+  return __hip_atomic_load(ring + (pos & mask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Voxel-index words drawn by uniform_int_distribution<int>(0, N-1) use Lemire's
 // multiply-shift with rejection of low 32 bits below (2^32 - N) mod N
@@ -2040,17 +2047,32 @@ __global__ void k_extract(Geo<D> g, CellStore<D> cs, const int* __restrict__ blk
   }
 }
 
+// The stencil tables (packed offsets, then linear offsets) from the launch arguments.
+struct StencilTable {
+  int n;  // entries in v: 2 * 3^D
+  int v[2 * 81];
+};
+__global__ void k_set_table(int* __restrict__ dst, StencilTable t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < t.n) dst[i] = t.v[i];
+}
+
+// rsa_domain::add's spheres into their cells, read from pinned host memory:
+// sphere i goes to slot slot[i] of cell cell[i]; the sphere that fills a cell's
+// last new slot (last[i]) also writes the cell's count.
 template <int D>
 __global__ void k_insert(CellStore<D> cs, int64_t n, const int* __restrict__ cell, const int* __restrict__ slot,
-                         const double* __restrict__ pos, const double* __restrict__ rad,
-                         const uint32_t* __restrict__ ph) {
+                         const uint8_t* __restrict__ last, const double* __restrict__ pos,
+                         const double* __restrict__ rad, const uint32_t* __restrict__ ph) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Slot<D> s;
   for (int d = 0; d < D; d++) s.c[d] = pos[i * D + d];
   s.r = rad[i];
-  cs.sph[(size_t)cell[i] * cs.cap + slot[i]] = s;
-  cs.phase[(size_t)cell[i] * cs.cap + slot[i]] = ph[i];
+  const int c = cell[i], k = slot[i];
+  cs.sph[(size_t)c * cs.cap + k] = s;
+  cs.phase[(size_t)c * cs.cap + k] = ph[i];
+  if (last[i]) cs.cnt[c] = (uint8_t)(k + 1);
 }
 
 // ============================================================== host engine
@@ -2235,24 +2257,31 @@ class Engine final : public IEngine {
       cell[i] = (int)c;
       slot[i] = cnt[c]++;
     }
-    DevBuf<int> dcell, dslot;
-    DevBuf<double> dpos, drd;
-    DevBuf<uint32_t> dph;
-    dcell.reserve(n);
-    dslot.reserve(n);
-    dpos.reserve(n * D);
-    drd.reserve(n);
-    dph.reserve(n);
-    std::vector<uint32_t> ph(n);
-    for (int64_t i = 0; i < n; i++) ph[i] = (uint32_t)phase[i];
-    HIP_OK(hipMemcpyAsync(dcell.p, cell.data(), n * sizeof(int), hipMemcpyHostToDevice, st_));
-    HIP_OK(hipMemcpyAsync(dslot.p, slot.data(), n * sizeof(int), hipMemcpyHostToDevice, st_));
-    HIP_OK(hipMemcpyAsync(dpos.p, pos, n * D * sizeof(double), hipMemcpyHostToDevice, st_));
-    HIP_OK(hipMemcpyAsync(drd.p, rad, n * sizeof(double), hipMemcpyHostToDevice, st_));
-    HIP_OK(hipMemcpyAsync(dph.p, ph.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st_));
-    HIP_OK(hipMemcpyAsync(cells_cnt_.p, cnt.data(), geo_.ncells, hipMemcpyHostToDevice, st_));
-    hipLaunchKernelGGL(k_insert<D>, dim3(nblocks(n)), dim3(WG), 0, st_, cs(), n, dcell.p, dslot.p, dpos.p, drd.p, dph.p);
-    HIP_OK(hipStreamSynchronize(st_));
+    // the inputs in pinned, coherent host memory, read there by k_insert
+    const size_t off_rad = (size_t)n * D * sizeof(double), off_cell = off_rad + (size_t)n * sizeof(double);
+    const size_t off_slot = off_cell + (size_t)n * sizeof(int), off_ph = off_slot + (size_t)n * sizeof(int);
+    const size_t off_last = off_ph + (size_t)n * sizeof(uint32_t), bytes = off_last + (size_t)n;
+    char* h = static_cast<char*>(PinnedPool::get().alloc(bytes, hipHostMallocCoherent | hipHostMallocMapped));
+    char* hd = nullptr;
+    hipError_t e = hipHostGetDevicePointer((void**)&hd, h, 0);
+    if (e != hipSuccess) {
+      PinnedPool::get().release(h);
+      HIP_OK(e);
+    }
+    std::memcpy(h, pos, (size_t)n * D * sizeof(double));
+    std::memcpy(h + off_rad, rad, (size_t)n * sizeof(double));
+    std::memcpy(h + off_cell, cell.data(), (size_t)n * sizeof(int));
+    std::memcpy(h + off_slot, slot.data(), (size_t)n * sizeof(int));
+    for (int64_t i = 0; i < n; i++) {
+      reinterpret_cast<uint32_t*>(h + off_ph)[i] = (uint32_t)phase[i];
+      reinterpret_cast<uint8_t*>(h + off_last)[i] = slot[i] + 1 == cnt[cell[i]];
+    }
+    hipLaunchKernelGGL(k_insert<D>, dim3(nblocks(n)), dim3(WG), 0, st_, cs(), n, (const int*)(hd + off_cell),
+                       (const int*)(hd + off_slot), (const uint8_t*)(hd + off_last), (const double*)hd,
+                       (const double*)(hd + off_rad), (const uint32_t*)(hd + off_ph));
+    e = hipStreamSynchronize(st_);
+    PinnedPool::get().release(h);
+    HIP_OK(e);
   }
 
   uint64_t count() const override { return nb_fast_; }
@@ -2615,9 +2644,14 @@ class Engine final : public IEngine {
       lin[i] = (int)off;
     }
     stab_.reserve(2 * n3);
-    HIP_OK(hipMemcpyAsync(stab_.p, packed.data(), n3 * sizeof(int), hipMemcpyHostToDevice, st_));
-    HIP_OK(hipMemcpyAsync(stab_.p + n3, lin.data(), n3 * sizeof(int), hipMemcpyHostToDevice, st_));
-    HIP_OK(hipStreamSynchronize(st_));
+    StencilTable tab{};
+    tab.n = 2 * n3;
+    for (int i = 0; i < n3; i++) {
+      tab.v[i] = packed[i];
+      tab.v[n3 + i] = lin[i];
+    }
+    hipLaunchKernelGGL(k_set_table, dim3(1), dim3(256), 0, st_, stab_.p, tab);
+    HIP_OK(hipGetLastError());
     geo_.stab = stab_.p;
     geo_.slin = stab_.p + n3;
   }

@@ -266,3 +266,23 @@ def test_reference_bug_2d_matches_golden():
     g = gpu_run(2, 1.0, [(0.0005, 1.0, 0)], seed=0)
     _check_golden(g, gold)
     assert uncovered_samples(g["pos"], g["rad"], [1.0] * 2, 0.0005, n=100000) == 0
+
+
+@pytest.mark.gpu
+def test_domain_add_round_trip():
+    """rsa_domain::add (rsa_domain.hxx:84, build_grid with Validated tags): a packing's spheres added to a fresh
+    domain in extraction order come back bit for bit in the same order; proceed on the full packing then adds
+    nothing, and the add_to_sample counter (get_number_of_spheres_fast) is left as add leaves it."""
+    g = gpu_run(3, 0.2, [(0.01, 1.0, 0)], seed=4)
+    dom = R.Domain([0.0] * 3, [0.2] * 3, 0.01)
+    gen = R.RadiusGenerator([(0.01, 1.0, 0)], dom.get_total_volume(), 0.0)
+    algo = R.Algo(dom, gen, 6000)
+    dom.add(g["pos"], g["rad"], g["phase"])
+    pos, rad, ph = dom.extract_arrays()
+    assert np.array_equal(pos.view(np.uint64), g["pos"].view(np.uint64))
+    assert np.array_equal(rad.view(np.uint64), g["rad"].view(np.uint64))
+    assert np.array_equal(ph, g["phase"])
+    assert dom.number_of_spheres() == 0
+    st = algo.proceed(4)
+    assert st["accepted"] == 0 and st["nvox_final"] == 0
+    assert len(dom.extract_arrays()[1]) == len(g["rad"])
