@@ -111,7 +111,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-side", type=float, default=0.8)
-    ap.add_argument("--batch-workers", type=int, default=4,
+    ap.add_argument("--batch-workers", type=int, default=6,
                     help="realizations run at once in the batched measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
