@@ -774,7 +774,10 @@ template <int D>
 struct LaneGroup {
   static constexpr int NST = ipow_c(3, D);
   static constexpr int LPC = D == 2 ? 16 : (D == 3 ? 32 : 64);  // lanes per candidate
-  static constexpr int CPL = (NST + LPC - 1) / LPC;              // stencil cells per lane
+  // Stencil cells per lane.  Lane `sub` takes cells sub * CPL .. sub * CPL + CPL - 1,
+  // so lane order is stencil order: a partner list assembled lane by lane is in
+  // the reference's detection order (4D: 81 cells on 64 lanes, two per lane).
+  static constexpr int CPL = (NST + LPC - 1) / LPC;
   __device__ static inline uint64_t mask(int lane) {
     if constexpr (LPC == 64) {
       return ~0ull;
@@ -816,9 +819,19 @@ struct DrawCtl {
   // closed, every kernel of that draw returns at once.
   int gate;
   int swept;     // seq of the last sweep that ran (k_update_commit of that round runs)
-  int nunres;    // candidates k_partner_lists left unresolved (listed in unres)
+  int nunres;    // candidates k_partner_lists left unresolved (one URec each)
   int acc_draw;  // spheres accepted so far in the current draw
 };
+// One unresolved candidate as k_sweep_labels reads it, written by
+// k_partner_lists at the candidate's place in the unresolved list: index,
+// partner count, list offset (-1: the candidate's slab of KMAX entries, -2: the
+// pool overflowed) and its first PKEEP partners (the list itself holds the rest).
+struct alignas(32) URec {
+  int q, m, o, pad;
+  int e[PKEEP];
+};
+static_assert(sizeof(URec) == 32, "two 16-byte loads per record");
+
 // What k_sweep_labels publishes to pinned host memory (seq last).
 struct DrawPub {
   int acc, con, label_stuck, dropped, img_mismatch, pool_overflow, need_pool, cell_overflow, gate_open;
@@ -896,14 +909,16 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
     cell_coords<D>(g, cell, cc);
 #pragma unroll
     for (int jj = 0; jj < LG::CPL; jj++) {
-      const int s = sub + jj * LG::LPC;
+      const int s = sub * LG::CPL + jj;
       if (s >= LG::NST || hit) continue;
       Nb<D> nb;
       if (!neighbour<D>(g, cc, s, nb)) continue;
-      const int nv = cs.cnt[nb.src];
+      // the first slot is loaded with the count (every cell has cap >= 1 slots)
       const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
+      const int nv = cs.cnt[nb.src];
+      const Slot<D> q0 = sl[0];
       for (int k = 0; k < nv && !hit; k++) {
-        const Slot<D> q = sl[k];
+        const Slot<D> q = k == 0 ? q0 : sl[k];
         double qp[D];
         if (nb.ghost) {
           if (!image_of<D>(g, nb, q.c, qp)) continue;
@@ -958,20 +973,27 @@ __global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict_
                                               int ncells) {
   const int gate = ctl->gate;  // tested before the first store (loads overlap)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool on = i < nshots && owner[i];
-  // With a closed gate k_decode_filter did not run, and owner/ccell hold stale or
-  // never-written values: clamp the cell so that the read below stays in bounds.
-  const int c0 = on ? ccell[i] : 0;
+  const bool in = i < nshots;
+  // The cell, its count and its fixed slots are read by every thread, owner or
+  // not, so that the loads do not wait for each other's tests.  With a closed
+  // gate k_decode_filter did not run, and owner/ccell hold stale or never-written
+  // values: the clamped cell keeps these reads in bounds.
+  const bool on = in && owner[i];
+  const int c0 = in ? ccell[i] : 0;
   const int c = (c0 >= 0 && c0 < ncells) ? c0 : 0;
-  const int n = on ? cellcc[c] : 0;
+  const int n = in ? cellcc[c] : 0;
+  const int4* row = reinterpret_cast<const int4*>(cslot + (size_t)c * CSLOT);
+  int4 r0 = make_int4(0, 0, 0, 0), r1 = r0;
+  if (in) {
+    r0 = row[0];
+    r1 = row[1];
+  }
   if (gate != draw_id || !on) return;
   // the owner's list: CSLOT entries at i * CSLOT, longer lists after all of those
   const int base = n <= CSLOT ? i * CSLOT : nshots * CSLOT + atomicAdd(&ctl->seg_cursor, n);
   int* a = ord + base;
   int m = n;
   if (n <= CSLOT) {  // in registers: a sorting network, then the removal with selects
-    const int4* row = reinterpret_cast<const int4*>(cslot + (size_t)c * CSLOT);
-    const int4 r0 = row[0], r1 = row[1];
     int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
 #pragma unroll
     for (int k = 0; k < CSLOT; k++)
@@ -1054,14 +1076,16 @@ __global__ void __launch_bounds__(WG) k_refilter(Geo<D> g, CellStore<D> cs, int 
     cell_coords<D>(g, ccell[i], cc);
 #pragma unroll
     for (int jj = 0; jj < LG::CPL; jj++) {
-      const int s = sub + jj * LG::LPC;
+      const int s = sub * LG::CPL + jj;
       if (s >= LG::NST || hit) continue;
       Nb<D> nb;
       if (!neighbour<D>(g, cc, s, nb)) continue;
-      const int nv = cs.cnt[nb.src];
+      // the first slot is loaded with the count (every cell has cap >= 1 slots)
       const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
+      const int nv = cs.cnt[nb.src];
+      const Slot<D> q0 = sl[0];
       for (int k = 0; k < nv && !hit; k++) {
-        const Slot<D> q = sl[k];
+        const Slot<D> q = k == 0 ? q0 : sl[k];
         double qp[D];
         if (nb.ghost) {
           if (!image_of<D>(g, nb, q.c, qp)) continue;
@@ -1124,20 +1148,39 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
   first_ghost = false;
 #pragma unroll
   for (int jj = 0; jj < LG::CPL; jj++) {
-    const int s = sub + jj * LG::LPC;
+    const int s = sub * LG::CPL + jj;
     if (s >= LG::NST) continue;
     Nb<D> nb;
     if (!neighbour<D>(g, cc, s, nb)) continue;
     const int t = cell2t[nb.src];
     if (t < 0) continue;
-    const int* o = ord + seg[t];
+    // A list of at most CSLOT entries starts at t * CSLOT (k_lists): that row is
+    // read with the list's offset and length (t < nshots keeps it in bounds) and
+    // used whenever the offset says the list starts there.
+    const int sg = seg[t];
     const int ln = len[t];
+    const int4* row = reinterpret_cast<const int4*>(ord + (size_t)t * CSLOT);
+    const int4 r0 = row[0], r1 = row[1];
+    const int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    const bool in_row = sg == t * CSLOT;
+    const int* o = ord + sg;
     for (int k = 0; k < ln; k++) {
-      const int j = o[k];
-      if (ctag[j] != T_UNDECIDED) continue;
-      if (!(cprio[j] < pi)) continue;
+      int j;
+      if (in_row && k < CSLOT) {
+        j = v[0];
+#pragma unroll
+        for (int q = 1; q < CSLOT; q++) j = k == q ? v[q] : j;
+      } else {
+        j = o[k];
+      }
+      // a candidate's tag, priority, position and radius are loaded together
+      const uint8_t tj = ctag[j];
+      const uint32_t pj = cprio[j];
+      const double rj = crad[j];
       double jp[D], qp[D];
       cand_pos<D>(cpos, nshots, j, jp);
+      if (tj != T_UNDECIDED) continue;
+      if (!(pj < pi)) continue;
       if (nb.ghost) {
         if (!image_of<D>(g, nb, jp, qp)) continue;
         if (out == nullptr && mismatch) {  // the image must fall in this ghost cell
@@ -1150,7 +1193,7 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
 #pragma unroll
         for (int d = 0; d < D; d++) qp[d] = jp[d];
       }
-      if (collide<D>(r, p, crad[j], qp)) {
+      if (collide<D>(r, p, rj, qp)) {
         if (n == 0) first_ghost = nb.ghost;
         const int e = j | (nb.ghost ? (int)0x80000000 : 0);
         if (out) out[n] = e;
@@ -1175,10 +1218,9 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
                                                  const int* __restrict__ ccell, const uint8_t* __restrict__ ctag,
                                                  uint8_t* __restrict__ lab, const int* __restrict__ cell2t,
                                                  const int* __restrict__ seg, const int* __restrict__ len,
-                                                 const int* __restrict__ ord, int* __restrict__ poff,
-                                                 int* __restrict__ pcnt, int* __restrict__ slab,
+                                                 const int* __restrict__ ord, int* __restrict__ slab,
                                                  int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int gate_id,
-                                                 int* __restrict__ unres) {
+                                                 URec* __restrict__ urec) {
   using LG = LaneGroup<D>;
   const int gate = gate_id >= 0 ? ctl->gate : gate_id;  // tested before the first store (loads overlap)
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1218,8 +1260,8 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
   const int first_lane = has ? (__ffsll((unsigned long long)has) - 1) : 0;
   const int fgi = __shfl(fg ? 1 : 0, first_lane, 64);
   const bool pend = active && total > 0 && !fgi;
-  int* dst = nullptr;
   int off = -2;
+  int u = 0;
   if (active && sub == 0) {
     if (total == 0) {
       lab[i] = T_UNDECIDED;
@@ -1227,8 +1269,7 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
       lab[i] = T_CONFLICT;
     } else {
       lab[i] = T_UNRESOLVED;
-      unres[atomicAdd(&ctl->nunres, 1)] = i;
-      pcnt[i] = total;
+      u = atomicAdd(&ctl->nunres, 1);
       if (total <= KMAX) {
         off = -1;
       } else {
@@ -1239,20 +1280,32 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
           off = -2;
         }
       }
-      poff[i] = off;
+      *reinterpret_cast<int4*>(urec + u) = make_int4(i, total, off, 0);
     }
   }
-  off = __shfl(off, (lane / LG::LPC) * LG::LPC, 64);
+  const int lead = (lane / LG::LPC) * LG::LPC;
+  off = __shfl(off, lead, 64);
+  u = __shfl(u, lead, 64);
   if (pend && off != -2) {
-    dst = off == -1 ? slab + (size_t)i * KMAX : pool + off;
+    // positions < PKEEP go to the record, the others to the slab or pool
+    int* dst = off == -1 ? slab + (size_t)i * KMAX : pool + off;
+    int* head = urec[u].e;
+    const int b = incl - n;  // this lane's first position in the candidate's list
     if (n > 0 && n <= PKEEP) {
 #pragma unroll
-      for (int t = 0; t < PKEEP; t++)
-        if (t < n) dst[incl - n + t] = keep[t];
+      for (int t = 0; t < PKEEP; t++) {
+        if (t < n) {
+          if (b + t < PKEEP)
+            head[b + t] = keep[t];
+          else
+            dst[b + t] = keep[t];
+        }
+      }
     } else if (n > PKEEP) {  // rare: walk the cells again, writing as it goes
       bool dummy;
-      lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord,
-                       dst + (incl - n), dummy, nullptr);
+      lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, dst + b,
+                       dummy, nullptr);
+      for (int t = b; t < PKEEP; t++) head[t] = dst[t];
     }
   }
 }
@@ -1267,25 +1320,53 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
 // dynamic shared memory).  The round's counts are then published to pinned
 // host memory.
 __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
-                                                const int* __restrict__ poff, const int* __restrict__ pcnt,
                                                 const int* __restrict__ slab, const int* __restrict__ pool,
                                                 DrawCtl* ctl, DrawPub* host, int seq, int gate_id, int draw_id,
-                                                double aimed, uint64_t miss_base, const int* __restrict__ unres) {
-  extern __shared__ uint8_t s_lab[];
+                                                double aimed, uint64_t miss_base, const URec* __restrict__ urec) {
+  extern __shared__ __align__(16) uint8_t s_lab[];
   __shared__ int s_prog, s_left, s_acc, s_con, s_stuck;
   // everything below reads first and stores only once the gate is known open
   const int gate = gate_id >= 0 ? ctl->gate : gate_id;
   const bool ovf = ctl->pool_overflow != 0;
   const int nu = ctl->nunres;
   const int tid = threadIdx.x;
+  // the counters published at the end: no kernel of this round changes them
+  // after k_partner_lists, so they are read now, with the loads above
+  int pub_dropped = 0, pub_mism = 0, pub_povf = 0, pub_need = 0, pub_cell = 0, acc_before = 0;
   if (tid == 0) {
+    pub_dropped = ctl->dropped;
+    pub_mism = ctl->img_mismatch;
+    pub_povf = ctl->pool_overflow;
+    pub_need = ctl->need_pool;
+    pub_cell = ctl->cell_overflow;
+    acc_before = ctl->acc_draw;
     s_acc = 0;
     s_con = 0;
     s_stuck = 0;
   }
-  for (int q = tid; q < nshots; q += blockDim.x) s_lab[q] = ctag[q] == T_UNDECIDED ? lab[q] : T_DROPPED;
+  // labels into LDS, 8 candidates per thread and load (the buffers start 256-byte aligned)
+  for (int q8 = tid * 8; q8 < nshots; q8 += 8 * (int)blockDim.x) {
+    if (q8 + 8 <= nshots) {
+      const uint2 t = *reinterpret_cast<const uint2*>(ctag + q8);
+      const uint2 l = *reinterpret_cast<const uint2*>(lab + q8);
+      auto merge = [](uint32_t tw, uint32_t lw) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int b = 0; b < 32; b += 8) {
+          const uint32_t tb = (tw >> b) & 0xffu;
+          o |= (tb == T_UNDECIDED ? (lw >> b) & 0xffu : (uint32_t)T_DROPPED) << b;
+        }
+        return o;
+      };
+      *reinterpret_cast<uint2*>(s_lab + q8) = make_uint2(merge(t.x, l.x), merge(t.y, l.y));
+    } else {
+      for (int q = q8; q < nshots; q++) s_lab[q] = ctag[q] == T_UNDECIDED ? lab[q] : T_DROPPED;
+    }
+  }
   // each thread keeps up to two unresolved candidates and their first PKEEP
-  // partners in registers for all passes; further ones are re-read per pass
+  // partners in registers for all passes; further ones are re-read per pass.
+  // Records below nshots are read whatever the count says (they are in bounds)
+  // and used below the count only, so their loads overlap the count's.
   constexpr int NC = 2;
   int cq[NC], cm[NC];
   int ce[NC][PKEEP];
@@ -1293,23 +1374,20 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const int u = tid + c * (int)blockDim.x;
-    cq[c] = -1;
-    cm[c] = 0;
-    cpl[c] = nullptr;
-#pragma unroll
-    for (int t = 0; t < PKEEP; t++) ce[c][t] = 0;
-    if (!ovf && u < nu) {
-      const int q = unres[u];
-      const int m = pcnt[q];
-      const int o = poff[q];
-      const int* pl = o == -1 ? slab + (size_t)q * KMAX : pool + o;
-      cq[c] = q;
-      cm[c] = m;
-      cpl[c] = pl;
-#pragma unroll
-      for (int t = 0; t < PKEEP; t++)
-        if (t < m) ce[c][t] = pl[t];
+    int4 h = make_int4(-1, 0, 0, 0), e = make_int4(0, 0, 0, 0);
+    if (u < nshots) {
+      const int4* rc = reinterpret_cast<const int4*>(urec + u);
+      h = rc[0];
+      e = rc[1];
     }
+    const bool use = !ovf && u < nu;
+    cq[c] = use ? h.x : -1;
+    cm[c] = use ? h.y : 0;
+    cpl[c] = use ? (h.z == -1 ? slab + (size_t)h.x * KMAX : pool + h.z) : nullptr;
+    ce[c][0] = e.x;
+    ce[c][1] = e.y;
+    ce[c][2] = e.z;
+    ce[c][3] = e.w;
   }
   __syncthreads();
   if (gate != gate_id) return;  // the whole workgroup
@@ -1339,7 +1417,8 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
     return true;
   };
   if (!ovf) {
-    for (int pass = 0; pass < nshots + 2; pass++) {
+    // with no unresolved candidate (nu is the same for the whole workgroup) no pass is needed
+    for (int pass = 0; nu > 0 && pass < nshots + 2; pass++) {
       if (tid == 0) {
         s_prog = 0;
         s_left = 0;
@@ -1360,14 +1439,13 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
         }
       }
       for (int u = tid + NC * (int)blockDim.x; u < nu; u += blockDim.x) {
-        const int q = unres[u];
+        const int4* rc = reinterpret_cast<const int4*>(urec + u);
+        const int4 h = rc[0], e = rc[1];
+        const int q = h.x;
         if (s_lab[q] != T_UNRESOLVED) continue;
-        const int m = pcnt[q];
-        const int o = poff[q];
-        const int* pl = o == -1 ? slab + (size_t)q * KMAX : pool + o;
-        int cache[PKEEP];
-#pragma unroll
-        for (int t = 0; t < PKEEP; t++) cache[t] = t < m ? pl[t] : 0;
+        const int m = h.y;
+        const int* pl = h.z == -1 ? slab + (size_t)q * KMAX : pool + h.z;
+        const int cache[PKEEP] = {e.x, e.y, e.z, e.w};
         uint8_t res;
         if (resolve(m, pl, cache, res)) {
           s_lab[q] = res;
@@ -1388,45 +1466,65 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
         break;
       }
     }
+    // counts over 8 labels per thread, summed over the wave, one LDS atomic per wave
     int acc = 0, con = 0;
-    for (int q = tid; q < nshots; q += blockDim.x) {
-      acc += s_lab[q] == T_UNDECIDED;
-      con += s_lab[q] == T_CONFLICT;
+    for (int q8 = tid * 8; q8 < nshots; q8 += 8 * (int)blockDim.x) {
+      if (q8 + 8 <= nshots) {
+        const uint2 w = *reinterpret_cast<const uint2*>(s_lab + q8);
+#pragma unroll
+        for (int b = 0; b < 64; b += 8) {
+          const uint32_t v = ((b < 32 ? w.x >> b : w.y >> (b - 32))) & 0xffu;
+          acc += v == T_UNDECIDED;
+          con += v == T_CONFLICT;
+        }
+      } else {
+        for (int q = q8; q < nshots; q++) {
+          acc += s_lab[q] == T_UNDECIDED;
+          con += s_lab[q] == T_CONFLICT;
+        }
+      }
     }
-    if (acc) atomicAdd(&s_acc, acc);
-    if (con) atomicAdd(&s_con, con);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      acc += __shfl_down(acc, off, 64);
+      con += __shfl_down(con, off, 64);
+    }
+    if ((tid & 63) == 0) {
+      if (acc) atomicAdd(&s_acc, acc);
+      if (con) atomicAdd(&s_con, con);
+    }
   }
   __syncthreads();
   if (tid != 0) return;
   ctl->acc = s_acc;
   ctl->con = s_con;
   ctl->label_stuck = s_stuck;
+  // counters whose values are published below
+  ctl->dropped = 0;
+  ctl->img_mismatch = 0;
+  ctl->seg_cursor = 0;
+  ctl->nunres = 0;
+  if (!ovf) ctl->pool_cursor = 0;
   // the draw's refinement decision, as the host makes it after the draw
   // (operator_algorithm.ixx:170-180): miss rate = misses / (1e-6 + shots),
   // misses = shots - (spheres now - spheres after the previous draw of this
   // proceed) in unsigned 64-bit arithmetic; miss_base carries the part of the
   // difference placed before the draw (non-zero on a proceed's first draw).
   int open = 0;
-  if (!ovf) ctl->acc_draw += s_acc;
+  const int acc_draw = ovf ? acc_before : acc_before + s_acc;
+  ctl->acc_draw = acc_draw;
   if (!ovf && !s_stuck && s_con == 0) {
-    const uint64_t miss = (uint64_t)nshots - (uint64_t)ctl->acc_draw - miss_base;
+    const uint64_t miss = (uint64_t)nshots - (uint64_t)acc_draw - miss_base;
     const double rate = (double)miss / (0.000001 + (double)(uint64_t)nshots);
     open = !(rate > aimed);
   }
   ctl->gate = open ? draw_id + 1 : -1;
   ctl->swept = seq;
-  const int v[kPubFields] = {s_acc,           s_con,          s_stuck, ctl->dropped, ctl->img_mismatch,
-                             ctl->pool_overflow, ctl->need_pool, ctl->cell_overflow, open};
+  const int v[kPubFields] = {s_acc, s_con, s_stuck, pub_dropped, pub_mism, pub_povf, pub_need, pub_cell, open};
   DrawPub* hp = host + (seq & (kPubSlots - 1));
   for (int k = 0; k < kPubFields; k++) __atomic_store_n(&reinterpret_cast<int*>(hp)[k], v[k], __ATOMIC_RELAXED);
   __threadfence_system();
   __atomic_store_n(&hp->seq, seq, __ATOMIC_RELEASE);
-  // counters whose values are now published
-  ctl->dropped = 0;
-  ctl->img_mismatch = 0;
-  ctl->seg_cursor = 0;
-  ctl->nunres = 0;
-  if (!ovf) ctl->pool_cursor = 0;
 }
 
 // update_sphere_tags on the touched real cells (operator_algorithm.ixx:446-473),
@@ -1446,10 +1544,97 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
   const int swept = ctl->swept;  // all reads are issued before the tests
   const bool skip_round = ctl->pool_overflow || ctl->label_stuck;
   const bool commit = ctl->con == 0;
-  const bool on = i < nshots && owner[i];
+  const bool in = i < nshots;
+  // Every thread also reads its list's offset, length and cell, and the row at
+  // i * CSLOT where a list of at most CSLOT entries starts (k_lists), so that
+  // these loads overlap the ones above (all in bounds whatever they hold).
+  const bool on = in && owner[i];
+  const int sg = in ? seg[i] : 0;
+  const int n0 = in ? len[i] : 0;
+  const int c = in ? ccell[i] : 0;
+  int4 r0 = make_int4(0, 0, 0, 0), r1 = r0;
+  if (in) {
+    const int4* row = reinterpret_cast<const int4*>(ord + (size_t)i * CSLOT);
+    r0 = row[0];
+    r1 = row[1];
+  }
   if (swept != seq || !on || skip_round) return;  // its sweep did not run (closed gate), or no owner
-  int* a = ord + seg[i];
-  int n = len[i];
+  // an owner's cell is in the grid
+  const int cnt0 = commit ? (int)cs.cnt[c] : 0;
+  if (sg == i * CSLOT && n0 <= CSLOT) {
+    // the list in registers: its tags are loaded together, the removal (same
+    // steps as the loop below) works on the registers
+    int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    uint8_t e[CSLOT];
+    bool und[CSLOT];
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++) {
+      e[k] = T_DROPPED;
+      und[k] = false;
+      if (k < n0) {
+        const uint8_t tg = ctag[v[k]], lb = lab[v[k]];
+        e[k] = tg == T_UNDECIDED ? lb : tg;
+        und[k] = tg == T_UNDECIDED && lb == T_UNDECIDED;
+      }
+    }
+    int m = n0;
+#pragma unroll
+    for (int q = CSLOT - 1; q >= 0; q--) {
+      if (q < n0 && e[q] == T_SKIP) {
+        ctag[v[q]] = T_SKIP;
+        int x = v[q];
+        bool ux = und[q];
+#pragma unroll
+        for (int k = q + 1; k < CSLOT; k++) {
+          x = (m - 1 == k) ? v[k] : x;
+          ux = (m - 1 == k) ? und[k] : ux;
+        }
+        v[q] = x;
+        und[q] = ux;
+        m--;
+      }
+    }
+    len[i] = m;
+    int* a = ord + sg;
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++) {
+      if (k < m) {
+        a[k] = v[k];
+        if (und[k]) ctag[v[k]] = T_VALIDATED;
+        // Conflict candidates keep the Undecided tag for the next round
+      }
+    }
+    if (!commit) return;
+    Slot<D> sl[CSLOT];
+    uint32_t ph[CSLOT];
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++) {
+      if (k < m) {
+#pragma unroll
+        for (int d = 0; d < D; d++) sl[k].c[d] = cpos[(size_t)d * nshots + v[k]];
+        sl[k].r = crad[v[k]];
+        ph[k] = cphase[v[k]];
+      }
+    }
+    int k0 = cnt0;
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++) {
+      if (k < m) {
+        if (k0 >= cs.cap) {
+          atomicMax(&ctl->cell_overflow, 1);
+          break;
+        }
+        cs.sph[(size_t)c * cs.cap + k0] = sl[k];
+        cs.phase[(size_t)c * cs.cap + k0] = ph[k];
+        k0++;
+      }
+    }
+    cs.cnt[c] = (uint8_t)k0;
+    cell2t[c] = -1;
+    return;
+  }
+  int* a = ord + sg;
+  int n = n0;
   for (int q = n - 1; q >= 0; q--) {
     const int j = a[q];
     const uint8_t e = ctag[j] == T_UNDECIDED ? lab[j] : ctag[j];
@@ -1466,8 +1651,7 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
     // Conflict candidates keep the Undecided tag for the next round
   }
   if (!commit) return;
-  const int c = ccell[i];
-  int k0 = cs.cnt[c];
+  int k0 = cnt0;
   for (int q = 0; q < n; q++) {
     const int j = a[q];
     if (k0 >= cs.cap) {
@@ -2843,7 +3027,7 @@ class Engine final : public IEngine {
     next_.reserve(c);
     owner_.reserve(c);
     slab_.reserve(c * KMAX);
-    unres_.reserve(c);
+    urec_.reserve(c);
     // kernels of a draw whose gate stays closed read these before returning
     HIP_OK(hipMemsetAsync(owner_.p, 0, c * sizeof(uint8_t), st_));
     HIP_OK(hipMemsetAsync(ccell_.p, 0xff, c * sizeof(int), st_));
@@ -3080,7 +3264,8 @@ class Engine final : public IEngine {
   rsa_gpu_radius_law law_ = nullptr;
   void* law_user_ = nullptr;
   uint32_t law_phase_ = 0;
-  DevBuf<int> next_, cslot_, ovhead_, slab_, pool_, unres_;
+  DevBuf<int> next_, cslot_, ovhead_, slab_, pool_;
+  DevBuf<URec> urec_;
   DevBuf<uint8_t> owner_;
   DevBuf<int> stab_;  // packed stencil offsets (Geo::stab), then linear ones (Geo::slin)
 };
@@ -3350,13 +3535,12 @@ void Engine<D>::enqueue_round(const Prep& pr, int seq, int gate_id) {
   const unsigned nbs = nblocks(nshots);
   timed("k_partner_lists", (double)nshots * (8.0 * (D + 1) + 10.0), [&] {
     hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, cpos_.p, crad_.p, cprio_.p,
-                       ccell_.p, ctag_.p, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, poff_.p, pcnt_.p, slab_.p,
-                       pool_.p, (int)pool_.cap, dctl_.p, gate_id, unres_.p);
+                       ccell_.p, ctag_.p, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, slab_.p, pool_.p,
+                       (int)pool_.cap, dctl_.p, gate_id, urec_.p);
   });
   timed("k_sweep_labels", (double)nshots * 4.0, [&] {
-    hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(1024), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, poff_.p,
-                       pcnt_.p, slab_.p, pool_.p, dctl_.p, hpub_dev_, seq, gate_id, pr.id, aimed_, pr.miss_base,
-                       unres_.p);
+    hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(1024), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, slab_.p,
+                       pool_.p, dctl_.p, hpub_dev_, seq, gate_id, pr.id, aimed_, pr.miss_base, urec_.p);
   });
   timed("k_update_commit", (double)nshots * (8.0 * (D + 1) + 12.0), [&] {
     hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
