@@ -794,7 +794,8 @@ constexpr int KREJ = 96;  // rejected voxel words passed with the launch; more: 
 
 // The Lemire rejection chain of a draw as launch arguments: shot i's voxel word
 // sits at i * (2D + 1) + #{k : idx[k] <= i} (idx = shot index of each rejected
-// word, non-decreasing).  n < 0: read the offsets from the pinned array.
+// word, non-decreasing; entries n .. KREJ-1 hold INT_MAX).  n < 0: read the
+// offsets from the pinned array.
 struct RejList {
   int n;
   int idx[KREJ];
@@ -848,7 +849,7 @@ constexpr int kPubSlots = 8;
 // (operator_cells_and_data_storage.ixx:54-76: at this point only placed spheres
 // and their images are validated); each candidate takes an arrival slot in its
 // cell (shot order is restored in k_lists).
-template <int D>
+template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
                                                       int nshots, uint32_t N, const uint32_t* __restrict__ vpos,
                                                       const double* __restrict__ hrad, RejList rl,
@@ -877,22 +878,43 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
   // radius: the class radius, or the shot's value of a custom law (host-evaluated)
   double r = cls.radius;
   if (valid) {
-    if (hrad) r = hrad[i];
-    prio = ring_at(ring, mask, w0 + (uint64_t)end_rel + i) >> 1;
     uint32_t v;
     if (rl.n >= 0) {
+      // rejected words before shot i, the list read 16 entries (one scalar load) at a time
       int k = 0;
-      for (int t = 0; t < rl.n; t++) k += rl.idx[t] <= i;
+#pragma unroll
+      for (int c0 = 0; c0 < KREJ; c0 += 16) {
+        if (c0 >= rl.n) break;
+#pragma unroll
+        for (int t = 0; t < 16; t++) k += rl.idx[c0 + t] <= i ? 1 : 0;
+      }
       v = (uint32_t)i * (uint32_t)(2 * D + 1) + (uint32_t)k;
     } else {
       v = vpos[i];
     }
+    if (hrad) r = hrad[i];
+    // every ring word of the shot is requested before any of them is used
+    const uint32_t pw = ring_at(ring, mask, w0 + (uint64_t)end_rel + i);
     const uint32_t gw = ring_at(ring, mask, w0 + v);
+    uint32_t uw[2 * D];
+#pragma unroll
+    for (int q = 0; q < 2 * D; q++) uw[q] = ring_at(ring, mask, w0 + v + 1 + q);
+    prio = pw >> 1;
     const uint32_t id = (uint32_t)(((uint64_t)gw * (uint64_t)N) >> 32);
+    // the voxel's coordinates (the launch picks the instance for the lattice's
+    // width, so the D loads are issued together)
+    int64_t vc[D];
+    if constexpr (WIDE) {
+#pragma unroll
+      for (int d = 0; d < D; d++) vc[d] = vx.w[d][id];
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; d++) vc[d] = (int64_t)vx.c[d][id];
+    }
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      const double o = vx.origin[d] + (double)vx.coord(d, id) * vx.len[d];
-      const double u = canonical(ring_at(ring, mask, w0 + v + 1 + 2 * d), ring_at(ring, mask, w0 + v + 2 + 2 * d));
+      const double o = vx.origin[d] + (double)vc[d] * vx.len[d];
+      const double u = canonical(uw[2 * d], uw[2 * d + 1]);
       p[d] = o + vx.len[d] * u;
     }
     cell = cell_index<D>(g, p);
@@ -3402,6 +3424,9 @@ typename Engine<D>::Prep Engine<D>::prepare(uint64_t w, int64_t nvox, int b) {
   p.w_shots = w;
   p.rej.n = 0;
   p.end_rel = host_chain(p.nshots, w, (uint32_t)nvox, b, &p.rej);
+  // entries past the list's end compare above every shot index
+  if (p.rej.n >= 0)
+    for (int t = p.rej.n; t < KREJ; t++) p.rej.idx[t] = INT_MAX;
   p.law = custom_;
   if (p.law && p.nshots <= kFastShots) law_radii(p, hrad_[b]);  // larger draws: in single_draw_slow
   return p;
@@ -3567,11 +3592,12 @@ void Engine<D>::enqueue_first(Prep& pr, bool forced, uint64_t miss_base) {
   const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
   const unsigned nbs = nblocks(nshots);
   timed("k_decode_filter", (double)nshots * (4.0 * (2 * D + 2) + 4.0 * D + 8.0 * (D + 1) + 13.0), [&] {
-    hipLaunchKernelGGL(k_decode_filter<D>, dim3(nbl), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N,
-                       hvpos_dev_[pr.buf], pr.law ? hrad_dev_[pr.buf] : nullptr, pr.rej, pr.end_rel, vview(), cls,
-                       geo_, cs(), cpos_.p, crad_.p, cprio_.p,
-                       cphase_.p, ccell_.p, ctag_.p, owner_.p, cellcc_.p, cslot_.p, ovhead_.p, next_.p, dctl_.p, pr.id,
-                       forced ? 1 : 0);
+    const VoxView<D> vv = vview();
+    auto kern = vv.wide ? k_decode_filter<D, true> : k_decode_filter<D, false>;
+    hipLaunchKernelGGL(kern, dim3(nbl), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N, hvpos_dev_[pr.buf],
+                       pr.law ? hrad_dev_[pr.buf] : nullptr, pr.rej, pr.end_rel, vv, cls, geo_, cs(), cpos_.p,
+                       crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p, owner_.p, cellcc_.p, cslot_.p, ovhead_.p,
+                       next_.p, dctl_.p, pr.id, forced ? 1 : 0);
   });
   timed("k_lists", (double)nshots * 24.0, [&] {
     hipLaunchKernelGGL(k_lists, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, owner_.p, ctag_.p, cellcc_.p, cslot_.p,
