@@ -125,11 +125,11 @@ struct Nb {
   bool ghost;
 };
 
+// neighbour() with stencil entry s's packed offsets already loaded (packed = g.stab[s])
 template <int D>
-__device__ __forceinline__ bool neighbour(const Geo<D>& g, const int* c, int s, Nb<D>& nb) {
+__device__ __forceinline__ bool neighbour_at(const Geo<D>& g, const int* c, int packed, Nb<D>& nb) {
   int64_t idx = 0;
   nb.ghost = false;
-  const int packed = g.stab[s];
 #pragma unroll
   for (int d = D - 1; d >= 0; d--) {
     int k = c[d] + stencil_off<D>(packed, d);
@@ -150,6 +150,11 @@ __device__ __forceinline__ bool neighbour(const Geo<D>& g, const int* c, int s, 
   }
   nb.src = (int)idx;
   return true;
+}
+
+template <int D>
+__device__ __forceinline__ bool neighbour(const Geo<D>& g, const int* c, int s, Nb<D>& nb) {
+  return neighbour_at<D>(g, c, g.stab[s], nb);
 }
 
 // Ghost-area membership of a real sphere for the mirrored neighbour nb

@@ -226,6 +226,36 @@ enum Ctr : int {
 
 // ============================================================== kernels: word stream
 
+// A load issued where it is written: relaxed atomic loads are not moved past
+// the branches that follow them (the compiler sinks plain loads into the block
+// that uses them, behind the test that guards it).  Workgroup scope: an
+// ordinary global load, for data written by earlier kernels.
+template <class T>
+__device__ __forceinline__ T load_now(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// A cell slot through load_now.
+template <int D>
+__device__ __forceinline__ Slot<D> slot_now(const Slot<D>* p) {
+  Slot<D> q;
+#pragma unroll
+  for (int d = 0; d < D; d++) q.c[d] = load_now(&p->c[d]);
+  q.r = load_now(&p->r);
+  return q;
+}
+
+// The packed stencil entries of a lane's CPL cells (lane `sub` takes entries
+// sub * CPL ..), read at kernel start: they depend on the lane only.
+template <int D, int CPL, int NST>
+__device__ __forceinline__ void lane_stencil(const Geo<D>& g, int sub, int (&pk)[CPL]) {
+#pragma unroll
+  for (int jj = 0; jj < CPL; jj++) {
+    const int s = sub * CPL + jj;
+    pk[jj] = load_now(g.stab + (s < NST ? s : 0));
+  }
+}
+
 // Kernels never read device memory that a copy engine wrote through a plain
 // load: an XCD's L2 may still hold an older line of it (the ring's slots are
 // rewritten every kRing words, and pool memory changes hands).  Ring words are
@@ -935,12 +965,12 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
       if (s >= LG::NST || hit) continue;
       Nb<D> nb;
       if (!neighbour<D>(g, cc, s, nb)) continue;
-      // the first slot is loaded with the count (every cell has cap >= 1 slots)
-      const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
+      // slots are read only for occupied cells: most neighbour cells of a late
+      // draw are empty, and a slot is a cache line of its own
       const int nv = cs.cnt[nb.src];
-      const Slot<D> q0 = sl[0];
+      const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
       for (int k = 0; k < nv && !hit; k++) {
-        const Slot<D> q = k == 0 ? q0 : sl[k];
+        const Slot<D> q = sl[k];
         double qp[D];
         if (nb.ghost) {
           if (!image_of<D>(g, nb, q.c, qp)) continue;
@@ -996,20 +1026,21 @@ __global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict_
   const int gate = ctl->gate;  // tested before the first store (loads overlap)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < nshots;
-  // The cell, its count and its fixed slots are read by every thread, owner or
-  // not, so that the loads do not wait for each other's tests.  With a closed
-  // gate k_decode_filter did not run, and owner/ccell hold stale or never-written
-  // values: the clamped cell keeps these reads in bounds.
-  const bool on = in && owner[i];
-  const int c0 = in ? ccell[i] : 0;
+  // Every thread reads, without a branch (index clamped to candidate 0), its
+  // owner flag, cell, the cell's count and fixed slots, so that these loads are
+  // issued together.  With a closed gate k_decode_filter did not run, and
+  // owner/ccell hold stale or never-written values: the clamped cell keeps these
+  // reads in bounds.
+  const int ic = in ? i : 0;
+  const uint8_t own = load_now(owner + ic);
+  const bool on = in && own;
+  const int c0 = load_now(ccell + ic);
   const int c = (c0 >= 0 && c0 < ncells) ? c0 : 0;
-  const int n = in ? cellcc[c] : 0;
-  const int4* row = reinterpret_cast<const int4*>(cslot + (size_t)c * CSLOT);
-  int4 r0 = make_int4(0, 0, 0, 0), r1 = r0;
-  if (in) {
-    r0 = row[0];
-    r1 = row[1];
-  }
+  const int n = load_now(cellcc + c);
+  const int64_t* row = reinterpret_cast<const int64_t*>(cslot + (size_t)c * CSLOT);
+  const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
+  const int4 r0 = make_int4((int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32));
+  const int4 r1 = make_int4((int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32));
   if (gate != draw_id || !on) return;
   // the owner's list: CSLOT entries at i * CSLOT, longer lists after all of those
   const int base = n <= CSLOT ? i * CSLOT : nshots * CSLOT + atomicAdd(&ctl->seg_cursor, n);
@@ -1029,9 +1060,13 @@ __global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict_
       v[NET[e][0]] = min(x, y);
       v[NET[e][1]] = max(x, y);
     }
+    // the tags are loaded without a branch (entries past n read the owner's own tag)
+    uint8_t tg[CSLOT];
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++) tg[k] = ctag[k < n ? v[k] : i];
     bool sk[CSLOT];
 #pragma unroll
-    for (int k = 0; k < CSLOT; k++) sk[k] = k < n && ctag[v[k]] == T_SKIP;
+    for (int k = 0; k < CSLOT; k++) sk[k] = k < n && tg[k] == T_SKIP;
     // remove Skip from the back, the last entry taking the removed one's place
 #pragma unroll
     for (int q = CSLOT - 1; q >= 0; q--) {
@@ -1088,6 +1123,8 @@ __global__ void __launch_bounds__(WG) k_refilter(Geo<D> g, CellStore<D> cs, int 
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
   const int sub = gtid % LG::LPC;
+  int pk[LG::CPL];
+  lane_stencil<D, LG::CPL, LG::NST>(g, sub, pk);
   const bool active = i < nshots && ctag[i] == T_UNDECIDED;
   bool hit = false;
   if (active) {
@@ -1101,13 +1138,13 @@ __global__ void __launch_bounds__(WG) k_refilter(Geo<D> g, CellStore<D> cs, int 
       const int s = sub * LG::CPL + jj;
       if (s >= LG::NST || hit) continue;
       Nb<D> nb;
-      if (!neighbour<D>(g, cc, s, nb)) continue;
-      // the first slot is loaded with the count (every cell has cap >= 1 slots)
-      const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
+      if (!neighbour_at<D>(g, cc, pk[jj], nb)) continue;
+      // slots are read only for occupied cells: most neighbour cells of a late
+      // draw are empty, and a slot is a cache line of its own
       const int nv = cs.cnt[nb.src];
-      const Slot<D> q0 = sl[0];
+      const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
       for (int k = 0; k < nv && !hit; k++) {
-        const Slot<D> q = k == 0 ? q0 : sl[k];
+        const Slot<D> q = sl[k];
         double qp[D];
         if (nb.ghost) {
           if (!image_of<D>(g, nb, q.c, qp)) continue;
@@ -1163,8 +1200,9 @@ template <int D>
 __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int sub, int nshots, const double* p,
                                              double r, uint32_t pi, const double* cpos, const double* crad,
                                              const uint32_t* cprio, const uint8_t* ctag, const int* cell2t,
-                                             const int* seg, const int* len, const int* ord, int* out,
-                                             bool& first_ghost, int* mismatch, int (*keep)[PKEEP] = nullptr) {
+                                             const int* seg, const int* len, const int* ord, const int* pk,
+                                             int* out, bool& first_ghost, int* mismatch,
+                                             int (*keep)[PKEEP] = nullptr) {
   using LG = LaneGroup<D>;
   int n = 0;
   first_ghost = false;
@@ -1173,17 +1211,18 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
     const int s = sub * LG::CPL + jj;
     if (s >= LG::NST) continue;
     Nb<D> nb;
-    if (!neighbour<D>(g, cc, s, nb)) continue;
+    if (!neighbour_at<D>(g, cc, pk[jj], nb)) continue;
     const int t = cell2t[nb.src];
     if (t < 0) continue;
     // A list of at most CSLOT entries starts at t * CSLOT (k_lists): that row is
     // read with the list's offset and length (t < nshots keeps it in bounds) and
     // used whenever the offset says the list starts there.
-    const int sg = seg[t];
-    const int ln = len[t];
-    const int4* row = reinterpret_cast<const int4*>(ord + (size_t)t * CSLOT);
-    const int4 r0 = row[0], r1 = row[1];
-    const int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    const int sg = load_now(seg + t);
+    const int ln = load_now(len + t);
+    const int64_t* row = reinterpret_cast<const int64_t*>(ord + (size_t)t * CSLOT);
+    const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
+    const int v[CSLOT] = {(int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32),
+                          (int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32)};
     const bool in_row = sg == t * CSLOT;
     const int* o = ord + sg;
     for (int k = 0; k < ln; k++) {
@@ -1196,11 +1235,12 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
         j = o[k];
       }
       // a candidate's tag, priority, position and radius are loaded together
-      const uint8_t tj = ctag[j];
-      const uint32_t pj = cprio[j];
-      const double rj = crad[j];
+      const uint8_t tj = load_now(ctag + j);
+      const uint32_t pj = load_now(cprio + j);
+      const double rj = load_now(crad + j);
       double jp[D], qp[D];
-      cand_pos<D>(cpos, nshots, j, jp);
+#pragma unroll
+      for (int d = 0; d < D; d++) jp[d] = load_now(cpos + (size_t)d * nshots + j);
       if (tj != T_UNDECIDED) continue;
       if (!(pj < pi)) continue;
       if (nb.ghost) {
@@ -1249,6 +1289,8 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
   const int sub = gtid % LG::LPC;
+  int pk[LG::CPL];
+  lane_stencil<D, LG::CPL, LG::NST>(g, sub, pk);
   // read before the gate test: with a closed gate the candidate arrays are stale
   // or never written, so a cell outside the grid disables the lane
   const int ci = i < nshots ? ccell[i] : -1;
@@ -1266,8 +1308,8 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
     r = crad[i];
     pi = cprio[i];
     cell_coords<D>(g, ccell[i], cc);
-    n = lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, nullptr, fg,
-                         &mism, &keep);
+    n = lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, pk, nullptr,
+                         fg, &mism, &keep);
   }
   if (gate != gate_id) return;  // whole grid
   if (mism) atomicAdd(&ctl->img_mismatch, mism);
@@ -1325,7 +1367,7 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
       }
     } else if (n > PKEEP) {  // rare: walk the cells again, writing as it goes
       bool dummy;
-      lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, dst + b,
+      lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, pk, dst + b,
                        dummy, nullptr);
       for (int t = b; t < PKEEP; t++) head[t] = dst[t];
     }
@@ -1567,37 +1609,41 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
   const bool skip_round = ctl->pool_overflow || ctl->label_stuck;
   const bool commit = ctl->con == 0;
   const bool in = i < nshots;
-  // Every thread also reads its list's offset, length and cell, and the row at
-  // i * CSLOT where a list of at most CSLOT entries starts (k_lists), so that
-  // these loads overlap the ones above (all in bounds whatever they hold).
-  const bool on = in && owner[i];
-  const int sg = in ? seg[i] : 0;
-  const int n0 = in ? len[i] : 0;
-  const int c = in ? ccell[i] : 0;
-  int4 r0 = make_int4(0, 0, 0, 0), r1 = r0;
-  if (in) {
-    const int4* row = reinterpret_cast<const int4*>(ord + (size_t)i * CSLOT);
-    r0 = row[0];
-    r1 = row[1];
-  }
+  // Every thread also reads, without a branch (index clamped to candidate 0), its
+  // owner flag, its list's offset, length and cell, and the row at i * CSLOT where
+  // a list of at most CSLOT entries starts (k_lists), so that these loads are
+  // issued with the ones above (all in bounds whatever they hold).
+  const int ic = in ? i : 0;
+  const uint8_t own = load_now(owner + ic);
+  const bool on = in && own;
+  const int sg = load_now(seg + ic);
+  const int n0 = load_now(len + ic);
+  const int c = load_now(ccell + ic);
+  const int64_t* row = reinterpret_cast<const int64_t*>(ord + (size_t)ic * CSLOT);
+  const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
+  const int4 r0 = make_int4((int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32));
+  const int4 r1 = make_int4((int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32));
   if (swept != seq || !on || skip_round) return;  // its sweep did not run (closed gate), or no owner
   // an owner's cell is in the grid
-  const int cnt0 = commit ? (int)cs.cnt[c] : 0;
+  const int cnt0 = load_now(cs.cnt + c);
   if (sg == i * CSLOT && n0 <= CSLOT) {
     // the list in registers: its tags are loaded together, the removal (same
     // steps as the loop below) works on the registers
     int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    // tags and labels without a branch (entries past n0 read the owner's own)
+    uint8_t tg[CSLOT], lb[CSLOT];
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++) {
+      const int j = k < n0 ? v[k] : i;
+      tg[k] = ctag[j];
+      lb[k] = lab[j];
+    }
     uint8_t e[CSLOT];
     bool und[CSLOT];
 #pragma unroll
     for (int k = 0; k < CSLOT; k++) {
-      e[k] = T_DROPPED;
-      und[k] = false;
-      if (k < n0) {
-        const uint8_t tg = ctag[v[k]], lb = lab[v[k]];
-        e[k] = tg == T_UNDECIDED ? lb : tg;
-        und[k] = tg == T_UNDECIDED && lb == T_UNDECIDED;
-      }
+      e[k] = k < n0 ? (tg[k] == T_UNDECIDED ? lb[k] : tg[k]) : T_DROPPED;
+      und[k] = k < n0 && tg[k] == T_UNDECIDED && lb[k] == T_UNDECIDED;
     }
     int m = n0;
 #pragma unroll
@@ -1631,7 +1677,7 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
     uint32_t ph[CSLOT];
 #pragma unroll
     for (int k = 0; k < CSLOT; k++) {
-      if (k < m) {
+      if (k < m) {  // m is small: the loads of the kept entries, in one block each
 #pragma unroll
         for (int d = 0; d < D; d++) sl[k].c[d] = cpos[(size_t)d * nshots + v[k]];
         sl[k].r = crad[v[k]];
