@@ -221,6 +221,7 @@ enum Ctr : int {
   C_KEEP,         // uncovered voxels
   C_NCHILD,       // children kept by subdivide
   C_GROUPED,      // voxels binned by cell (k_cover's processing order)
+  C_SCATTER_ERR,  // k_scatter found a workgroup's children inconsistent with the counting pass
   C_NUM
 };
 
@@ -2196,50 +2197,113 @@ __device__ __forceinline__ uint32_t kept_code(const uint32_t* code, const uint32
   return c != 0 ? c : moved[i];
 }
 
+// Exclusive prefix of one int per thread over a workgroup of WG threads, and
+// the workgroup's total: a shuffle scan per wave, then the wave totals through
+// LDS (one barrier).
+__device__ __forceinline__ int block_excl_scan(int c, int& total) {
+  __shared__ int s_w[WG / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int incl = c;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  if (lane == 63) s_w[wv] = incl;
+  __syncthreads();
+  int base = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < WG / 64; w++) {
+    const int x = s_w[w];
+    base += w < wv ? x : 0;
+    total += x;
+  }
+  return base + incl - c;
+}
+
+// kept_code with both words requested together (moved is read whatever code says)
+__device__ __forceinline__ uint32_t kept_code_now(const uint32_t* code, const uint32_t* moved, int64_t i) {
+  const uint32_t c = code[i], m = moved[i];
+  return c != 0 ? c : m;
+}
+
 __global__ void __launch_bounds__(WG) k_child_count(int64_t n, const int* __restrict__ ctr,
                                                     const uint32_t* __restrict__ code,
                                                     const uint32_t* __restrict__ moved, int* __restrict__ blk) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t keep = ctr[C_KEEP];
-  const int c = i < n && i < keep ? __popc(kept_code(code, moved, i) & 0xffffu) : 0;
-  __shared__ int sh[WG];
-  sh[threadIdx.x] = c;
-  __syncthreads();
-  for (int off = WG / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) blk[blockIdx.x] = sh[0];
+  const uint32_t k = i < n && i < keep ? kept_code(code, moved, i) : 0u;
+  const int c = __popc(k & 0xffffu);
+  int total;
+  block_excl_scan(c, total);
+  if (threadIdx.x == 0) blk[blockIdx.x] = total;
 }
 
 // stable append of the kept children, parents in list order, children in
 // TabCorner order (list_of_voxels.ixx:76-99): coordinates 2c + tab[k]
-template <int D>
+// SW / DW: the source and destination lattices hold 64-bit coordinates (the
+// launch picks the instance, so coordinate loads and stores carry no branch).
+template <int D, bool SW, bool DW>
 __global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __restrict__ code,
                                                 const uint32_t* __restrict__ moved, const int* __restrict__ blk,
-                                                VoxView<D> src, VoxView<D> dst, VoxParams<D> vp) {
+                                                VoxView<D> src, VoxView<D> dst, int64_t nchild,
+                                                int* __restrict__ err) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t m = i < n ? (kept_code(code, moved, i) & 0xffffu) : 0u;
-  const int c = __popc(m);
-  __shared__ int sh[WG];
-  sh[threadIdx.x] = c;
-  __syncthreads();
-  for (int off = 1; off < WG; off <<= 1) {
-    const int add = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += add;
-    __syncthreads();
-  }
-  if (i >= n || m == 0) return;
-  int o = blk[blockIdx.x] + sh[threadIdx.x] - c;
+  const bool in = i < n;
+  const int64_t ic = in ? i : 0;
+  const uint32_t m = in ? (kept_code_now(code, moved, ic) & 0xffffu) : 0u;
   int64_t pc[D];
-  for (int d = 0; d < D; d++) pc[d] = src.coord(d, i);
-  constexpr int NCH = ipow_c(2, D);
 #pragma unroll
-  for (int k = 0; k < NCH; k++) {
-    if (!(m & (1u << k))) continue;
-    for (int d = 0; d < D; d++) dst.set(d, o, 2 * pc[d] + TabCorner<D>::at(k, d));
-    o++;
+  for (int d = 0; d < D; d++) pc[d] = SW ? src.w[d][ic] : (int64_t)src.c[d][ic];
+  const int c = __popc(m);
+  int total;
+  const int excl = block_excl_scan(c, total);
+  const int base = blk[blockIdx.x];  // the workgroup's children are base .. base + total - 1
+  constexpr int NCH = ipow_c(2, D);
+  constexpr uint32_t FULLM = (1u << NCH) - 1u;  // NCH <= 16
+  // The children must agree with the counting pass (k_child_count + scan): the
+  // workgroup's range inside the new list and masks of NCH bits.  Otherwise
+  // nothing is written and the host is told (base and total are uniform).
+  if (__syncthreads_or((m & ~FULLM) != 0) || (int64_t)base + total > nchild) {
+    if (threadIdx.x == 0) atomicOr(err, 1);
+    return;
+  }
+  if constexpr (!DW && D <= 3) {
+    // 32-bit children staged in LDS, then stored as the workgroup's contiguous
+    // range with consecutive lanes on consecutive entries
+    __shared__ int32_t s_ch[D][WG * NCH];
+    if (in && m != 0) {
+      int o = excl;
+#pragma unroll
+      for (int k = 0; k < NCH; k++) {
+        if (!(m & (1u << k))) continue;
+#pragma unroll
+        for (int d = 0; d < D; d++) s_ch[d][o] = (int32_t)(2 * pc[d] + TabCorner<D>::at(k, d));
+        o++;
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < total; q += WG) {
+#pragma unroll
+      for (int d = 0; d < D; d++) const_cast<int32_t*>(dst.c[d])[base + q] = s_ch[d][q];
+    }
+  } else {
+    if (!in || m == 0) return;
+    int o = base + excl;
+#pragma unroll
+    for (int k = 0; k < NCH; k++) {
+      if (!(m & (1u << k))) continue;
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const int64_t v = 2 * pc[d] + TabCorner<D>::at(k, d);
+        if constexpr (DW)
+          const_cast<int64_t*>(dst.w[d])[o] = v;
+        else
+          const_cast<int32_t*>(dst.c[d])[o] = (int32_t)v;
+      }
+      o++;
+    }
   }
 }
 
@@ -2304,6 +2368,19 @@ struct StencilTable {
   int n;  // entries in v: 2 * 3^D
   int v[2 * 81];
 };
+// The cell store at a larger per-cell capacity: slot k of cell c moves from
+// c * cap0 + k to c * cap1 + k (grow_cells).
+template <int D>
+__global__ void k_grow_cells(int64_t total, int cap0, int cap1, const Slot<D>* __restrict__ src,
+                             const uint32_t* __restrict__ srcph, Slot<D>* __restrict__ dst,
+                             uint32_t* __restrict__ dstph) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int64_t c = t / cap0, k = t % cap0;
+  dst[c * cap1 + k] = src[t];
+  dstph[c * cap1 + k] = srcph[t];
+}
+
 __global__ void k_set_table(int* __restrict__ dst, StencilTable t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < t.n) dst[i] = t.v[i];
@@ -2383,15 +2460,24 @@ class Engine final : public IEngine {
       sup_[d] = ginf_a[d] + (0 + 1) * size;
       L_[d] = dsize[d];
     }
-    define_grid(domain_rad);  // the domain's own grid (rsa_domain.ixx:26); replaced by algo_init
-    for (int d = 1; d < D; d++)
-      if (sup_[d] - inf_[d] < 2 * domain_rad) throw Error("The domain is not sufficiently large!");
-    // non-blocking: no implicit ordering with the null stream (every copy and
-    // fill of the engine is on its own streams), so engines run side by side
+    // non-blocking: no implicit ordering with the null stream (every copy, fill
+    // and launch of the engine is on its own streams, the stencil table of the
+    // first define_grid included), so engines run side by side
     HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&cst_, hipStreamNonBlocking));
+    try {
+      define_grid(domain_rad);  // the domain's own grid (rsa_domain.ixx:26); replaced by algo_init
+      for (int d = 1; d < D; d++)
+        if (sup_[d] - inf_[d] < 2 * domain_rad) throw Error("The domain is not sufficiently large!");
+    } catch (...) {  // no destructor runs for a constructor that throws
+      (void)hipStreamSynchronize(st_);
+      (void)hipStreamDestroy(cst_);
+      (void)hipStreamDestroy(st_);
+      throw;
+    }
     hctr_ = static_cast<int*>(PinnedPool::get().alloc(sizeof(int) * C_NUM, hipHostMallocDefault));
     dctr_.reserve(C_NUM);
+    reset_ctr();  // pool memory: counters start at zero
     ring_.reserve(kRing);
     hs_.buf = static_cast<uint32_t*>(PinnedPool::get().alloc(sizeof(uint32_t) * kRing, hipHostMallocDefault));
     for (int b = 0; b < kPrepBufs; b++) {
@@ -2967,10 +3053,11 @@ class Engine final : public IEngine {
     DevBuf<uint32_t> ph;
     sph.reserve((size_t)geo_.ncells * cap);
     ph.reserve((size_t)geo_.ncells * cap);
-    HIP_OK(hipMemcpy2D(sph.p, cap * sizeof(Slot<D>), cells_sph_.p, cap_ * sizeof(Slot<D>), cap_ * sizeof(Slot<D>),
-                       geo_.ncells, hipMemcpyDeviceToDevice));
-    HIP_OK(hipMemcpy2D(ph.p, cap * sizeof(uint32_t), cells_ph_.p, cap_ * sizeof(uint32_t), cap_ * sizeof(uint32_t),
-                       geo_.ncells, hipMemcpyDeviceToDevice));
+    // copied by a kernel on the engine's stream (no copy-engine writes that later kernels read)
+    const int64_t total = (int64_t)geo_.ncells * cap_;
+    hipLaunchKernelGGL(k_grow_cells<D>, dim3(nblocks(total)), dim3(WG), 0, st_, total, cap_, cap, cells_sph_.p,
+                       cells_ph_.p, sph.p, ph.p);
+    HIP_OK(hipGetLastError());
     std::swap(cells_sph_.p, sph.p);
     std::swap(cells_sph_.cap, sph.cap);
     std::swap(cells_ph_.p, ph.p);
@@ -3007,6 +3094,10 @@ class Engine final : public IEngine {
   void read_ctr() {
     HIP_OK(hipMemcpyAsync(hctr_, dctr_.p, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, st_));
     HIP_OK(hipStreamSynchronize(st_));
+  }
+  // after read_ctr: a voxel list written inconsistently with its count is an error
+  void check_scatter() const {
+    if (hctr_[C_SCATTER_ERR]) throw Error("internal: voxel children disagree with their count (list not written)");
   }
 
   // one chunk [dev_hi_, end) of generated words onto cst_ (split at the ring's end)
@@ -3241,6 +3332,7 @@ class Engine final : public IEngine {
     hipLaunchKernelGGL(k_child_count, dim3(nb), dim3(WG), 0, st_, nvox_, dctr_.p, code_.p, moved_.p, blk_.p);
     scan_excl(blk_.p, nb, dctr_.p + C_NCHILD);
     read_ctr();
+    check_scatter();  // the previous refinement's children
     const int64_t keep = hctr_[C_KEEP];
     if (keep == 0) {  // subdivide_uncovered returns early on an empty list (list_of_voxels.ixx:60)
       nvox_ = 0;
@@ -3262,7 +3354,10 @@ class Engine final : public IEngine {
     const VoxView<D> dst = vview(nxt);
     const double cw = wide_[vcur_] ? 8.0 : 4.0, nw = wide_[nxt] ? 8.0 : 4.0;
     timed("k_scatter", (double)keep * (cw * D + 4.0) + (double)nchild * nw * D, [&] {
-      hipLaunchKernelGGL(k_scatter<D>, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, moved_.p, blk_.p, vx, dst, vp);
+      auto kern = wide_[vcur_] ? (wide_[nxt] ? k_scatter<D, true, true> : k_scatter<D, true, false>)
+                               : (wide_[nxt] ? k_scatter<D, false, true> : k_scatter<D, false, false>);
+      hipLaunchKernelGGL(kern, dim3(nb2), dim3(WG), 0, st_, keep, code_.p, moved_.p, blk_.p, vx, dst, nchild,
+                         dctr_.p + C_SCATTER_ERR);
     }, 1);
     HIP_OK(hipGetLastError());
     vcur_ = nxt;
@@ -3944,7 +4039,8 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     cur_live = nxt_live;
   }
   if (prof_) prof_add("host_wait_mt_words", 1e-9 * (double)hs_.wait_ns.load(), 0);
-  HIP_OK(hipStreamSynchronize(st_));
+  read_ctr();  // the stream is idle after this
+  check_scatter();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   st->nvox_final = nvox_;
