@@ -195,8 +195,13 @@ def main():
         "roofline": roofline,
         "phases_ms": {"voxel_refine": round(1e3 * stats["voxel_seconds"], 2),
                       "draws": round(1e3 * (stats["seconds"] - stats["voxel_seconds"]), 2),
-                      "rounds": stats["rounds"], "rounds_max": stats["rounds_max"]},
-        "profile": {kk: dict(launches=v["launches"] // args.steps, ms=round(1e3 * v["seconds"] / args.steps, 3))
+                      "rounds": stats["rounds"], "rounds_max": stats["rounds_max"],
+                      # the dart phase is latency-bound: its rate per draw
+                      "us_per_draw": round(1e6 * (stats["seconds"] - stats["voxel_seconds"]) / stats["draws"], 2),
+                      "draws_per_s": round(stats["draws"] / (stats["seconds"] - stats["voxel_seconds"]), 1)},
+        # per kernel (event-timed); GB/s = algorithmic bytes over kernel time where bytes are counted
+        "profile": {kk: dict(launches=v["launches"] // args.steps, ms=round(1e3 * v["seconds"] / args.steps, 3),
+                             **({"GBps": round(v["bytes"] / v["seconds"] / 1e9, 1)} if v["bytes"] > 0 else {}))
                     for kk, v in prof.items()},
         "kernels_all_bracketed": dict(step_ms=round(1e3 * dt2, 1), **detail),
         "batched": None if args.no_batch else batched(R, args.batch_workers),

@@ -2504,8 +2504,11 @@ class Engine final : public IEngine {
     PinnedPool::get().release(hrad_big_);
     PinnedPool::get().release(hs_.buf);
     PinnedPool::get().release(hpub_);
-    if (pe0_) (void)hipEventDestroy(pe0_);
-    if (pe1_) (void)hipEventDestroy(pe1_);
+    for (auto& k : pend_) {
+      (void)hipEventDestroy(k.a);
+      (void)hipEventDestroy(k.b);
+    }
+    for (auto e : evpool_) (void)hipEventDestroy(e);
     if (cst_) (void)hipStreamSynchronize(cst_);
     for (auto& c : chunks_) (void)hipEventDestroy(c.ev);
     for (auto e : free_ev_) (void)hipEventDestroy(e);
@@ -2709,15 +2712,17 @@ class Engine final : public IEngine {
 
   void profile_enable(int level) override {
     const bool on = level > 0;
+    StreamScope scope(st_, cst_);
+    resolve_prof();
     prof_ = on;
     prof_level_ = level;
     kprof_.clear();
-    if (on && !pe0_) {
-      HIP_OK(hipEventCreate(&pe0_));
-      HIP_OK(hipEventCreate(&pe1_));
-    }
   }
   bool profile_get(int k, const char** name, int64_t* launches, double* seconds, double* bytes) override {
+    {
+      StreamScope scope(st_, cst_);
+      resolve_prof();
+    }
     if (k < 0 || k >= (int)kprof_.size()) return false;
     *name = kprof_[k].name;
     *launches = kprof_[k].launches;
@@ -2880,23 +2885,50 @@ class Engine final : public IEngine {
   };
   bool prof_ = false;
   std::vector<KProf> kprof_;
-  hipEvent_t pe0_ = nullptr, pe1_ = nullptr;
-  // launch() runs the kernel launch; with profiling on it is bracketed by events
-  // on the context's stream and its algorithmic bytes are accumulated
+  // launch() runs the kernel launch; with profiling on it is bracketed by a pair
+  // of events on the context's stream (the host does not wait for them), and
+  // the pair is read in resolve_prof() once the stream has drained
   int prof_level_ = 0;  // 1: refine kernels only, 2: every bracketed kernel
+  struct PendingK {
+    const char* name;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  std::vector<PendingK> pend_;
+  std::vector<hipEvent_t> evpool_;
+  hipEvent_t take_event() {
+    if (evpool_.empty()) {
+      hipEvent_t e;
+      HIP_OK(hipEventCreate(&e));
+      return e;
+    }
+    const hipEvent_t e = evpool_.back();
+    evpool_.pop_back();
+    return e;
+  }
   template <class F>
   void timed(const char* name, double bytes, F&& launch, int level = 2) {
     if (!prof_ || level > prof_level_) {
       launch();
       return;
     }
-    HIP_OK(hipEventRecord(pe0_, st_));
+    PendingK k{name, bytes, take_event(), take_event()};
+    HIP_OK(hipEventRecord(k.a, st_));
     launch();
-    HIP_OK(hipEventRecord(pe1_, st_));
-    HIP_OK(hipEventSynchronize(pe1_));
-    float ms = 0;
-    HIP_OK(hipEventElapsedTime(&ms, pe0_, pe1_));
-    prof_add(name, ms * 1e-3, bytes);
+    HIP_OK(hipEventRecord(k.b, st_));
+    pend_.push_back(k);
+  }
+  void resolve_prof() {
+    if (pend_.empty()) return;
+    HIP_OK(hipStreamSynchronize(st_));
+    for (auto& k : pend_) {
+      float ms = 0;
+      HIP_OK(hipEventElapsedTime(&ms, k.a, k.b));
+      prof_add(k.name, ms * 1e-3, k.bytes);
+      evpool_.push_back(k.a);
+      evpool_.push_back(k.b);
+    }
+    pend_.clear();
   }
   void prof_add(const char* name, double seconds, double bytes) {
     KProf* k = nullptr;
@@ -4041,6 +4073,7 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
   if (prof_) prof_add("host_wait_mt_words", 1e-9 * (double)hs_.wait_ns.load(), 0);
   read_ctr();  // the stream is idle after this
   check_scatter();
+  resolve_prof();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   st->nvox_final = nvox_;
