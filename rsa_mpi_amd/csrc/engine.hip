@@ -853,6 +853,7 @@ struct DrawCtl {
   int swept;     // seq of the last sweep that ran (k_update_commit of that round runs)
   int nunres;    // candidates k_partner_lists left unresolved (one URec each)
   int acc_draw;  // spheres accepted so far in the current draw
+  int pubv[9];   // the round's publication (DrawPub fields), left by k_sweep_labels for k_update_commit
 };
 // One unresolved candidate as k_sweep_labels reads it, written by
 // k_partner_lists at the candidate's place in the unresolved list: index,
@@ -864,12 +865,15 @@ struct alignas(32) URec {
 };
 static_assert(sizeof(URec) == 32, "two 16-byte loads per record");
 
-// What k_sweep_labels publishes to pinned host memory (seq last).
+// A round's results in pinned host memory (seq last): computed by
+// k_sweep_labels, written to the host by the first thread of k_update_commit,
+// so that the PCIe writes overlap the tag update instead of ending the sweep.
 struct DrawPub {
   int acc, con, label_stuck, dropped, img_mismatch, pool_overflow, need_pool, cell_overflow, gate_open;
   int seq;
 };
 constexpr int kPubFields = 9;
+static_assert(sizeof(((DrawCtl*)nullptr)->pubv) == kPubFields * sizeof(int), "DrawCtl::pubv holds a DrawPub");
 // Publications go to slot seq % kPubSlots: a queued next draw may publish
 // before the host has read the current one.
 constexpr int kPubSlots = 8;
@@ -1382,11 +1386,11 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
 // partner gives Skip, none leaves it Undecided.  Partners have strictly smaller
 // priorities, so in the reference's sorted sweep they are labelled first; the
 // passes reach the same labels.  One workgroup, labels in LDS (nshots bytes of
-// dynamic shared memory).  The round's counts are then published to pinned
-// host memory.
+// dynamic shared memory).  The round's counts are left in DrawCtl::pubv, and
+// k_update_commit writes them to pinned host memory.
 __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
                                                 const int* __restrict__ slab, const int* __restrict__ pool,
-                                                DrawCtl* ctl, DrawPub* host, int seq, int gate_id, int draw_id,
+                                                DrawCtl* ctl, int seq, int gate_id, int draw_id,
                                                 double aimed, uint64_t miss_base, const URec* __restrict__ urec) {
   extern __shared__ __align__(16) uint8_t s_lab[];
   __shared__ int s_prog, s_left, s_acc, s_con, s_stuck;
@@ -1395,7 +1399,7 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
   const bool ovf = ctl->pool_overflow != 0;
   const int nu = ctl->nunres;
   const int tid = threadIdx.x;
-  // the counters published at the end: no kernel of this round changes them
+  // the counters the round publishes: no kernel of this round changes them
   // after k_partner_lists, so they are read now, with the loads above
   int pub_dropped = 0, pub_mism = 0, pub_povf = 0, pub_need = 0, pub_cell = 0, acc_before = 0;
   if (tid == 0) {
@@ -1564,7 +1568,7 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
   ctl->acc = s_acc;
   ctl->con = s_con;
   ctl->label_stuck = s_stuck;
-  // counters whose values are published below
+  // counters whose values go into the publication below
   ctl->dropped = 0;
   ctl->img_mismatch = 0;
   ctl->seg_cursor = 0;
@@ -1586,10 +1590,7 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
   ctl->gate = open ? draw_id + 1 : -1;
   ctl->swept = seq;
   const int v[kPubFields] = {s_acc, s_con, s_stuck, pub_dropped, pub_mism, pub_povf, pub_need, pub_cell, open};
-  DrawPub* hp = host + (seq & (kPubSlots - 1));
-  for (int k = 0; k < kPubFields; k++) __atomic_store_n(&reinterpret_cast<int*>(hp)[k], v[k], __ATOMIC_RELAXED);
-  __threadfence_system();
-  __atomic_store_n(&hp->seq, seq, __ATOMIC_RELEASE);
+  for (int k = 0; k < kPubFields; k++) ctl->pubv[k] = v[k];  // published by k_update_commit
 }
 
 // update_sphere_tags on the touched real cells (operator_algorithm.ixx:446-473),
@@ -1604,9 +1605,20 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
                                                       uint8_t* __restrict__ ctag, const uint8_t* __restrict__ lab,
                                                       const int* __restrict__ seg, int* __restrict__ len,
                                                       int* __restrict__ ord, int* __restrict__ cell2t,
-                                                      DrawCtl* __restrict__ ctl, int seq) {
+                                                      DrawCtl* __restrict__ ctl, int seq, DrawPub* host) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int swept = ctl->swept;  // all reads are issued before the tests
+  if (i == 0 && swept == seq) {
+    // the round's results to the host, as the sweep left them (seq last)
+    int v[kPubFields];
+#pragma unroll
+    for (int k = 0; k < kPubFields; k++) v[k] = ctl->pubv[k];
+    DrawPub* hp = host + (seq & (kPubSlots - 1));
+#pragma unroll
+    for (int k = 0; k < kPubFields; k++) __atomic_store_n(&reinterpret_cast<int*>(hp)[k], v[k], __ATOMIC_RELAXED);
+    __threadfence_system();
+    __atomic_store_n(&hp->seq, seq, __ATOMIC_RELEASE);
+  }
   const bool skip_round = ctl->pool_overflow || ctl->label_stuck;
   const bool commit = ctl->con == 0;
   const bool in = i < nshots;
@@ -3709,8 +3721,9 @@ void Engine<D>::wait_pub(int seq) {
 
 // generate_spheres + add_to_sample loop (operator_algorithm.ixx:534-582, 383-443)
 // for draws that cannot overshoot the class target: five kernels per round and
-// one host round trip (k_sweep_labels publishes the round's counts to pinned
-// host memory; the tag update and commit run while the host moves on).
+// one host round trip (the round's counts reach pinned host memory from the
+// first thread of k_update_commit; the tag update and commit run while the
+// host moves on).
 // enqueue_first queues a draw's first round (gated unless forced);
 // finish_draw waits for its rounds and queues the further ones.
 template <int D>
@@ -3738,11 +3751,12 @@ void Engine<D>::enqueue_round(const Prep& pr, int seq, int gate_id) {
   });
   timed("k_sweep_labels", (double)nshots * 4.0, [&] {
     hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(1024), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, slab_.p,
-                       pool_.p, dctl_.p, hpub_dev_, seq, gate_id, pr.id, aimed_, pr.miss_base, urec_.p);
+                       pool_.p, dctl_.p, seq, gate_id, pr.id, aimed_, pr.miss_base, urec_.p);
   });
   timed("k_update_commit", (double)nshots * (8.0 * (D + 1) + 12.0), [&] {
     hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
-                       ccell_.p, owner_.p, ctag_.p, lab_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, seq);
+                       ccell_.p, owner_.p, ctag_.p, lab_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, seq,
+                       hpub_dev_);
   });
   HIP_OK(hipGetLastError());
 }
