@@ -216,6 +216,20 @@ def test_small_4d_bit_exact(L, r, seed):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("L,r,seed", [(0.2, 0.05, 0), (0.3, 0.05, 3), (0.5, 0.05, 1), (0.56, 0.04, 2)])
+def test_dense_4d_conflicts_bit_exact(L, r, seed):
+    """4D boxes of 2 to 7 cells per side with 6000 candidates per draw: most candidates of the early draws
+    collide with each other, so the conflict sweep sees long partner lists over the 81-cell stencil, whose
+    order (stencil, then in-cell) decides the labels (operator_cells_and_data_storage.ixx:16-52, 127-142)."""
+    cls = [(r, 1.0, 0)]
+    g = gpu_run(4, L, cls, seed=seed)
+    o = orc_run(4, L, cls, seed=seed)
+    assert_same(g, o)
+    assert g["stats"]["words"] == o["words"]
+    assert g["stats"]["draws"] == len(o["trace"])
+
+
+@pytest.mark.gpu
 def test_cfg5b_4d_r002_matches_golden():
     """4D monodisperse r=0.02 in the unit hypercube (305 580 spheres, 1584 draws) against the oracle's
     fixture.  The reference's 4D corner table leaves component 0 at 0, so the voxel path is not
