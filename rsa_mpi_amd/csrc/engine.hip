@@ -853,7 +853,8 @@ struct DrawCtl {
   int swept;     // seq of the last sweep that ran (k_update_commit of that round runs)
   int nunres;    // candidates k_partner_lists left unresolved (one URec each)
   int acc_draw;  // spheres accepted so far in the current draw
-  int pubv[9];   // the round's publication (DrawPub fields), left by k_sweep_labels for k_update_commit
+  int long_list; // a cell got more than CSLOT candidates this draw (set by k_decode_filter)
+  int pubv[10];  // the round's publication (DrawPub fields), left by k_sweep_labels for k_update_commit
 };
 // One unresolved candidate as k_sweep_labels reads it, written by
 // k_partner_lists at the candidate's place in the unresolved list: index,
@@ -870,9 +871,10 @@ static_assert(sizeof(URec) == 32, "two 16-byte loads per record");
 // so that the PCIe writes overlap the tag update instead of ending the sweep.
 struct DrawPub {
   int acc, con, label_stuck, dropped, img_mismatch, pool_overflow, need_pool, cell_overflow, gate_open;
+  int need_lists;  // the first round found a cell of more than CSLOT candidates: build the lists, redo it
   int seq;
 };
-constexpr int kPubFields = 9;
+constexpr int kPubFields = 10;
 static_assert(sizeof(((DrawCtl*)nullptr)->pubv) == kPubFields * sizeof(int), "DrawCtl::pubv holds a DrawPub");
 // Publications go to slot seq % kPubSlots: a queued next draw may publish
 // before the host has read the current one.
@@ -1008,13 +1010,50 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
       ccell[i] = (int)cell;
       ctag[i] = anyhit ? T_SKIP : T_UNDECIDED;
       const int a = atomicAdd(&cellcc[cell], 1);
-      if (a < CSLOT)
+      if (a < CSLOT) {
         cslot[(size_t)cell * CSLOT + a] = i;
-      else
+      } else {
         next[i] = atomicExch(&ovhead[cell], i);
+        if (a == CSLOT) atomicOr(&ctl->long_list, 1);
+      }
       owner[i] = a == 0;
     }
   }
+}
+
+// The list of a cell of n <= CSLOT candidates as k_lists builds it: the arrival
+// slots v (n entries) in shot order (a sorting network), then the round-0 Skips
+// removed from the back with swap-with-last (rsa_data_storage.ixx:108-116,
+// 305-315).  The list is left in v; returns its length.  self: any valid
+// candidate index (the tag loads of unused entries read it, without a branch).
+__device__ __forceinline__ int short_list(int n, int (&v)[CSLOT], const uint8_t* __restrict__ ctag, int self) {
+#pragma unroll
+  for (int k = 0; k < CSLOT; k++)
+    if (k >= n) v[k] = INT_MAX;
+  // shot order: 19-comparator network for 8 keys (Batcher odd-even merge)
+  constexpr int NET[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                              {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
+#pragma unroll
+  for (int e = 0; e < 19; e++) {
+    const int x = v[NET[e][0]], y = v[NET[e][1]];
+    v[NET[e][0]] = min(x, y);
+    v[NET[e][1]] = max(x, y);
+  }
+  uint8_t tg[CSLOT];
+#pragma unroll
+  for (int k = 0; k < CSLOT; k++) tg[k] = ctag[k < n ? v[k] : self];
+  int m = n;
+#pragma unroll
+  for (int q = CSLOT - 1; q >= 0; q--) {
+    if (q < n && tg[q] == T_SKIP) {
+      int x = v[q];
+#pragma unroll
+      for (int k = q + 1; k < CSLOT; k++) x = (m - 1 == k) ? v[k] : x;
+      v[q] = x;
+      m--;
+    }
+  }
+  return m;
 }
 
 // One thread per touched cell (its owner candidate): the cell's candidates in
@@ -1046,43 +1085,15 @@ __global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict_
   const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
   const int4 r0 = make_int4((int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32));
   const int4 r1 = make_int4((int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32));
-  if (gate != draw_id || !on) return;
+  // draw_id < 0: a forced launch (the host's redo of a first round), no gate test
+  if ((draw_id >= 0 && gate != draw_id) || !on) return;
   // the owner's list: CSLOT entries at i * CSLOT, longer lists after all of those
   const int base = n <= CSLOT ? i * CSLOT : nshots * CSLOT + atomicAdd(&ctl->seg_cursor, n);
   int* a = ord + base;
   int m = n;
-  if (n <= CSLOT) {  // in registers: a sorting network, then the removal with selects
+  if (n <= CSLOT) {  // in registers
     int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
-#pragma unroll
-    for (int k = 0; k < CSLOT; k++)
-      if (k >= n) v[k] = INT_MAX;
-    // shot order: 19-comparator network for 8 keys (Batcher odd-even merge)
-    constexpr int NET[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
-                                {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
-#pragma unroll
-    for (int e = 0; e < 19; e++) {
-      const int x = v[NET[e][0]], y = v[NET[e][1]];
-      v[NET[e][0]] = min(x, y);
-      v[NET[e][1]] = max(x, y);
-    }
-    // the tags are loaded without a branch (entries past n read the owner's own tag)
-    uint8_t tg[CSLOT];
-#pragma unroll
-    for (int k = 0; k < CSLOT; k++) tg[k] = ctag[k < n ? v[k] : i];
-    bool sk[CSLOT];
-#pragma unroll
-    for (int k = 0; k < CSLOT; k++) sk[k] = k < n && tg[k] == T_SKIP;
-    // remove Skip from the back, the last entry taking the removed one's place
-#pragma unroll
-    for (int q = CSLOT - 1; q >= 0; q--) {
-      if (sk[q]) {
-        int x = v[q];
-#pragma unroll
-        for (int k = q + 1; k < CSLOT; k++) x = (m - 1 == k) ? v[k] : x;
-        v[q] = x;
-        m--;
-      }
-    }
+    m = short_list(n, v, ctag, i);
 #pragma unroll
     for (int k = 0; k < CSLOT; k++)
       if (k < m) a[k] = v[k];
@@ -1206,8 +1217,8 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
                                              double r, uint32_t pi, const double* cpos, const double* crad,
                                              const uint32_t* cprio, const uint8_t* ctag, const int* cell2t,
                                              const int* seg, const int* len, const int* ord, const int* pk,
-                                             int* out, bool& first_ghost, int* mismatch,
-                                             int (*keep)[PKEEP] = nullptr) {
+                                             const int* cellcc, const int* cslot, int self, int* out,
+                                             bool& first_ghost, int* mismatch, int (*keep)[PKEEP] = nullptr) {
   using LG = LaneGroup<D>;
   int n = 0;
   first_ghost = false;
@@ -1217,19 +1228,37 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
     if (s >= LG::NST) continue;
     Nb<D> nb;
     if (!neighbour_at<D>(g, cc, pk[jj], nb)) continue;
-    const int t = cell2t[nb.src];
-    if (t < 0) continue;
-    // A list of at most CSLOT entries starts at t * CSLOT (k_lists): that row is
-    // read with the list's offset and length (t < nshots keeps it in bounds) and
-    // used whenever the offset says the list starts there.
-    const int sg = load_now(seg + t);
-    const int ln = load_now(len + t);
-    const int64_t* row = reinterpret_cast<const int64_t*>(ord + (size_t)t * CSLOT);
-    const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
-    const int v[CSLOT] = {(int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32),
-                          (int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32)};
-    const bool in_row = sg == t * CSLOT;
-    const int* o = ord + sg;
+    int v[CSLOT];
+    int ln;
+    bool in_row;
+    const int* o;
+    if (cellcc != nullptr) {
+      // first round: the cell's list built here from its count and arrival slots,
+      // as k_lists would (the kernel gives up when some cell has more than CSLOT)
+      const int na = load_now(cellcc + nb.src);
+      const int64_t* row = reinterpret_cast<const int64_t*>(cslot + (size_t)nb.src * CSLOT);
+      const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
+      if (na == 0) continue;
+      v[0] = (int)q0; v[1] = (int)(q0 >> 32); v[2] = (int)q1; v[3] = (int)(q1 >> 32);
+      v[4] = (int)q2; v[5] = (int)(q2 >> 32); v[6] = (int)q3; v[7] = (int)(q3 >> 32);
+      ln = short_list(na < CSLOT ? na : CSLOT, v, ctag, self);
+      in_row = true;
+      o = ord;  // not read: the list is in v (ln <= CSLOT)
+    } else {
+      const int t = cell2t[nb.src];
+      if (t < 0) continue;
+      // A list of at most CSLOT entries starts at t * CSLOT (k_lists): that row is
+      // read with the list's offset and length (t < nshots keeps it in bounds) and
+      // used whenever the offset says the list starts there.
+      const int sg = load_now(seg + t);
+      ln = load_now(len + t);
+      const int64_t* row = reinterpret_cast<const int64_t*>(ord + (size_t)t * CSLOT);
+      const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
+      v[0] = (int)q0; v[1] = (int)(q0 >> 32); v[2] = (int)q1; v[3] = (int)(q1 >> 32);
+      v[4] = (int)q2; v[5] = (int)(q2 >> 32); v[6] = (int)q3; v[7] = (int)(q3 >> 32);
+      in_row = sg == t * CSLOT;
+      o = ord + sg;
+    }
     for (int k = 0; k < ln; k++) {
       int j;
       if (in_row && k < CSLOT) {
@@ -1287,9 +1316,14 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
                                                  const int* __restrict__ seg, const int* __restrict__ len,
                                                  const int* __restrict__ ord, int* __restrict__ slab,
                                                  int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int gate_id,
-                                                 URec* __restrict__ urec) {
+                                                 URec* __restrict__ urec, const int* __restrict__ cellcc,
+                                                 const int* __restrict__ cslot) {
   using LG = LaneGroup<D>;
-  const int gate = gate_id >= 0 ? ctl->gate : gate_id;  // tested before the first store (loads overlap)
+  // A draw's first round (gate_id >= 0) builds the cell lists from the arrival
+  // slots; the later ones read the lists k_update_commit kept.
+  const bool first = gate_id >= 0;
+  const int gate = first ? ctl->gate : gate_id;  // tested before the first store (loads overlap)
+  const int long_list = first ? ctl->long_list : 0;
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
@@ -1313,10 +1347,11 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
     r = crad[i];
     pi = cprio[i];
     cell_coords<D>(g, ccell[i], cc);
-    n = lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, pk, nullptr,
-                         fg, &mism, &keep);
+    n = lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, pk,
+                         first ? cellcc : nullptr, cslot, i, nullptr, fg, &mism, &keep);
   }
   if (gate != gate_id) return;  // whole grid
+  if (long_list) return;  // a cell of more than CSLOT candidates: the host builds the lists and redoes the round
   if (mism) atomicAdd(&ctl->img_mismatch, mism);
   int incl = n;  // inclusive scan over the lane group (stencil order)
 #pragma unroll
@@ -1372,8 +1407,8 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
       }
     } else if (n > PKEEP) {  // rare: walk the cells again, writing as it goes
       bool dummy;
-      lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, pk, dst + b,
-                       dummy, nullptr);
+      lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, pk,
+                       first ? cellcc : nullptr, cslot, i, dst + b, dummy, nullptr);
       for (int t = b; t < PKEEP; t++) head[t] = dst[t];
     }
   }
@@ -1397,6 +1432,10 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
   // everything below reads first and stores only once the gate is known open
   const int gate = gate_id >= 0 ? ctl->gate : gate_id;
   const bool ovf = ctl->pool_overflow != 0;
+  // a first round that met a cell of more than CSLOT candidates did nothing: the
+  // host builds the lists and redoes it (like a pool overflow)
+  const bool longl = gate_id >= 0 && ctl->long_list != 0;
+  const bool redo = ovf || longl;
   const int nu = ctl->nunres;
   const int tid = threadIdx.x;
   // the counters the round publishes: no kernel of this round changes them
@@ -1449,7 +1488,7 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
       h = rc[0];
       e = rc[1];
     }
-    const bool use = !ovf && u < nu;
+    const bool use = !redo && u < nu;
     cq[c] = use ? h.x : -1;
     cm[c] = use ? h.y : 0;
     cpl[c] = use ? (h.z == -1 ? slab + (size_t)h.x * KMAX : pool + h.z) : nullptr;
@@ -1485,7 +1524,7 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
     }
     return true;
   };
-  if (!ovf) {
+  if (!redo) {
     // with no unresolved candidate (nu is the same for the whole workgroup) no pass is needed
     for (int pass = 0; nu > 0 && pass < nshots + 2; pass++) {
       if (tid == 0) {
@@ -1580,16 +1619,17 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
   // proceed) in unsigned 64-bit arithmetic; miss_base carries the part of the
   // difference placed before the draw (non-zero on a proceed's first draw).
   int open = 0;
-  const int acc_draw = ovf ? acc_before : acc_before + s_acc;
+  const int acc_draw = redo ? acc_before : acc_before + s_acc;
   ctl->acc_draw = acc_draw;
-  if (!ovf && !s_stuck && s_con == 0) {
+  if (!redo && !s_stuck && s_con == 0) {
     const uint64_t miss = (uint64_t)nshots - (uint64_t)acc_draw - miss_base;
     const double rate = (double)miss / (0.000001 + (double)(uint64_t)nshots);
     open = !(rate > aimed);
   }
   ctl->gate = open ? draw_id + 1 : -1;
   ctl->swept = seq;
-  const int v[kPubFields] = {s_acc, s_con, s_stuck, pub_dropped, pub_mism, pub_povf, pub_need, pub_cell, open};
+  const int v[kPubFields] = {s_acc,    s_con,    s_stuck, pub_dropped, pub_mism,
+                             pub_povf, pub_need, pub_cell, open,        longl ? 1 : 0};
   for (int k = 0; k < kPubFields; k++) ctl->pubv[k] = v[k];  // published by k_update_commit
 }
 
@@ -1603,9 +1643,11 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
                                                       const uint32_t* __restrict__ cphase,
                                                       const int* __restrict__ ccell, const uint8_t* __restrict__ owner,
                                                       uint8_t* __restrict__ ctag, const uint8_t* __restrict__ lab,
-                                                      const int* __restrict__ seg, int* __restrict__ len,
+                                                      int* __restrict__ seg, int* __restrict__ len,
                                                       int* __restrict__ ord, int* __restrict__ cell2t,
-                                                      DrawCtl* __restrict__ ctl, int seq, DrawPub* host) {
+                                                      DrawCtl* __restrict__ ctl, int seq, DrawPub* host, int first,
+                                                      int* __restrict__ cellcc, const int* __restrict__ cslot,
+                                                      int ncells) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int swept = ctl->swept;  // all reads are issued before the tests
   if (i == 0 && swept == seq) {
@@ -1619,35 +1661,47 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
     __threadfence_system();
     __atomic_store_n(&hp->seq, seq, __ATOMIC_RELEASE);
   }
-  const bool skip_round = ctl->pool_overflow || ctl->label_stuck;
+  const bool skip_round = ctl->pool_overflow || ctl->label_stuck || (first && ctl->long_list);
   const bool commit = ctl->con == 0;
   const bool in = i < nshots;
   // Every thread also reads, without a branch (index clamped to candidate 0), its
-  // owner flag, its list's offset, length and cell, and the row at i * CSLOT where
-  // a list of at most CSLOT entries starts (k_lists), so that these loads are
-  // issued with the ones above (all in bounds whatever they hold).
+  // owner flag and cell, and its list: in a draw's first round the cell's count
+  // and arrival slots (cell clamped into the grid), later the list kept by the
+  // previous round (offset, length, and the row at i * CSLOT where a list of at
+  // most CSLOT entries starts).  The loads are issued with the ones above, all in
+  // bounds whatever they hold.
   const int ic = in ? i : 0;
   const uint8_t own = load_now(owner + ic);
   const bool on = in && own;
-  const int sg = load_now(seg + ic);
-  const int n0 = load_now(len + ic);
   const int c = load_now(ccell + ic);
-  const int64_t* row = reinterpret_cast<const int64_t*>(ord + (size_t)ic * CSLOT);
+  const int cl = (c >= 0 && c < ncells) ? c : 0;
+  const int sg = first ? ic * CSLOT : load_now(seg + ic);
+  const int n0 = first ? load_now(cellcc + cl) : load_now(len + ic);
+  const int64_t* row =
+      reinterpret_cast<const int64_t*>(first ? cslot + (size_t)cl * CSLOT : ord + (size_t)ic * CSLOT);
   const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
   const int4 r0 = make_int4((int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32));
   const int4 r1 = make_int4((int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32));
   if (swept != seq || !on || skip_round) return;  // its sweep did not run (closed gate), or no owner
   // an owner's cell is in the grid
   const int cnt0 = load_now(cs.cnt + c);
-  if (sg == i * CSLOT && n0 <= CSLOT) {
+  if (first || (sg == i * CSLOT && n0 <= CSLOT)) {
     // the list in registers: its tags are loaded together, the removal (same
     // steps as the loop below) works on the registers
     int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
-    // tags and labels without a branch (entries past n0 read the owner's own)
+    int nn = n0;
+    if (first) {
+      // the list k_lists would build (shot order, round-0 Skips removed), kept at
+      // i * CSLOT for the later rounds; the cell's arrivals restart at 0
+      nn = short_list(n0 < CSLOT ? n0 : CSLOT, v, ctag, i);
+      cellcc[c] = 0;
+      seg[i] = i * CSLOT;
+    }
+    // tags and labels without a branch (entries past the list read the owner's own)
     uint8_t tg[CSLOT], lb[CSLOT];
 #pragma unroll
     for (int k = 0; k < CSLOT; k++) {
-      const int j = k < n0 ? v[k] : i;
+      const int j = k < nn ? v[k] : i;
       tg[k] = ctag[j];
       lb[k] = lab[j];
     }
@@ -1655,13 +1709,13 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
     bool und[CSLOT];
 #pragma unroll
     for (int k = 0; k < CSLOT; k++) {
-      e[k] = k < n0 ? (tg[k] == T_UNDECIDED ? lb[k] : tg[k]) : T_DROPPED;
-      und[k] = k < n0 && tg[k] == T_UNDECIDED && lb[k] == T_UNDECIDED;
+      e[k] = k < nn ? (tg[k] == T_UNDECIDED ? lb[k] : tg[k]) : T_DROPPED;
+      und[k] = k < nn && tg[k] == T_UNDECIDED && lb[k] == T_UNDECIDED;
     }
-    int m = n0;
+    int m = nn;
 #pragma unroll
     for (int q = CSLOT - 1; q >= 0; q--) {
-      if (q < n0 && e[q] == T_SKIP) {
+      if (q < nn && e[q] == T_SKIP) {
         ctag[v[q]] = T_SKIP;
         int x = v[q];
         bool ux = und[q];
@@ -1685,7 +1739,10 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
         // Conflict candidates keep the Undecided tag for the next round
       }
     }
-    if (!commit) return;
+    if (!commit) {
+      if (first) cell2t[c] = i;  // the cell's list for the later rounds
+      return;
+    }
     Slot<D> sl[CSLOT];
     uint32_t ph[CSLOT];
 #pragma unroll
@@ -3747,7 +3804,7 @@ void Engine<D>::enqueue_round(const Prep& pr, int seq, int gate_id) {
   timed("k_partner_lists", (double)nshots * (8.0 * (D + 1) + 10.0), [&] {
     hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, cpos_.p, crad_.p, cprio_.p,
                        ccell_.p, ctag_.p, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, slab_.p, pool_.p,
-                       (int)pool_.cap, dctl_.p, gate_id, urec_.p);
+                       (int)pool_.cap, dctl_.p, gate_id, urec_.p, cellcc_.p, cslot_.p);
   });
   timed("k_sweep_labels", (double)nshots * 4.0, [&] {
     hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(1024), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, slab_.p,
@@ -3756,7 +3813,7 @@ void Engine<D>::enqueue_round(const Prep& pr, int seq, int gate_id) {
   timed("k_update_commit", (double)nshots * (8.0 * (D + 1) + 12.0), [&] {
     hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
                        ccell_.p, owner_.p, ctag_.p, lab_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, seq,
-                       hpub_dev_);
+                       hpub_dev_, gate_id >= 0 ? 1 : 0, cellcc_.p, cslot_.p, geo_.ncells);
   });
   HIP_OK(hipGetLastError());
 }
@@ -3786,10 +3843,8 @@ void Engine<D>::enqueue_first(Prep& pr, bool forced, uint64_t miss_base) {
                        crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p, owner_.p, cellcc_.p, cslot_.p, ovhead_.p,
                        next_.p, dctl_.p, pr.id, forced ? 1 : 0);
   });
-  timed("k_lists", (double)nshots * 24.0, [&] {
-    hipLaunchKernelGGL(k_lists, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, owner_.p, ctag_.p, cellcc_.p, cslot_.p,
-                       ovhead_.p, next_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, pr.id, geo_.ncells);
-  });
+  // the first round's partner and update kernels build the cell lists themselves
+  // (k_lists runs only when a cell got more than CSLOT candidates)
   enqueue_round(pr, pr.seq, pr.id);
   if (prof_)
     prof_add("host_enqueue_draw", std::chrono::duration<double>(std::chrono::steady_clock::now() - tq).count(), 0);
@@ -3808,6 +3863,16 @@ int64_t Engine<D>::finish_draw(const Prep& pr, int64_t& rounds, bool* next_live,
   int seq = pr.seq;
   int64_t local_new = 0;
   int r = 0;
+  // whether the cell lists of this draw are kept in seg/len/ord/cell2t: not after
+  // a first round that gave up (those lists are built by its update kernel)
+  bool lists_kept = false;
+  auto build_lists = [&]() {  // k_lists, forced: once per draw, before the cell counts are reset
+    timed("k_lists", (double)nshots * 24.0, [&] {
+      hipLaunchKernelGGL(k_lists, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, owner_.p, ctag_.p, cellcc_.p,
+                         cslot_.p, ovhead_.p, next_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, -1, geo_.ncells);
+    });
+    lists_kept = true;
+  };
   for (;;) {
     host_timed("host_wait_pub", [&] { wait_pub(seq); });
     const DrawPub h = hpub_[seq & (kPubSlots - 1)];
@@ -3815,11 +3880,21 @@ int64_t Engine<D>::finish_draw(const Prep& pr, int64_t& rounds, bool* next_live,
     if (h.cell_overflow) throw Error("internal: cell capacity exceeded");
     st->ghost_cell_drops += h.dropped;
     st->image_cell_mismatch += h.img_mismatch;
+    if (h.need_lists) {  // a cell got more than CSLOT candidates: build the lists, redo the first round
+      *next_live = false;
+      HIP_OK(hipMemsetAsync(&dctl_.p->long_list, 0, sizeof(int), st_));
+      build_lists();
+      if (prof_) prof_add("host_list_redo", 0, 0);
+      seq = ++seq_;
+      enqueue_round(pr, seq, -1);
+      continue;
+    }
     if (h.pool_overflow) {  // long partner lists: grow the pool and redo the round
       *next_live = false;
       HIP_OK(hipStreamSynchronize(st_));  // nothing queued may hold the old pool
       pool_.reserve((size_t)h.need_pool * 2);
       HIP_OK(hipMemsetAsync(&dctl_.p->pool_cursor, 0, 3 * sizeof(int), st_));
+      if (r == 0 && !lists_kept) build_lists();  // the first round's update kernel did not run
       seq = ++seq_;
       enqueue_round(pr, seq, -1);
       continue;
@@ -3831,6 +3906,7 @@ int64_t Engine<D>::finish_draw(const Prep& pr, int64_t& rounds, bool* next_live,
       break;
     }
     if (++r >= 100000) throw Error("too many conflict rounds in one draw");
+    lists_kept = true;  // kept by the round's update kernel
     *next_live = false;
     timed("k_refilter", (double)nshots * (8.0 * (D + 1) + 5.0), [&] {
       hipLaunchKernelGGL(k_refilter<D>, dim3(nbl), dim3(WG), 0, st_, geo_, cs(), nshots, cpos_.p, crad_.p, ccell_.p,
