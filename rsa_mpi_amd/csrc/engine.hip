@@ -28,6 +28,7 @@
 #include <chrono>
 #include <cstddef>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -897,7 +898,8 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
                                                       uint8_t* __restrict__ ctag, uint8_t* __restrict__ owner,
                                                       int* __restrict__ cellcc, int* __restrict__ cslot,
                                                       int* __restrict__ ovhead, int* __restrict__ next,
-                                                      DrawCtl* __restrict__ ctl, int draw_id, int forced) {
+                                                      DrawCtl* __restrict__ ctl, int draw_id, int forced,
+                                                      int list_thr) {
   using LG = LaneGroup<D>;
   // The gate does not change during this kernel.  It is read first but tested
   // only before the first store, so that its load overlaps the reads below
@@ -1010,12 +1012,13 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
       ccell[i] = (int)cell;
       ctag[i] = anyhit ? T_SKIP : T_UNDECIDED;
       const int a = atomicAdd(&cellcc[cell], 1);
-      if (a < CSLOT) {
+      if (a < CSLOT)
         cslot[(size_t)cell * CSLOT + a] = i;
-      } else {
+      else
         next[i] = atomicExch(&ovhead[cell], i);
-        if (a == CSLOT) atomicOr(&ctl->long_list, 1);
-      }
+      // more than list_thr (<= CSLOT) candidates in a cell: the first round's
+      // kernels give up and the host builds the lists (k_lists) and redoes it
+      if (a == list_thr) atomicOr(&ctl->long_list, 1);
       owner[i] = a == 0;
     }
   }
@@ -2534,6 +2537,10 @@ class Engine final : public IEngine {
     // first define_grid included), so engines run side by side
     HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&cst_, hipStreamNonBlocking));
+    // Cells of more than list_thr_ candidates send a draw's first round to the
+    // host-built lists.  RSA_GPU_LIST_THRESHOLD lowers it (1 .. CSLOT) so that tests
+    // run that path, which the usual workloads almost never reach.
+    if (const char* e = std::getenv("RSA_GPU_LIST_THRESHOLD")) list_thr_ = std::max(1, std::min(CSLOT, std::atoi(e)));
     try {
       define_grid(domain_rad);  // the domain's own grid (rsa_domain.ixx:26); replaced by algo_init
       for (int d = 1; d < D; d++)
@@ -2826,6 +2833,7 @@ class Engine final : public IEngine {
   hipStream_t cst_ = nullptr;  // copies of MT words into the device ring
   int* hctr_ = nullptr;
   DevBuf<int> dctr_;
+  int list_thr_ = CSLOT;  // see the constructor
 
   // radius generator (radius_generator.hxx:131-137)
   std::vector<std::tuple<double, uint64_t, int>> classes_;
@@ -3256,8 +3264,13 @@ class Engine final : public IEngine {
       wait = chunks_.front().ev;
       chunks_.pop_front();
     }
+    // The host waits for the copy (issued ahead, normally done already) instead of
+    // queueing a wait on st_: a stream-wait is a barrier packet that stalls its
+    // whole hardware queue, and with the streams of several engines sharing the 4
+    // hardware queues, two engines' barriers can each sit ahead of the marker the
+    // other waits for, so neither queue moves again.
     if (wait) {
-      HIP_OK(hipStreamWaitEvent(st_, wait, 0));
+      HIP_OK(hipEventSynchronize(wait));
       free_ev_.push_back(wait);
     }
   }
@@ -3841,7 +3854,7 @@ void Engine<D>::enqueue_first(Prep& pr, bool forced, uint64_t miss_base) {
     hipLaunchKernelGGL(kern, dim3(nbl), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N, hvpos_dev_[pr.buf],
                        pr.law ? hrad_dev_[pr.buf] : nullptr, pr.rej, pr.end_rel, vv, cls, geo_, cs(), cpos_.p,
                        crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p, owner_.p, cellcc_.p, cslot_.p, ovhead_.p,
-                       next_.p, dctl_.p, pr.id, forced ? 1 : 0);
+                       next_.p, dctl_.p, pr.id, forced ? 1 : 0, list_thr_);
   });
   // the first round's partner and update kernels build the cell lists themselves
   // (k_lists runs only when a cell got more than CSLOT candidates)

@@ -300,3 +300,27 @@ def test_domain_add_round_trip():
     st = algo.proceed(4)
     assert st["accepted"] == 0 and st["nvox_final"] == 0
     assert len(dom.extract_arrays()[1]) == len(g["rad"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,L,r,seed", [(2, 1.0, 0.01, 0), (3, 0.2, 0.005, 1), (4, 0.3, 0.05, 3), (4, 0.56, 0.04, 2)])
+def test_list_redo_path_bit_exact(monkeypatch, dim, L, r, seed):
+    """A draw's first round builds the cell lists inside its kernels; a cell of more candidates than the threshold
+    (8, lowered here to 1 through RSA_GPU_LIST_THRESHOLD) makes the host build them with k_lists and redo the round.
+    With the threshold at 1 most first rounds take that path: the profile counts the redos, and the packing, word
+    and draw counts are the oracle's bit for bit."""
+    monkeypatch.setenv("RSA_GPU_LIST_THRESHOLD", "1")
+    cls = [(r, 1.0, 0)]
+    dom = R.Domain([0.0] * dim, [L] * dim, r)
+    gen = R.RadiusGenerator(cls, dom.get_total_volume(), 0.0)
+    algo = R.Algo(dom, gen, 6000)
+    dom.profile(1)
+    st = algo.proceed(seed)
+    redo = dom.kernel_profile().get("host_list_redo", {}).get("launches", 0)
+    print("list redos", redo, "draws", st["draws"])
+    assert redo > 0
+    pos, rad, ph = dom.extract_arrays()
+    o = orc_run(dim, L, cls, seed=seed)
+    assert_same(dict(pos=pos, rad=rad, phase=ph, stats=st), o)
+    assert st["words"] == o["words"]
+    assert st["draws"] == len(o["trace"])
