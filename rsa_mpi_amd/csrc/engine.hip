@@ -792,10 +792,13 @@ __global__ void k_commit(CellStore<D> cs, int nshots, const double* __restrict__
 
 // ============================================================== kernels: one draw, one host round trip
 //
-// A draw is five kernels: decode + validated filter + cell binning
-// (k_decode_filter), per-cell ordered lists (k_lists), partner lists
-// (k_partner_lists), sweep labels + publication to the host (k_sweep_labels, one
-// workgroup), tag update + commit (k_update_commit).  Gathers over the 3^D
+// A draw is four kernels: decode + validated filter + cell binning
+// (k_decode_filter), partner lists (k_partner_lists, which builds each
+// neighbour cell's ordered list from its arrival slots in the first round),
+// sweep labels (k_sweep_labels, one workgroup), tag update + commit + the
+// round's publication to the host (k_update_commit, which keeps the cell lists
+// for later rounds).  k_lists builds the lists instead when a cell got more
+// than CSLOT candidates (the host redoes the round).  Gathers over the 3^D
 // stencil are spread over a lane group per candidate (one lane per stencil
 // cell) so the loads of all neighbour cells are in flight together.  No kernel
 // uses a device-wide fence or a counter shared by all lanes: touched cells are
@@ -3790,7 +3793,7 @@ void Engine<D>::wait_pub(int seq) {
 }
 
 // generate_spheres + add_to_sample loop (operator_algorithm.ixx:534-582, 383-443)
-// for draws that cannot overshoot the class target: five kernels per round and
+// for draws that cannot overshoot the class target: four kernels per round and
 // one host round trip (the round's counts reach pinned host memory from the
 // first thread of k_update_commit; the tag update and commit run while the
 // host moves on).
