@@ -237,16 +237,6 @@ __device__ __forceinline__ T load_now(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// A cell slot through load_now.
-template <int D>
-__device__ __forceinline__ Slot<D> slot_now(const Slot<D>* p) {
-  Slot<D> q;
-#pragma unroll
-  for (int d = 0; d < D; d++) q.c[d] = load_now(&p->c[d]);
-  q.r = load_now(&p->r);
-  return q;
-}
-
 // The packed stencil entries of a lane's CPL cells (lane `sub` takes entries
 // sub * CPL ..), read at kernel start: they depend on the lane only.
 template <int D, int CPL, int NST>
@@ -3850,7 +3840,6 @@ void Engine<D>::enqueue_first(Prep& pr, bool forced, uint64_t miss_base) {
   const ClassDev cls = class_dev();
   using LG = LaneGroup<D>;
   const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
-  const unsigned nbs = nblocks(nshots);
   timed("k_decode_filter", (double)nshots * (4.0 * (2 * D + 2) + 4.0 * D + 8.0 * (D + 1) + 13.0), [&] {
     const VoxView<D> vv = vview();
     auto kern = vv.wide ? k_decode_filter<D, true> : k_decode_filter<D, false>;
