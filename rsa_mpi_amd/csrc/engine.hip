@@ -1022,11 +1022,12 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
 // removed from the back with swap-with-last (rsa_data_storage.ixx:108-116,
 // 305-315).  The list is left in v; returns its length.  self: any valid
 // candidate index (the tag loads of unused entries read it, without a branch).
-__device__ __forceinline__ int short_list(int n, int (&v)[CSLOT], const uint8_t* __restrict__ ctag, int self) {
+// The first n arrival slots in shot order (entries n.. become INT_MAX):
+// 19-comparator network for 8 keys (Batcher odd-even merge).
+__device__ __forceinline__ void sort_slots(int n, int (&v)[CSLOT]) {
 #pragma unroll
   for (int k = 0; k < CSLOT; k++)
     if (k >= n) v[k] = INT_MAX;
-  // shot order: 19-comparator network for 8 keys (Batcher odd-even merge)
   constexpr int NET[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
                               {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
 #pragma unroll
@@ -1035,6 +1036,10 @@ __device__ __forceinline__ int short_list(int n, int (&v)[CSLOT], const uint8_t*
     v[NET[e][0]] = min(x, y);
     v[NET[e][1]] = max(x, y);
   }
+}
+
+__device__ __forceinline__ int short_list(int n, int (&v)[CSLOT], const uint8_t* __restrict__ ctag, int self) {
+  sort_slots(n, v);
   uint8_t tg[CSLOT];
 #pragma unroll
   for (int k = 0; k < CSLOT; k++) tg[k] = ctag[k < n ? v[k] : self];
@@ -1686,20 +1691,49 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
     // steps as the loop below) works on the registers
     int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
     int nn = n0;
-    if (first) {
-      // the list k_lists would build (shot order, round-0 Skips removed), kept at
-      // i * CSLOT for the later rounds; the cell's arrivals restart at 0
-      nn = short_list(n0 < CSLOT ? n0 : CSLOT, v, ctag, i);
-      cellcc[c] = 0;
-      seg[i] = i * CSLOT;
-    }
     // tags and labels without a branch (entries past the list read the owner's own)
     uint8_t tg[CSLOT], lb[CSLOT];
+    if (first) {
+      // the list k_lists would build (shot order, round-0 Skips removed from the
+      // back with swap-with-last), its tags and labels loaded once and moved with
+      // their entries; kept at i * CSLOT for the later rounds; the cell's
+      // arrivals restart at 0
+      const int na = n0 < CSLOT ? n0 : CSLOT;
+      sort_slots(na, v);
 #pragma unroll
-    for (int k = 0; k < CSLOT; k++) {
-      const int j = k < nn ? v[k] : i;
-      tg[k] = ctag[j];
-      lb[k] = lab[j];
+      for (int k = 0; k < CSLOT; k++) {
+        const int j = k < na ? v[k] : i;
+        tg[k] = ctag[j];
+        lb[k] = lab[j];
+      }
+      nn = na;
+#pragma unroll
+      for (int q = CSLOT - 1; q >= 0; q--) {
+        if (q < na && tg[q] == T_SKIP) {
+          int x = v[q];
+          uint8_t xt = tg[q], xl = lb[q];
+#pragma unroll
+          for (int k = q + 1; k < CSLOT; k++) {
+            const bool last = nn - 1 == k;
+            x = last ? v[k] : x;
+            xt = last ? tg[k] : xt;
+            xl = last ? lb[k] : xl;
+          }
+          v[q] = x;
+          tg[q] = xt;
+          lb[q] = xl;
+          nn--;
+        }
+      }
+      cellcc[c] = 0;
+      seg[i] = i * CSLOT;
+    } else {
+#pragma unroll
+      for (int k = 0; k < CSLOT; k++) {
+        const int j = k < nn ? v[k] : i;
+        tg[k] = ctag[j];
+        lb[k] = lab[j];
+      }
     }
     uint8_t e[CSLOT];
     bool und[CSLOT];
