@@ -88,17 +88,20 @@ class DevicePool {
     return *pool;
   }
   void* alloc(size_t bytes, size_t& got) {
+    void* p = nullptr;
     {
       std::lock_guard<std::mutex> lk(m_);
       auto it = free_.lower_bound(bytes);
       if (it != free_.end() && it->first / 2 <= bytes) {
         got = it->first;
-        void* p = it->second;
+        p = it->second;
         free_.erase(it);
-        return p;
       }
     }
-    void* p = nullptr;
+    if (p) {
+      poison(p, got);
+      return p;
+    }
     hipError_t e = hipMalloc(&p, bytes);
     if (e == hipErrorOutOfMemory) {
       (void)hipGetLastError();
@@ -107,7 +110,24 @@ class DevicePool {
     }
     if (e != hipSuccess) throw Error(std::string("HIP error ") + hipGetErrorString(e) + " at hipMalloc");
     got = bytes;
+    poison(p, got);
     return p;
+  }
+  // RSA_GPU_POISON=<byte>: every buffer handed out is first filled with that
+  // byte, so that a kernel reading memory nobody wrote in this engine reads the
+  // same garbage whatever ran there before (a test setting).
+  static int poison_byte() {
+    static const int v = [] {
+      const char* e = std::getenv("RSA_GPU_POISON");
+      return e ? (std::atoi(e) & 0xff) : -1;
+    }();
+    return v;
+  }
+  static void poison(void* p, size_t bytes) {
+    const int b = poison_byte();
+    if (b < 0) return;
+    HIP_OK(hipMemsetAsync(p, b, bytes, nullptr));
+    HIP_OK(hipStreamSynchronize(nullptr));
   }
   void release(void* p, size_t bytes) {
     if (!tl_rel.set) {
