@@ -243,6 +243,7 @@ enum Ctr : int {
   C_NCHILD,       // children kept by subdivide
   C_GROUPED,      // voxels binned by cell (k_cover's processing order)
   C_SCATTER_ERR,  // k_scatter found a workgroup's children inconsistent with the counting pass
+  C_COMPACT_ERR,  // k_compact found more stored spheres than the packed copy holds
   C_NUM
 };
 
@@ -1916,14 +1917,40 @@ __device__ __forceinline__ double far_term(double delta, double len) {
   }
 }
 
+// The placed spheres packed in cell order, for k_cover: off[c] = cnt[c] here,
+// an exclusive scan (with off[ncells] = total) next, then k_compact copies
+// cell c's spheres to off[c] .. off[c + 1] - 1.  The cells along dimension 0
+// are consecutive, so three neighbours in a row are one contiguous range.
+__global__ void __launch_bounds__(WG) k_cell_cnt(int64_t ncells, const uint8_t* __restrict__ cnt,
+                                                 int* __restrict__ off) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) off[c] = cnt[c];
+}
+
+template <int D>
+__global__ void __launch_bounds__(WG) k_compact(int64_t ncells, CellStore<D> cs, const int* __restrict__ off,
+                                                Slot<D>* __restrict__ out, int64_t cap, int* __restrict__ err) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int n = cs.cnt[c], o = off[c];
+  if ((int64_t)o + n > cap) {
+    atomicOr(err, 1);
+    return;
+  }
+  for (int k = 0; k < n; k++) out[o + k] = cs.sph[(size_t)c * cs.cap + k];
+}
+
 // remove_covered's coverage flag (list_of_voxels.ixx:30-40, 240-259) and
 // subdivide_uncovered's children test (:56-100, 261-307) in one pass:
 // out[i] = 0 if voxel i is covered, else 0x10000 | (bit k set when child k in
 // TabCorner order stays uncovered).  blk[b] counts uncovered voxels per block.
+// Interior voxels of D <= 3 read the packed copy (coff, csph) by stencil rows
+// when the host made one (coff not null); the others read the cell store.
 template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView<D> vx, int64_t nvox,
                                               VoxParams<D> vp, const uint32_t* __restrict__ perm,
-                                              uint32_t* __restrict__ out, int* __restrict__ blk) {
+                                              const int* __restrict__ coff, const Slot<D>* __restrict__ csph,
+                                              int ncs, uint32_t* __restrict__ out, int* __restrict__ blk) {
   using TC = TabCorner<D>;
   constexpr int NCH = ipow_c(2, D);
   constexpr int NST = ipow_c(3, D);
@@ -1931,35 +1958,37 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
   // thread t takes voxel perm[t] (the cell-grouped order) or, without perm, voxel t
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t i = perm == nullptr ? t : (t < nvox ? (int64_t)perm[t] : nvox);
-  // linear offsets of the stencil cells (any order: coverage is an OR)
-  constexpr int NSTL = NST <= 32 ? NST : 1;
-  __shared__ int s_off[NSTL];
-  __shared__ uint8_t s_last[NSTL];  // 1 where the next stencil cell is farther from the centre cell
+  // The stencil as NR rows of three cells along dimension 0 (any order:
+  // coverage is an OR): the linear offset of each row's first cell.
+  constexpr int NR = NST / 3;
+  constexpr int NRL = NST <= 32 ? NR : 1;
+  __shared__ int s_row[NRL];
+  __shared__ uint8_t s_last[NRL];  // 1 where the next row is farther from the centre row
   constexpr int KLIST = 16;  // listed spheres per lane before a flush
   __shared__ uint32_t s_list[KLIST][WG];
-  if (NST <= 32 && threadIdx.x < NSTL) {
-    // centre cell first, then by the number of non-zero offsets (faces, edges,
-    // corners): a covering sphere is met early and a covered voxel stops there
+  if (NST <= 32 && threadIdx.x < NRL) {
+    // centre row first, then by the number of non-zero offsets in dimensions
+    // 1 .. D-1: a covering sphere is met early and a covered voxel stops there
     auto nz = [](int r) {
       int n = 0;
-      for (int d = 0; d < D; d++, r /= 3) n += (r % 3 != 1);
+      for (int d = 1; d < D; d++, r /= 3) n += (r % 3 != 1);
       return n;
     };
     const int t = threadIdx.x, nt = nz(t);
     int pos = 0, same_after = 0;
-    for (int u = 0; u < NST; u++) {
+    for (int u = 0; u < NR; u++) {
       const int nu = nz(u);
       pos += (nu < nt) || (nu == nt && u < t);
       same_after += (nu == nt && u > t);
     }
-    int o = 0, r = t, stride = 1;
+    int o = -1, r = t, stride = g.n[0];
 #pragma unroll
-    for (int d = 0; d < D; d++) {
+    for (int d = 1; d < D; d++) {
       o += (r % 3 - 1) * stride;
       r /= 3;
       stride *= g.n[d];
     }
-    s_off[pos] = o;
+    s_row[pos] = o;
     s_last[pos] = same_after == 0;
   }
   __syncthreads();
@@ -2034,7 +2063,7 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
       bool interior = true;  // the whole stencil lies in real cells: plain index offsets
 #pragma unroll
       for (int d = 0; d < D; d++) interior = interior && cc[d] > g.gl && cc[d] < g.n[d] - g.gl - 1;
-      if (interior && NST <= 32) {
+      if (interior && NST <= 32 && coff != nullptr) {
         // Two phases against divergence: every sphere of the stencil gets the
         // cheap distance test and the ones either test can use are listed (per
         // lane, in LDS); the full tests run over the short lists at the end of
@@ -2045,24 +2074,22 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
         int nl = 0;
         auto flush = [&]() {
           for (int t = 0; t < nl && !covered; t++) {
-            const Slot<D> q = cs.sph[s_list[t][threadIdx.x]];
+            const Slot<D> q = csph[s_list[t][threadIdx.x]];
             test(q.c, q.r, do_cov, do_child);
           }
           nl = 0;
         };
-        // software pipeline: a cell's count and first slot (always allocated)
-        // are read together, one cell ahead of the tests
-        int src = (int)cell + s_off[0];
-        int nvn = cs.cnt[src];
-        Slot<D> q0n = cs.sph[(uint32_t)(src * cs.cap)];
-        for (int st = 0; st < NST && !covered; st++) {
-          const uint32_t base = (uint32_t)(src * cs.cap);
-          const int nv = nvn;
-          const Slot<D> q0 = q0n;
-          if (st + 1 < NST) {
-            src = (int)cell + s_off[st + 1];
-            nvn = cs.cnt[src];
-            q0n = cs.sph[(uint32_t)(src * cs.cap)];
+        // a row's spheres are csph[coff[s] .. coff[s + 3]); the bounds are read
+        // one row ahead of the tests (the stencil lies in real cells: s >= 0,
+        // s + 3 <= ncells), and kept within the ncs spheres the copy holds
+        int src = (int)cell + s_row[0];
+        int an = coff[src], bn = coff[src + 3];
+        for (int rw = 0; rw < NR && !covered; rw++) {
+          const int a = an, b = min(bn, ncs);
+          if (rw + 1 < NR) {
+            src = (int)cell + s_row[rw + 1];
+            an = coff[src];
+            bn = coff[src + 3];
           }
           auto consider = [&](const Slot<D>& q, uint32_t idx) {
             const double rr = q.r + vp.rmin;
@@ -2079,10 +2106,8 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
               s_list[nl++][threadIdx.x] = idx;
             }
           };
-          // the first slot came with the count; the others are plain loads
-          if (nv > 0) consider(q0, base);
-          for (int k = 1; k < nv; k++) consider(cs.sph[base + k], base + k);
-          if (s_last[st]) flush();  // end of a distance shell: test what was listed
+          for (int k = a; k < b; k++) consider(csph[k], (uint32_t)k);
+          if (s_last[rw]) flush();  // end of a distance shell: test what was listed
         }
         flush();
         continue;
@@ -2746,6 +2771,7 @@ class Engine final : public IEngine {
     e = hipStreamSynchronize(st_);
     PinnedPool::get().release(h);
     HIP_OK(e);
+    added_ += (uint64_t)n;
   }
 
   uint64_t count() const override { return nb_fast_; }
@@ -2898,6 +2924,7 @@ class Engine final : public IEngine {
   DevBuf<uint32_t> cells_ph_;
   DevBuf<int> cell2t_, cellcc_;
   uint64_t nb_fast_ = 0;
+  uint64_t added_ = 0;  // spheres given by add_spheres since the cell store was made
 
   // voxels
   DevBuf<int32_t> vc_[2][D];
@@ -2917,6 +2944,8 @@ class Engine final : public IEngine {
   // refine scratch
   DevBuf<uint32_t> code_, moved_, perm_;
   DevBuf<int> blk_, holes_, ccount_, vcell_, vrank_, tsum_;
+  DevBuf<int> coff_;         // packed copy of the cell store: offsets per cell (+ total)
+  DevBuf<Slot<D>> csph_;     // and the spheres in cell order
 
   // word stream: generated on the host into a pinned ring (the MT19937
   // recurrence is sequential), copied asynchronously into the device ring
@@ -3237,6 +3266,7 @@ class Engine final : public IEngine {
     HIP_OK(hipMemsetAsync(cell2t_.p, 0xff, (size_t)geo_.ncells * sizeof(int), st_));
     HIP_OK(hipMemsetAsync(cellcc_.p, 0, (size_t)geo_.ncells * sizeof(int), st_));
     nb_fast_ = 0;
+    added_ = 0;
   }
 
   uint64_t current_number() const { return std::get<1>(classes_[cur_]); }
@@ -3254,6 +3284,7 @@ class Engine final : public IEngine {
   // after read_ctr: a voxel list written inconsistently with its count is an error
   void check_scatter() const {
     if (hctr_[C_SCATTER_ERR]) throw Error("internal: voxel children disagree with their count (list not written)");
+    if (hctr_[C_COMPACT_ERR]) throw Error("internal: more stored spheres than placed ones (packed copy not written)");
   }
 
   // one chunk [dev_hi_, end) of generated words onto cst_ (split at the ring's end)
@@ -3475,13 +3506,32 @@ class Engine final : public IEngine {
       }, 1);
     }
     const uint32_t* perm = grouped ? perm_.p : nullptr;
+    // the placed spheres packed in cell order (k_cover's stencil rows, D <= 3),
+    // for long lists only: below kGroupMin voxels the copy costs more than it saves
+    const int* coff = nullptr;
+    const Slot<D>* csph = nullptr;
+    int ncs = 0;
+    if (ipow_c(3, D) <= 32 && nvox_ >= kGroupMin) {
+      const int64_t nc = geo_.ncells;
+      coff_.reserve((size_t)nc + 1);
+      csph_.reserve((size_t)(nb_fast_ + added_) + 1);
+      timed("k_compact", (double)nc * 9.0 + (double)(nb_fast_ + added_) * 2.0 * sizeof(Slot<D>), [&] {
+        hipLaunchKernelGGL(k_cell_cnt, dim3(nblocks(nc)), dim3(WG), 0, st_, nc, cells_cnt_.p, coff_.p);
+        scan_excl(coff_.p, nc, coff_.p + nc);
+        hipLaunchKernelGGL(k_compact<D>, dim3(nblocks(nc)), dim3(WG), 0, st_, nc, cs(), coff_.p, csph_.p,
+                           (int64_t)(nb_fast_ + added_), dctr_.p + C_COMPACT_ERR);
+      }, 1);
+      coff = coff_.p;
+      csph = csph_.p;
+      ncs = (int)std::min<uint64_t>(nb_fast_ + added_, (uint64_t)INT32_MAX);
+    }
     timed("k_cover", cover_bytes, [&] {
       if (wide_[vcur_])
-        hipLaunchKernelGGL((k_cover<D, true>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, perm, code_.p,
-                           blk_.p);
+        hipLaunchKernelGGL((k_cover<D, true>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, perm, coff,
+                           csph, ncs, code_.p, blk_.p);
       else
-        hipLaunchKernelGGL((k_cover<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, perm, code_.p,
-                           blk_.p);
+        hipLaunchKernelGGL((k_cover<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, perm, coff,
+                           csph, ncs, code_.p, blk_.p);
     }, 1);
     if (grouped) hipLaunchKernelGGL(k_block_count, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p);
     scan_excl(blk_.p, nb, dctr_.p + C_KEEP);
