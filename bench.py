@@ -82,7 +82,7 @@ def batched(R, workers):
     """The batched workload: 256 independent 3D realizations (box edge L ~ U[0.2, 1],
     r ~ U[0.004, 0.008], seeds 0..255, each to maximal packing), timed once with
     `workers` realizations at a time and once one after another (the plain form)."""
-    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "cfg4_batch.json")))
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "batch_3d_256.json")))
     real = fx["realizations"]
     L = [c["L"] for c in real]
     r = [c["r"] for c in real]
@@ -159,7 +159,7 @@ def main():
     del dom2
 
     parity = None
-    gold = os.path.join(ROOT, "tests", "golden", "cfg2_3d_r0.005_seed0.json")
+    gold = os.path.join(ROOT, "tests", "golden", "3d_r0.005_seed0.json")
     if os.path.exists(gold):
         from _common import set_hash
         g = json.load(open(gold))

@@ -1,4 +1,4 @@
-"""Writes tests/golden/cfg4_batch.json: the batched workload (256 independent 3D
+"""Writes tests/golden/batch_3d_256.json: the batched workload (256 independent 3D
 realizations, box edge L ~ U[0.2, 1.0], radius r ~ U[0.004, 0.008], seeds
 0..255, drawn once from numpy's PCG64 with meta-seed 20261203) and the CPU
 oracle's results
@@ -40,7 +40,7 @@ def main():
         print(checks[-1], flush=True)
     out = dict(meta_seed=META_SEED, generator="numpy.random.PCG64", dim=3, nbshots=6000,
                realizations=real, oracle=checks)
-    with open(os.path.join(HERE, "cfg4_batch.json"), "w") as f:
+    with open(os.path.join(HERE, "batch_3d_256.json"), "w") as f:
         json.dump(out, f, indent=1)
 
 

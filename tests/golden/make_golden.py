@@ -20,18 +20,18 @@ from _common import set_hash  # noqa: E402
 
 CASES = {
     # 3D monodisperse r=0.005 in the periodic unit cube, seed 0 (the bench workload)
-    "cfg2_3d_r0.005_seed0": dict(dim=3, L=1.0, classes=[(0.005, 1.0, 0)], seed=0),
+    "3d_r0.005_seed0": dict(dim=3, L=1.0, classes=[(0.005, 1.0, 0)], seed=0),
     # 3D monodisperse r=0.002 in the periodic unit cube, seed 0 (~11.5M spheres)
-    "cfg5a_3d_r0.002_seed0": dict(dim=3, L=1.0, classes=[(0.002, 1.0, 0)], seed=0),
+    "3d_r0.002_seed0": dict(dim=3, L=1.0, classes=[(0.002, 1.0, 0)], seed=0),
     # 3D polydisperse: classes r=0.01/0.005/0.0025 at volume fraction 0.10 each,
     # then fill with r=0.0025 (class order as given; equal radii keep their order)
     # 4D monodisperse r=0.02 in the periodic unit hypercube, seed 0 (the reference's 4D voxel
     # path keeps its TabCorner defect: the run ends when its voxel list empties)
-    "cfg5b_4d_r0.02_seed0": dict(dim=4, L=1.0, classes=[(0.02, 1.0, 0)], seed=0),
+    "4d_r0.02_seed0": dict(dim=4, L=1.0, classes=[(0.02, 1.0, 0)], seed=0),
     # test.hxx:104-109 test_bug_2D: throw_spheres<2, 1> in the unit square, r=0.0005 (voxel coordinates
     # past 32 bits in the reference's history)
     "bug2d_2d_r0.0005_seed0": dict(dim=2, L=1.0, classes=[(0.0005, 1.0, 0)], seed=0),
-    "cfg3_3d_poly_seed0": dict(dim=3, L=1.0, classes=[(0.01, 0.10, 0), (0.005, 0.10, 1), (0.0025, 0.10, 2),
+    "3d_poly_seed0": dict(dim=3, L=1.0, classes=[(0.01, 0.10, 0), (0.005, 0.10, 1), (0.0025, 0.10, 2),
                                                      (0.0025, 1.0, 3)], seed=0),
 }
 

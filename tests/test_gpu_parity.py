@@ -44,7 +44,7 @@ def assert_same(g, o):
 
 
 @pytest.mark.parametrize("seed", [0, 1])
-def test_cfg1_2d_bit_exact(seed):
+def test_2d_bit_exact(seed):
     """2D r=0.01 in the unit square to saturation (the reference's CPU case)."""
     cls = [(0.01, 1.0, 0)]
     g = gpu_run(2, 1.0, cls, seed=seed)
@@ -68,13 +68,13 @@ def test_small_3d_bit_exact(L, r, seed):
     assert g["stats"]["words"] == o["words"]
 
 
-def test_cfg2_full_size_matches_golden():
+def test_3d_r0005_full_size_matches_golden():
     """3D r=0.005 unit cube (~734k spheres): the whole packing against the
     oracle's committed fixture, plus size-independent properties."""
     import json
     import os
     from _common import periodic_overlaps, uncovered_samples, volume_fraction
-    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cfg2_3d_r0.005_seed0.json")))
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "3d_r0.005_seed0.json")))
     g = gpu_run(3, 1.0, [(0.005, 1.0, 0)], seed=0)
     st = g["stats"]
     print(st)
@@ -167,20 +167,20 @@ def _check_golden(g, gold, classes_phase=None):
             assert int((g["phase"] == int(ph)).sum()) == cnt
 
 
-def test_cfg3_polydisperse_matches_golden():
+def test_3d_polydisperse_matches_golden():
     """3D polydisperse: classes r=0.01/0.005/0.0025 at volume fraction 0.10
     each, then a fill class r=0.0025 (4.35M spheres; 1.15 billion voxels at the
     peak), against the oracle's committed fixture."""
-    gold = _golden("cfg3_3d_poly_seed0")
+    gold = _golden("3d_poly_seed0")
     cls = [tuple(c) for c in gold["case"]["classes"]]
     g = gpu_run(3, 1.0, cls, seed=0)
     _check_golden(g, gold)
 
 
-def test_cfg5a_r0002_matches_golden():
+def test_3d_r0002_matches_golden():
     """3D monodisperse r=0.002 in the unit cube (11.46M spheres, 12144 draws)."""
     from _common import uncovered_samples, volume_fraction
-    gold = _golden("cfg5a_3d_r0.002_seed0")
+    gold = _golden("3d_r0.002_seed0")
     g = gpu_run(3, 1.0, [(0.002, 1.0, 0)], seed=0)
     _check_golden(g, gold)
     assert uncovered_samples(g["pos"], g["rad"], [1.0] * 3, 0.002, n=100000) == 0
@@ -230,11 +230,11 @@ def test_dense_4d_conflicts_bit_exact(L, r, seed):
 
 
 @pytest.mark.gpu
-def test_cfg5b_4d_r002_matches_golden():
+def test_4d_r002_matches_golden():
     """4D monodisperse r=0.02 in the unit hypercube (305 580 spheres, 1584 draws) against the oracle's
     fixture.  The reference's 4D corner table leaves component 0 at 0, so the voxel path is not
     maximal (kept for parity); only the set and the trace totals are checked."""
-    gold = _golden("cfg5b_4d_r0.02_seed0")
+    gold = _golden("4d_r0.02_seed0")
     g = gpu_run(4, 1.0, [(0.02, 1.0, 0)], seed=0)
     _check_golden(g, gold)
 

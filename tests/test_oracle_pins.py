@@ -201,7 +201,7 @@ def test_remove_covered_matches_literal_loop():
 
 # ---------------------------------------------------------------- whole runs
 
-def test_cfg1_saturation_properties():
+def test_2d_saturation_properties():
     """2D r=0.01, unit square: maximal, no overlap, coverage ~0.547."""
     o = O.run(2, [0, 0], [1, 1], [(0.01, 1.0, 0)], seed=0)
     assert o["msg"].endswith("fully packed")
