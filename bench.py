@@ -2,12 +2,15 @@
 """Headline benchmark: accepted spheres/s to maximal packing on one MI355X.
 
 A step is one proceed(seed) of the reference's voxel method (method 1) from an
-empty periodic box to a maximal packing: 3D monodisperse r=0.005 in the unit
-cube, seed 0, nbshots_singledraw 6000 (~734k spheres).  Inputs are synthetic
-and generated on the device from the seed; the timed region is proceed() only
-(domain and buffers are set up before it).  Prints one JSON line.
+empty periodic box to a maximal packing: 3D monodisperse r=0.002 in the unit
+cube, seed 0, nbshots_singledraw 6000 (11 463 041 spheres, 772 M voxels at the
+peak).  Inputs are synthetic and generated from the seed; the timed region is
+proceed() only (domain and buffers are set up before it).  The placed set of
+the last timed step is checked by SHA-256 against the oracle's fixture.  A
+secondary line runs the r=0.005 cube the same way.  Prints one JSON line.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -22,11 +25,48 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 METRIC = "accepted spheres/sec to maximal packing (3D, 1 MI355X) learned vs plain; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 
-WORKLOAD = dict(dim=3, L=1.0, r=0.005, seed=0, nbshots=6000)
+HEADLINE = dict(name="3D monodisperse RSA, r=0.002, periodic unit cube, seed 0, voxel method to maximal packing",
+                dim=3, L=1.0, r=0.002, seed=0, nbshots=6000, golden="3d_r0.002_seed0.json",
+                profile="k_cover_3d_r0.002.md")
+SECONDARY = dict(name="3D monodisperse RSA, r=0.005, periodic unit cube, seed 0, voxel method to maximal packing",
+                 dim=3, L=1.0, r=0.005, seed=0, nbshots=6000, golden="3d_r0.005_seed0.json")
 
 
-def run_gpu_step(R, profile_level=1):
-    w = WORKLOAD
+def source_sha16():
+    """SHA-256 (first 16 hex digits) of the HIP library's sources: rocprofv3 figures in
+    profiles/ are used only when they were measured on this exact source."""
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "rsa_mpi_amd", "csrc")
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".hpp", ".cpp", ".h")):
+            h.update(f.encode())
+            h.update(open(os.path.join(d, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
+def profile_figures(name):
+    """`key: value` lines of a rocprofv3 summary in profiles/, or {} when the summary is
+    missing or was measured on other sources than the ones built here."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return {}
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith(("|", "#", " ", "`")):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    if vals.get("source_sha16") != source_sha16():
+        return {}
+    out = {}
+    for k, v in vals.items():
+        try:
+            out[k] = float(v)
+        except ValueError:
+            pass
+    return out
+
+
+def run_gpu_step(R, w, profile_level=1):
     dom = R.Domain([0.0] * w["dim"], [w["L"]] * w["dim"], w["r"])
     gen = R.RadiusGenerator([(w["r"], 1.0, 0)], dom.get_total_volume(), 0.0)
     algo = R.Algo(dom, gen, w["nbshots"])
@@ -38,68 +78,127 @@ def run_gpu_step(R, profile_level=1):
     return dom, st, dt
 
 
-def profile_figure(key):
-    """A `key: value` line of profiles/k_cover_3d_r0.005.md (rocprofv3 PMC summary), or None."""
-    path = os.path.join(ROOT, "profiles", "k_cover_3d_r0.005.md")
-    if not os.path.exists(path):
+def parity(dom, w):
+    from _common import set_hash
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", w["golden"])))
+    pos, rad, _ = dom.extract_arrays()
+    ok = len(rad) == g["n"] and set_hash(pos, rad) == g["sha256"]
+    return "sha256 match" if ok else "MISMATCH"
+
+
+def measure(R, w, steps, warmup):
+    """warmup untimed steps, then `steps` timed ones; per-kernel event profile (refine
+    kernels) of the timed steps and the parity of the last one."""
+    for _ in range(warmup):
+        run_gpu_step(R, w, profile_level=0)
+    total_n, total_t, prof, stats, dom = 0, 0.0, {}, None, None
+    for s in range(steps):
+        dom, st, dt = run_gpu_step(R, w, profile_level=1)
+        print(f"#   step {s}: {st['accepted']} spheres in {dt:.3f} s", file=sys.stderr, flush=True)
+        total_n += st["accepted"]
+        total_t += dt
+        stats = st
+        for k, v in dom.kernel_profile().items():
+            a = prof.setdefault(k, dict(launches=0, seconds=0.0, bytes=0.0))
+            for f in a:
+                a[f] += v[f]
+    par = parity(dom, w)
+    del dom
+    return dict(value=total_n / total_t, ms=1e3 * total_t / steps, prof=prof, stats=stats, parity=par)
+
+
+def phases(stats):
+    draw_s = stats["seconds"] - stats["voxel_seconds"]
+    return {"voxel_refine": round(1e3 * stats["voxel_seconds"], 2), "draws": round(1e3 * draw_s, 2),
+            "rounds": stats["rounds"], "rounds_max": stats["rounds_max"],
+            # the dart phase is latency-bound: its rate per draw
+            "us_per_draw": round(1e6 * draw_s / stats["draws"], 2),
+            "draws_per_s": round(stats["draws"] / draw_s, 1)}
+
+
+def per_kernel(prof, steps):
+    """per kernel (event-timed on the engine's stream); GB/s = algorithmic bytes over kernel time"""
+    return {kk: dict(launches=v["launches"] // steps, ms=round(1e3 * v["seconds"] / steps, 3),
+                     **({"GBps": round(v["bytes"] / v["seconds"] / 1e9, 1)} if v["bytes"] > 0 else {}))
+            for kk, v in prof.items()}
+
+
+def roofline(prof, steps, figures):
+    """k_cover (the dominant refine kernel): algorithmic bytes per launch over its
+    event-timed average launch duration, against the 8 TB/s HBM peak; traffic and the
+    FP64 VALU issue fraction from the rocprofv3 summary of the same sources."""
+    k = prof.get("k_cover")
+    if not k or k["seconds"] <= 0:
         return None
-    for line in open(path):
-        if line.startswith(key + ":"):
-            return float(line.split(":", 1)[1])
-    return None
+    achieved = k["bytes"] / k["seconds"] / 1e9
+    out = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+               frac=round(achieved / HBM_PEAK_GBS, 4), traffic=figures.get("traffic_bytes_per_launch"),
+               kernel="k_cover", launches=k["launches"] // steps,
+               avg_launch_ms=round(1e3 * k["seconds"] / k["launches"], 3),
+               algorithmic_bytes_per_launch=round(k["bytes"] / k["launches"]))
+    # FP64 VALU issue: SQ_INSTS_VALU wave-instructions at 4 cycles per wave64 instruction on
+    # 1024 SIMDs at 2.4 GHz, over the measured kernel time
+    valu = figures.get("valu_wave_instructions_per_run")
+    if valu:
+        busy_s = valu * 4 / (1024 * 2.4e9)
+        out["valu"] = dict(wave_instructions_per_run=valu, issue_seconds_per_run=round(busy_s, 6),
+                           busy_frac=round(busy_s / (k["seconds"] / steps), 3))
+    return out
 
 
-def measured_traffic():
-    """HBM bytes per k_cover launch on this workload, from the rocprofv3 FETCH_SIZE and
-    WRITE_SIZE passes summarised in profiles/ (each counter scaled by what it reads for
-    a plain copy of known size); None when no summary is present."""
-    path = os.path.join(ROOT, "profiles", "k_cover_3d_r0.005.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+def host_cores():
+    """The host cores this process may run on (the box allots a share of its CPUs per GPU;
+    OMP_NUM_THREADS, when set, is that share)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = min(aff, int(env)) if env and env.isdigit() and int(env) > 0 else aff
+    return n, aff
 
 
-def cpu_baseline(side=0.8):
-    """The CPU oracle (restatement of the reference's single-rank path) on all
-    host cores, on a bounded sample of the same workload: the same radius and
-    seed in a periodic cube of side `side`, run to maximal packing."""
+def cpu_baseline(side):
+    """The CPU oracle (restatement of the reference's single-rank path, OpenMP loops as in
+    the reference) on every core of this process's CPU share, on a bounded sample of the
+    headline workload: the same radius and seed in a periodic cube of side `side`, run to
+    maximal packing."""
     import _oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, aff = host_cores()
     os.environ["OMP_NUM_THREADS"] = str(threads)
     O.build()
+    w = HEADLINE
     t0 = time.perf_counter()
-    o = O.run(3, [0.0] * 3, [side] * 3, [(WORKLOAD["r"], 1.0, 0)], seed=WORKLOAD["seed"])
+    o = O.run(3, [0.0] * 3, [side] * 3, [(w["r"], 1.0, 0)], seed=w["seed"])
     dt = time.perf_counter() - t0
     return dict(value=o["n"] / dt, unit="spheres/s", cores=threads, kind="port",
-                sample=f"3D r={WORKLOAD['r']} seed {WORKLOAD['seed']} in the periodic cube of side {side} "
-                       f"({side ** 3:.0%} of the workload volume) to maximal packing: {o['n']} spheres in {dt:.2f} s")
+                host_cpus=os.cpu_count(), affinity_cpus=aff,
+                sample=f"3D r={w['r']} seed {w['seed']} in the periodic cube of side {side} "
+                       f"({side ** 3:.1%} of the workload volume) to maximal packing: {o['n']} spheres in {dt:.2f} s")
 
 
 def batched(R, workers):
     """The batched workload: 256 independent 3D realizations (box edge L ~ U[0.2, 1],
-    r ~ U[0.004, 0.008], seeds 0..255, each to maximal packing), timed once with
-    `workers` realizations at a time and once one after another (the plain form)."""
+    r ~ U[0.004, 0.008], seeds 0..255, each to maximal packing), timed once `workers`
+    realizations at a time and once one after another; every placed set (extracted
+    inside the timed region) is checked against the oracle's SHA-256."""
+    from _common import set_hash
     fx = json.load(open(os.path.join(ROOT, "tests", "golden", "batch_3d_256.json")))
     real = fx["realizations"]
     L = [c["L"] for c in real]
     r = [c["r"] for c in real]
     s = [c["seed"] for c in real]
     R.batch_run(3, L[:8], r[:8], s[:8], workers=workers)  # warm-up: contexts, pools, threads
-    want = np.array([c["n"] for c in fx["oracle"]], dtype=np.uint64)
     res = {}
     for name, w in (("batched", workers), ("plain", 1)):
         t0 = time.perf_counter()
-        out = R.batch_run(3, L, r, s, workers=w)
+        out = R.batch_run(3, L, r, s, workers=w, extract=True)
         dt = time.perf_counter() - t0
+        bad = [k for k, (p, rr) in enumerate(out["spheres"])
+               if len(rr) != fx["oracle"][k]["n"] or set_hash(p, rr) != fx["oracle"][k]["sha256"]]
         res[name] = dict(seconds=round(dt, 3), value=round(float(out["counts"].sum()) / dt, 1),
-                         counts_match_oracle=bool(np.array_equal(out["counts"], want)))
+                         sets_match_oracle=f"{256 - len(bad)}/256", mismatched=bad[:8])
     return {"workload": "256 independent 3D monodisperse realizations, periodic box [0,L]^3, L~U[0.2,1], "
-                        "r~U[0.004,0.008], seeds 0..255, each to maximal packing",
-            "realizations": len(real), "spheres": int(want.sum()), "workers": workers, "unit": "spheres/s",
-            "value": res["batched"]["value"], "plain_value": res["plain"]["value"],
+                        "r~U[0.004,0.008], seeds 0..255, each to maximal packing (extraction in the timed region)",
+            "realizations": len(real), "spheres": int(sum(c["n"] for c in fx["oracle"])), "workers": workers,
+            "unit": "spheres/s", "value": res["batched"]["value"], "plain_value": res["plain"]["value"],
             "speedup_vs_plain": round(res["batched"]["value"] / res["plain"]["value"], 3), **{
                 k + "_" + kk: v for k, d in res.items() for kk, v in d.items() if kk != "value"}}
 
@@ -110,10 +209,13 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-side", type=float, default=0.8)
+    ap.add_argument("--cpu-side", type=float, default=0.4)
     ap.add_argument("--batch-workers", type=int, default=6,
                     help="realizations run at once in the batched measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the r=0.005 line")
+    ap.add_argument("--only", choices=["headline", "secondary"], default=None,
+                    help="run one workload's timed steps and nothing else (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (reference reads for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -134,78 +236,50 @@ def main():
 
     import rsa_mpi_amd as R
     R.lib()
-    for _ in range(args.warmup):
-        run_gpu_step(R, profile_level=0)
+    if args.only:
+        w = HEADLINE if args.only == "headline" else SECONDARY
+        m = measure(R, w, args.steps, args.warmup)
+        print(json.dumps({"workload": w["name"], "value": round(m["value"], 1), "ms_per_step": round(m["ms"], 2),
+                          "parity_vs_oracle": m["parity"], "profile": per_kernel(m["prof"], args.steps)}),
+              flush=True)
+        return
 
-    total_n, total_t = 0, 0.0
-    prof = {}
-    stats = None
-    for _ in range(args.steps):
-        dom, st, dt = run_gpu_step(R, profile_level=1)
-        total_n += st["accepted"]
-        total_t += dt
-        stats = st
-        for k, v in dom.kernel_profile().items():
-            a = prof.setdefault(k, dict(launches=0, seconds=0.0, bytes=0.0))
-            for f in a:
-                a[f] += v[f]
-    value = total_n / total_t
-
-    # one extra, untimed step with every per-draw kernel bracketed by events
-    dom2, _, dt2 = run_gpu_step(R, profile_level=2)
-    detail = {kk: dict(launches=v["launches"], ms=round(1e3 * v["seconds"], 3),
-                       us_per_launch=round(1e6 * v["seconds"] / max(1, v["launches"]), 2))
-              for kk, v in dom2.kernel_profile().items()}
-    del dom2
-
-    parity = None
-    gold = os.path.join(ROOT, "tests", "golden", "3d_r0.005_seed0.json")
-    if os.path.exists(gold):
-        from _common import set_hash
-        g = json.load(open(gold))
-        pos, rad, _ = dom.extract_arrays()
-        parity = "sha256 match" if set_hash(pos, rad) == g["sha256"] and len(rad) == g["n"] else "MISMATCH"
-
-    k = prof.get("k_cover")
-    roofline = None
-    if k and k["seconds"] > 0:
-        achieved = k["bytes"] / k["seconds"] / 1e9
-        roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(),
-                        kernel="k_cover", launches=k["launches"] // args.steps,
-                        avg_launch_ms=round(1e3 * k["seconds"] / k["launches"], 3),
-                        algorithmic_bytes_per_launch=round(k["bytes"] / k["launches"]))
-        # k_cover issues FP64 VALU work: its VALU instruction count (SQ_INSTS_VALU, from the
-        # committed PMC summary) at 4 cycles per wave64 instruction on 1024 SIMDs at 2.4 GHz,
-        # over the measured kernel time
-        valu = profile_figure("valu_wave_instructions_per_run")
-        if valu:
-            busy_s = valu * 4 / (1024 * 2.4e9)
-            roofline["valu"] = dict(wave_instructions_per_run=valu, issue_seconds_per_run=round(busy_s, 6),
-                                    busy_frac=round(busy_s / (k["seconds"] / args.steps), 3))
+    print(f"# headline: {args.warmup} warm-up + {args.steps} timed steps", file=sys.stderr, flush=True)
+    head = measure(R, HEADLINE, args.steps, args.warmup)
+    stats = head["stats"]
+    figures = profile_figures(HEADLINE["profile"])
+    sec = None
+    if not args.no_secondary:
+        print("# secondary", file=sys.stderr, flush=True)
+        m = measure(R, SECONDARY, args.steps, args.warmup)
+        sec = {"workload": SECONDARY["name"], "value": round(m["value"], 1), "unit": "spheres/s",
+               "ms_per_step": round(m["ms"], 2), "spheres": m["stats"]["accepted"], "draws": m["stats"]["draws"],
+               "refines": m["stats"]["refines"], "parity_vs_oracle": m["parity"], "phases_ms": phases(m["stats"]),
+               "roofline_k_cover_frac": (roofline(m["prof"], args.steps, {}) or {}).get("frac"),
+               "profile": per_kernel(m["prof"], args.steps)}
+    bat = None
+    if not args.no_batch:
+        print("# batched", file=sys.stderr, flush=True)
+        bat = batched(R, args.batch_workers)
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_side)
 
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "spheres/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * total_t / args.steps, 2), "higher_is_better": True,
+        "metric": METRIC, "value": round(head["value"], 1), "unit": "spheres/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(head["ms"], 2), "higher_is_better": True,
         "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "3D monodisperse RSA, r=0.005, periodic unit cube, seed 0, voxel method to "
-                               "maximal packing (plain GPU form)", "spheres": stats["accepted"],
+        "config": {"workload": HEADLINE["name"] + " (plain GPU form)", "spheres": stats["accepted"],
                    "draws": stats["draws"], "refines": stats["refines"], "nvox_max": stats["nvox_max"],
-                   "mt_words": stats["words"], "parity_vs_oracle": parity},
-        "roofline": roofline,
-        "phases_ms": {"voxel_refine": round(1e3 * stats["voxel_seconds"], 2),
-                      "draws": round(1e3 * (stats["seconds"] - stats["voxel_seconds"]), 2),
-                      "rounds": stats["rounds"], "rounds_max": stats["rounds_max"],
-                      # the dart phase is latency-bound: its rate per draw
-                      "us_per_draw": round(1e6 * (stats["seconds"] - stats["voxel_seconds"]) / stats["draws"], 2),
-                      "draws_per_s": round(stats["draws"] / (stats["seconds"] - stats["voxel_seconds"]), 1)},
-        # per kernel (event-timed); GB/s = algorithmic bytes over kernel time where bytes are counted
-        "profile": {kk: dict(launches=v["launches"] // args.steps, ms=round(1e3 * v["seconds"] / args.steps, 3),
-                             **({"GBps": round(v["bytes"] / v["seconds"] / 1e9, 1)} if v["bytes"] > 0 else {}))
-                    for kk, v in prof.items()},
-        "kernels_all_bracketed": dict(step_ms=round(1e3 * dt2, 1), **detail),
-        "batched": None if args.no_batch else batched(R, args.batch_workers),
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_side),
+                   "mt_words": stats["words"], "parity_vs_oracle": head["parity"]},
+        "roofline": roofline(head["prof"], args.steps, figures),
+        "source_sha16": source_sha16(),
+        "phases_ms": phases(stats),
+        "profile": per_kernel(head["prof"], args.steps),
+        "secondary": sec,
+        "batched": bat,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -244,6 +244,7 @@ enum Ctr : int {
   C_GROUPED,      // voxels binned by cell (k_cover's processing order)
   C_SCATTER_ERR,  // k_scatter found a workgroup's children inconsistent with the counting pass
   C_COMPACT_ERR,  // k_compact found more stored spheres than the packed copy holds
+  C_MOVED,        // voxels k_move moved into holes (the compaction's traffic)
   C_NUM
 };
 
@@ -2302,8 +2303,9 @@ __device__ __forceinline__ int block_rank(bool f, int blkoff) {
 // closed form: with keep uncovered voxels, the covered positions below keep,
 // in increasing order, receive the uncovered voxels at or above keep in
 // decreasing order.  Step 1 lists the holes.
+// The voxel at keep - 1 also leaves the number of holes (= voxels step 2 moves).
 __global__ void __launch_bounds__(WG) k_holes(int64_t nvox, const uint32_t* __restrict__ code,
-                                              const int* __restrict__ blk, const int* __restrict__ ctr,
+                                              const int* __restrict__ blk, int* __restrict__ ctr,
                                               int* __restrict__ holes) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < nvox;
@@ -2311,6 +2313,7 @@ __global__ void __launch_bounds__(WG) k_holes(int64_t nvox, const uint32_t* __re
   const int U = block_rank(unc, blk[blockIdx.x]);
   const int keep = ctr[C_KEEP];
   if (valid && !unc && i < keep) holes[i - U] = (int)i;
+  if (valid && i == (int64_t)keep - 1) ctr[C_MOVED] = keep - U - (unc ? 1 : 0);
 }
 
 // Step 2: the uncovered voxels at or above keep move into the holes.  Their
@@ -3535,15 +3538,30 @@ class Engine final : public IEngine {
     }, 1);
     if (grouped) hipLaunchKernelGGL(k_block_count, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p);
     scan_excl(blk_.p, nb, dctr_.p + C_KEEP);
-    hipLaunchKernelGGL(k_holes, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
-    hipLaunchKernelGGL(k_move<D>, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, moved_.p, blk_.p, dctr_.p, holes_.p,
-                       vx);
+    HIP_OK(hipMemsetAsync(dctr_.p + C_MOVED, 0, sizeof(int), st_));
+    // the compaction's bytes depend on the number of moved voxels (left by k_holes):
+    // filled in below, once the counters are read
+    const size_t holes_at = pend_.size();
+    timed("k_holes", 0., [&] {
+      hipLaunchKernelGGL(k_holes, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
+    }, 1);
+    timed("k_move", 0., [&] {
+      hipLaunchKernelGGL(k_move<D>, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, moved_.p, blk_.p, dctr_.p,
+                         holes_.p, vx);
+    }, 1);
     // children per block of the kept prefix, counted against the device's keep
     // (blocks past it count 0), so that one host round trip returns both totals
     hipLaunchKernelGGL(k_child_count, dim3(nb), dim3(WG), 0, st_, nvox_, dctr_.p, code_.p, moved_.p, blk_.p);
     scan_excl(blk_.p, nb, dctr_.p + C_NCHILD);
     read_ctr();
     check_scatter();  // the previous refinement's children
+    if (pend_.size() == holes_at + 2) {
+      // k_holes: codes and block offsets read, one index per hole written; k_move: the
+      // same reads, then per moved voxel its coordinates read and written and its code
+      const double cw = wide_[vcur_] ? 8.0 : 4.0, m = (double)hctr_[C_MOVED];
+      pend_[holes_at].bytes = (double)nvox_ * 4.0 + nb * 4.0 + m * 4.0;
+      pend_[holes_at + 1].bytes = (double)nvox_ * 4.0 + nb * 4.0 + m * (2.0 * cw * D + 4.0 + 4.0);
+    }
     const int64_t keep = hctr_[C_KEEP];
     if (keep == 0) {  // subdivide_uncovered returns early on an empty list (list_of_voxels.ixx:60)
       nvox_ = 0;
