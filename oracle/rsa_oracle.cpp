@@ -298,6 +298,18 @@ uint64_t orc_dist_samples(uint64_t seed, int kind, double param, int64_t n, doub
   return g.used;
 }
 
+// median_of_medians::find_pivot_for_first_elements on one rank (the oracle's mom_pivot):
+// returns 0 and the pivot, or 1 with the reference's error message
+int orc_mom_pivot(const uint64_t* values, int64_t n, int64_t nb_first, double* pivot) {
+  try {
+    *pivot = mom_pivot(std::vector<uint64_t>(values, values + n), nb_first);
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
 // the reference's visiting order of the 3^D stencil (rsa_grid_traversal.ixx:30-66)
 int orc_neighbour_table(int dim, const int* n, int* out) {
   auto build = [&](auto tag) {

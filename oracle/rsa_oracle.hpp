@@ -25,6 +25,27 @@
 //    writing out of bounds (rsa_grid.ixx:182-187).
 //  * get_list_of_spheres keeps every matching sphere instead of writing past a
 //    2^D*3^D-entry buffer (list_of_voxels.ixx:283-307).
+//
+// Kept as the reference writes it, defects included: the outreach pivot search
+// (median_of_medians.ixx:103-118) steps its dichotomy upwards while too many
+// values lie below the pivot and downwards while too few, always from the lower
+// candidate, so a draw meant to keep the N lowest priorities can keep fewer
+// (a 2D three-class run keeps 0 of 2; tests/test_oracle_pins.py shows it).
+//
+// How each part is pinned by something other than this file (tests/):
+//  * MT19937 + seed_seq: the C++ standard's known answer; distributions: closed
+//    forms (Lemire, g >> 1, canonical) and libstdc++'s own Poisson.
+//  * stencil order: a norm sort (libstdc++'s std::sort itself decides ties).
+//  * initial lattice: size formula and coordinate order restated.
+//  * coverage / subdivision predicates: brute force over corners and shifts;
+//    remove_covered: the literal two-pointer loop.
+//  * conflict rounds, periodic images, in-cell and extraction order, outreach
+//    retention: a second restatement in Python (tests/_rsa_python.py) replaying
+//    every draw of multi-class runs with outreach draws, per draw and in final
+//    order; non-outreach draws also against sequential greedy RSA.
+//  * pivot search: a second transcription in Python, bit for bit, plus what it
+//    selects (a prefix of the sorted priorities).
+//  * class counts: restated from the definition; volume: restated per-cell sum.
 #pragma once
 
 #include <algorithm>

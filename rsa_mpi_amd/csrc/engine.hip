@@ -1895,6 +1895,16 @@ struct VoxParams {
   // max(a^2, b^2) >= (a^2 + b^2) / 2 = (delta - len/2)^2 + len^2/4), so a sphere
   // with |c - centre|^2 > R^2 - h2 cannot cover the box.
   double h2;
+  // The pre-check evaluates |c - centre|^2 from the rounded centre() of the
+  // box, the full test from its rounded origin: their positions differ by a few
+  // roundings of the coordinates' magnitude X (per dimension at most 8 eps X
+  // with eps = 2^-53, over D dimensions |e| <= 8 sqrt(D) eps X).  With a
+  // covering sphere |c - centre| <= R, so the computed |c - centre|^2 exceeds
+  // the exact one by at most 2 R |e| + |e|^2; the sums' own roundings stay
+  // below 1e-9 R^2.  slack() is that bound times 4:
+  // 1e-9 R^2 + xs R + xs2 with xs = 64 sqrt(D) eps X, xs2 = 256 D eps^2 X^2.
+  double xs, xs2;
+  __device__ __forceinline__ double slack(double R2, double rr) const { return 1e-9 * R2 + xs * rr + xs2; }
 };
 
 // auxi::is_in_sphere (list_of_voxels.ixx:225-238) asks whether every corner
@@ -2025,8 +2035,8 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
         const double t = qp[d] - cb[d];
         s2 += t * t;
       }
-      // necessary for coverage (VoxParams::h2); the 1e-9 R^2 slack absorbs rounding
-      if (do_cov && (vp.h2 < 0. || s2 <= R2 - vp.h2 + 1e-9 * R2)) {
+      // necessary for coverage (VoxParams::h2); the slack absorbs rounding
+      if (do_cov && (vp.h2 < 0. || s2 <= R2 - vp.h2 + vp.slack(R2, rr))) {
         double sf = 0.;
 #pragma unroll
         for (int d = 0; d < D; d++) sf += TC::both(d) ? far_term<true>(qp[d] - ov[d], vp.len[d])
@@ -2101,7 +2111,8 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
               const double t = q.c[d] - cb[d];
               s2 += t * t;
             }
-            const bool use = (do_child && s2 < R2) || (do_cov && (vp.h2 < 0. || s2 <= R2 - vp.h2 + 1e-9 * R2));
+            const bool use =
+                (do_child && s2 < R2) || (do_cov && (vp.h2 < 0. || s2 <= R2 - vp.h2 + vp.slack(R2, rr)));
             if (use) {
               if (nl == KLIST) flush();
               s_list[nl++][threadIdx.x] = idx;
@@ -3457,6 +3468,13 @@ class Engine final : public IEngine {
     }
     if (distinct != prod) throw Error("internal: TabCorner rows are not a product set");
     vp.rmin = rmin;
+    // X: the largest coordinate magnitude of a voxel corner or a (shifted) sphere centre
+    double X = 0.;
+    for (int d = 0; d < D; d++)
+      X = std::max(X, std::max(std::fabs(inf_[d]), std::fabs(sup_[d])) + std::fabs(L_[d]));
+    const double eps = std::ldexp(1.0, -53);
+    vp.xs = 64. * std::sqrt((double)D) * eps * X;
+    vp.xs2 = 256. * D * (eps * X) * (eps * X);
     vp.h2 = 0.;
     for (int d = 0; d < D; d++) vp.h2 += (0.5 * vp.len[d]) * (0.5 * vp.len[d]);
     for (int d = 0; d < D; d++)

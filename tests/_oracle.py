@@ -163,6 +163,18 @@ def dist_samples(seed, kind, param, n):
     return out, int(used)
 
 
+def mom_pivot(values, nb_first):
+    """The oracle's find_pivot_for_first_elements (raises with the reference's message)."""
+    L = lib()
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    out = ctypes.c_double()
+    rc = L.orc_mom_pivot(v.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.c_int64(len(v)),
+                         ctypes.c_int64(nb_first), ctypes.byref(out))
+    if rc:
+        raise RuntimeError(L.orc_last_error().decode())
+    return out.value
+
+
 def neighbour_table(dim, n):
     out = np.zeros(3 ** dim, dtype=np.int32)
     nn = np.ascontiguousarray(n, dtype=np.int32)

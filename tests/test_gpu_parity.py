@@ -115,6 +115,9 @@ def test_stop_after_k_draws_bit_exact(dim, L, r, seed):
     (2, 1.0, [(0.02, 60, 0), (0.01, 500, 1), (0.005, 4000, 2)], 0),
     (2, 0.5, [(0.01, 300, 3), (0.004, 500, 4)], 5),
     (3, 0.2, [(0.01, 300, 0), (0.005, 3000, 1)], 2),
+    # an outreach draw whose pivot keeps none of its 4 Undecided candidates for 2 places
+    # (the reference's dichotomy, median_of_medians.ixx:103-118), then a fill class
+    (2, 0.3, [(0.02, 20, 0), (0.012, 50, 1), (0.008, 10 ** 9, 2)], 0),
 ])
 def test_count_classes_bit_exact(dim, L, classes, seed):
     """(radius, count, phase) classes placed largest first: draws that may
@@ -324,3 +327,74 @@ def test_list_redo_path_bit_exact(monkeypatch, dim, L, r, seed):
     assert_same(dict(pos=pos, rad=rad, phase=ph, stats=st), o)
     assert st["words"] == o["words"]
     assert st["draws"] == len(o["trace"])
+
+
+def reference_volume(dim, L, rmax, pos, rad):
+    """compute_total_volume_of_spheres on one rank, one OpenMP thread (rsa_domain.ixx:171-175,
+    rsa_grid.ixx:74-86): per real cell the sum of volumeSphere(r) = mult * puissance<D>(r) in
+    in-cell order, the cell sums added in cell index order; mult = volumeUnitSphere(D)
+    (auxiliary/Geometry/Sphere.ixx:31-57), puissance as in AuxiFunctions.ixx:34-44.  pos and
+    rad come in extraction order (cells by index, in-cell order)."""
+    import math
+    mult = 1.0
+    if dim % 2 == 0:
+        for k in range(dim // 2):
+            mult *= (2 * math.pi) / (dim - 2 * k)
+    else:
+        mult = 1.0 / math.pi
+        for k in range((dim - 1) // 2 + 1):
+            mult *= (2 * math.pi) / (dim - 2 * k)
+    r = np.asarray(rad, dtype=np.float64)
+    p = {2: r * r, 3: r * (r * r), 4: (r * r) * (r * r)}[dim]
+    v = mult * p
+    # cell of each centre (compute_cell_idx with the grid of rsa_grid_traversal::define)
+    nin = int(L / (2 * rmax))
+    cl = L / nin
+    shift = 0.0 - cl
+    inv = 1.0 / cl
+    n = nin + 2
+    idx = np.zeros(len(r), dtype=np.int64)
+    for d in range(dim - 1, -1, -1):
+        idx = idx * n + ((np.asarray(pos)[:, d] - shift) * inv).astype(np.int64)
+    assert np.all(np.diff(idx) >= 0)  # extraction order is cell order
+    starts = np.r_[0, np.nonzero(np.diff(idx))[0] + 1]
+    res = 0.0
+    for a, b in zip(starts, np.r_[starts[1:], len(r)]):
+        loc = 0.0
+        for x in v[a:b]:
+            loc += float(x)
+        res += loc
+    return res
+
+
+@pytest.mark.parametrize("dim,L,classes,seed", [(2, 0.5, [(0.01, 1.0, 0)], 1), (3, 0.2, [(0.005, 1.0, 0)], 0),
+                                                (3, 0.3, [(0.01, 0.1, 0), (0.005, 1.0, 1)], 3)])
+def test_volume_of_spheres_is_the_reference_sum(dim, L, classes, seed):
+    """rsa_gpu_volume_of_spheres equals, bit for bit, the reference's per-cell summation
+    restated on the oracle's spheres (the placed sets are equal, checked first)."""
+    g = gpu_run(dim, L, classes, seed=seed)
+    o = orc_run(dim, L, classes, seed=seed)
+    assert_same(g, o)
+    want = reference_volume(dim, L, max(c[0] for c in classes), o["pos"], o["rad"])
+    got = g["dom"].compute_total_volume_of_spheres()
+    assert np.float64(got).view(np.uint64) == np.float64(want).view(np.uint64), (got, want)
+
+
+@pytest.mark.parametrize("dim,off,L,r,seed", [(3, 1.0e5, 0.12, 0.01, 2), (2, -7.5e6, 0.3, 0.004, 1),
+                                              (3, 3.0e7, 0.1, 0.006, 0)])
+def test_offset_domain_bit_exact(dim, off, L, r, seed):
+    """A periodic box far from the origin ([off, off + L]^D): coordinates carry large
+    exponents while radii and voxels stay small, so the coverage pre-check's slack must
+    scale with the coordinates' magnitude.  The packing is the oracle's bit for bit."""
+    inf = [off] * dim
+    sup = [off + L] * dim
+    dom = R.Domain(inf, sup, r)
+    gen = R.RadiusGenerator([(r, 1.0, 0)], dom.get_total_volume(), 0.0)
+    algo = R.Algo(dom, gen, 6000)
+    st = algo.proceed(seed)
+    pos, rad, ph = dom.extract_arrays()
+    o = O.run(dim, inf, sup, [(r, 1.0, 0)], seed=seed)
+    assert st["end_reason"] == 1 and o["msg"].endswith("fully packed")
+    assert len(rad) == o["n"] > 0
+    assert np.array_equal(pos.view(np.uint64), o["pos"].view(np.uint64))
+    assert st["words"] == o["words"]

@@ -164,3 +164,89 @@ def test_binary_dump_round_trip_cpp_python(tmp_path):
     with pytest.raises(ValueError):
         open(str(tmp_path / "c.bin"), "wb").write(b"not a dump at all, 24+ bytes")
         R.read_binary(str(tmp_path / "c.bin"))
+
+
+# ---------------------------------------------------------------- coverage pre-check (k_cover)
+
+EPS53 = 2.0 ** -53
+TAB3 = TABS[3]
+
+
+def reference_covered(q, qr, rmin, origin, c, ln):
+    """auxi::is_covered for one sphere (list_of_voxels.ixx:146-153, 225-259): voxel origin
+    origin + c * len, delta = centre - origin, every TabCorner corner within r + rmin."""
+    o = [origin[d] + float(c[d]) * ln[d] for d in range(3)]
+    delta = [q[d] - o[d] for d in range(3)]
+    R2 = (qr + rmin) * (qr + rmin)
+    for row in TAB3:
+        s = 0.0
+        for d in range(3):
+            t = float(row[d]) * ln[d] - delta[d]
+            s += t * t
+        if s > R2:
+            return False
+    return True
+
+
+def kernel_covered(q, qr, rmin, origin, c, ln, X, slack="scaled"):
+    """k_cover's decision for one sphere, step for step in IEEE double: the pre-check
+    |q - centre()|^2 <= R^2 - h^2 + slack, then the farthest-corner sum from the
+    children's tab-0 origin origin + (2c) * (len / 2)."""
+    nl = [0.5 * v for v in ln]
+    ok0 = [origin[d] + float(2 * c[d]) * nl[d] for d in range(3)]
+    cb = [origin[d] + (float(c[d]) + 0.5) * ln[d] for d in range(3)]
+    rr = qr + rmin
+    R2 = rr * rr
+    h2 = 0.0
+    for d in range(3):
+        h2 += (0.5 * ln[d]) * (0.5 * ln[d])
+    s2 = 0.0
+    for d in range(3):
+        t = q[d] - cb[d]
+        s2 += t * t
+    if slack == "fixed":  # the earlier form: a slack relative to R^2 only
+        sl = 1e-9 * R2
+    else:  # VoxParams::slack
+        sl = 1e-9 * R2 + 64.0 * np.sqrt(3.0) * EPS53 * X * rr + 256.0 * 3 * (EPS53 * X) * (EPS53 * X)
+    if not (s2 <= R2 - h2 + sl):
+        return False
+    sf = 0.0
+    for d in range(3):
+        dl = q[d] - ok0[d]
+        aa = dl * dl
+        b = ln[d] - dl
+        bb = b * b
+        sf += bb if bb > aa else aa
+    return not (sf > R2)
+
+
+def test_coverage_precheck_never_skips_a_covering_sphere():
+    """Boxes far from the origin (|x| up to 1e8) with radii down to 1e-9 of the box and
+    voxels refined up to 60 times, each sphere placed so that the box's farthest corner
+    sits at the covering distance: k_cover's pre-check plus farthest-corner test decides
+    exactly as the reference's corner loop.  A slack relative to R^2 alone (the earlier
+    form) skips some covering spheres here, so the test can tell the two apart."""
+    rng = np.random.default_rng(0)
+    wrong_fixed = wrong = covered = 0
+    for _ in range(40000):
+        X0 = 10.0 ** rng.uniform(0, 8)
+        origin = [X0 * rng.uniform(0.5, 1.0) * rng.choice([-1, 1]) for _ in range(3)]
+        W = X0 * 10.0 ** rng.uniform(-3, 0)
+        R = W * 10.0 ** rng.uniform(-9, -2)
+        k = int(rng.integers(0, 60))
+        ln = [W / 7 * 2.0 ** -k] * 3
+        c = [int(rng.integers(0, 7 * 2 ** k)) for _ in range(3)]
+        o = [origin[d] + c[d] * ln[d] for d in range(3)]
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        h = np.sqrt(sum((0.5 * v) ** 2 for v in ln))
+        dist = (R - h) * rng.uniform(0.999999, 1.0) if R > h else 0.0
+        q = [o[d] + 0.5 * ln[d] + u[d] * dist for d in range(3)]
+        X = max(max(abs(origin[d]), abs(origin[d] + W)) + W for d in range(3))
+        ref = reference_covered(q, 0.5 * R, 0.5 * R, origin, c, ln)
+        covered += ref
+        wrong += kernel_covered(q, 0.5 * R, 0.5 * R, origin, c, ln, X) != ref
+        wrong_fixed += kernel_covered(q, 0.5 * R, 0.5 * R, origin, c, ln, X, slack="fixed") != ref
+    assert covered > 10000
+    assert wrong == 0
+    assert wrong_fixed > 0

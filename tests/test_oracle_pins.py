@@ -288,3 +288,144 @@ def test_class_counts_from_volume_fractions():
     assert int((ph == 0).sum()) == n0
     assert int((ph == 1).sum()) == n1
     assert o["msg"] == "End of generation, due to lack of new radii"
+
+
+# ---------------------------------------------------------------- a second restatement of the rounds
+
+def replay_in_python(dim, L, classes, seed, drad=None):
+    """Run the oracle with every draw recorded, replay the draws through tests/_rsa_python.py
+    (cell lists, swap-with-last removals, images, first-partner labels, outreach retention)
+    and return (oracle result, per-draw accepted flags of both, Python extraction order)."""
+    import _rsa_python as P
+    rmax = max(c[0] for c in classes)
+    drad = rmax if drad is None else drad
+    o = O.run(dim, [0.0] * dim, [L] * dim, classes, seed=seed, mode=0, drad=drad, dump=1 << 30)
+    g0 = P.Grid(dim, [0.0] * dim, [L] * dim, rmax, [0] * 3 ** dim)
+    grid = P.Grid(dim, [0.0] * dim, [L] * dim, rmax, O.neighbour_table(dim, g0.n))
+    sim = P.Sim(grid, drad)
+    counts = [int(c[1]) for c in sorted(classes, key=lambda c: -c[0])]
+    cur = 0
+    pairs = []
+    n_outreach = 0
+    for dd in o["dumps"]:
+        maximal = counts[cur]
+        may = len(dd["rad"]) >= maximal
+        assert bool(dd["outreach"]) == may
+        n_outreach += may
+        acc = sim.draw(dd["pos"], dd["rad"], dd["prio"], maximal, may)
+        pairs.append((acc, dd["accepted"]))
+        counts[cur] -= int(acc.sum())
+        if counts[cur] == 0:
+            cur += 1
+    pos, rad = sim.extraction()
+    return o, pairs, pos, rad, n_outreach
+
+
+# every class target but the last is reachable (an unreachable one makes the reference
+# refine forever, docs/source/algorithm.rst:57-62); the last class fills the box
+@pytest.mark.parametrize("dim,L,classes,seed", [
+    (2, 0.3, [(0.02, 20, 0), (0.012, 50, 1), (0.008, 10 ** 9, 2)], 0),
+    (2, 0.4, [(0.02, 40, 0), (0.012, 100, 1), (0.008, 10 ** 9, 2)], 4),
+    (2, 0.25, [(0.015, 40, 0), (0.01, 10 ** 9, 1)], 3),
+    (3, 0.15, [(0.012, 60, 0), (0.008, 150, 1), (0.006, 10 ** 9, 2)], 1),
+])
+def test_rounds_and_cell_order_match_a_second_restatement(dim, L, classes, seed):
+    """Every draw of a multi-class run, outreach draws included (the first-partner labels
+    and the retained smallest priorities), accepts the same candidates in the Python
+    restatement as in the oracle, and the final extraction order -- real cells by index,
+    in-cell order left by the swap-with-last removals -- is the oracle's bit for bit."""
+    o, pairs, pos, rad, n_outreach = replay_in_python(dim, L, classes, seed)
+    assert o["msg"].endswith("fully packed")
+    assert n_outreach >= 2
+    for k, (a, b) in enumerate(pairs):
+        assert np.array_equal(a, b), k
+    assert len(rad) == o["n"]
+    assert np.array_equal(pos.view(np.uint64), o["pos"].view(np.uint64))
+    assert np.array_equal(rad.view(np.uint64), o["rad"].view(np.uint64))
+
+
+def test_pivot_search_matches_a_second_transcription():
+    """The oracle's find_pivot_for_first_elements against the Python transcription of
+    median_of_medians.ixx:8-178 on random priority sets (distinct values, heavy ties,
+    values near 0 and near 2^31), every N first from 1 to the size: the same pivot bit for
+    bit, or the same error (the reference throws "Unexpected - 1" when its first candidate
+    already holds too many values and there is no lower one, which ties can cause)."""
+    import _rsa_python as P
+    rng = np.random.default_rng(7)
+    checked = errors = 0
+    for trial in range(400):
+        n = int(rng.integers(1, 40))
+        kind = trial % 4
+        if kind == 0:
+            v = rng.integers(0, 2 ** 31, size=n)
+        elif kind == 1:
+            v = rng.integers(0, 4, size=n)  # ties everywhere
+        elif kind == 2:
+            v = rng.integers(0, 50, size=n)
+        else:
+            v = 2 ** 31 - 1 - rng.integers(0, 1000, size=n)
+        for nb in range(1, n + 1):
+            def outcome(f):
+                try:
+                    return "value", np.float64(f()).view(np.uint64)
+                except RuntimeError as e:  # the reference's "Unexpected - 1" (ties at both candidates)
+                    return "error", str(e)
+            a = outcome(lambda: O.mom_pivot(v, nb))
+            b = outcome(lambda: P.reference_pivot(v.tolist(), nb))
+            assert a == b, (v.tolist(), nb, a, b)
+            errors += a[0] == "error"
+            checked += 1
+    for bad in (0, 5):
+        with pytest.raises(RuntimeError, match="Impossible to find the pivot"):
+            O.mom_pivot([1, 2, 3, 4], bad)
+    assert checked > 5000 and 0 < errors < checked // 4
+
+
+def test_pivot_search_keeps_the_smallest_when_it_reaches_the_count():
+    """What the pivot selects, stated directly: the values not above it are a prefix of
+    the sorted values, and whenever that prefix has N values it is the N smallest.  The
+    reference's dichotomy steps away from the count when too few values lie below
+    (median_of_medians.ixx:103-118), so it can keep fewer: for these four priorities and
+    N = 2 it keeps none (the 2D three-class case below meets exactly this draw)."""
+    import _rsa_python as P
+    rng = np.random.default_rng(3)
+    exact = fewer = 0
+    for _ in range(3000):
+        n = int(rng.integers(2, 30))
+        v = np.unique(rng.integers(0, 2 ** 31, size=n))
+        nb = int(rng.integers(1, len(v) + 1))
+        p = P.reference_pivot(v.tolist(), nb)
+        kept = v[~(v.astype(np.float64) > p)]
+        assert np.array_equal(kept, np.sort(v)[:len(kept)])  # a prefix
+        exact += len(kept) == nb
+        fewer += len(kept) < nb
+    assert exact > 1000 and fewer > 0
+    vals = [1907355172, 1641811781, 1812586757, 934655398]
+    p = O.mom_pivot(vals, 2)
+    assert p < min(vals)  # no priority below the pivot: the draw validates none
+
+
+def test_outreach_retention_defect_in_a_whole_run():
+    """2D three-class run: one outreach draw has 4 Undecided candidates for 2 places and
+    validates none (the pivot case above); the class fills one draw later."""
+    o, pairs, pos, rad, n_outreach = replay_in_python(2, 0.3, [(0.02, 20, 0), (0.012, 50, 1), (0.008, 10 ** 9, 2)], 0)
+    tr = o["trace"]
+    short = [k for k, (a, b) in enumerate(pairs) if b.sum() == 0 and len(b) > 50]
+    assert 14 in short
+    assert int(tr[14, 3]) == 0 and int(tr[15, 3]) == 2
+
+
+@pytest.mark.parametrize("dim,L,classes", [(2, 1.0, [(0.01, 1.0, 0)]), (3, 0.37, [(0.02, 1.0, 0), (0.013, 1.0, 1)]),
+                                           (4, 0.5, [(0.03, 1.0, 0)])])
+def test_initial_voxel_lattice(dim, L, classes):
+    """list_of_voxels(origin, width, r_max + r_min) (list_of_voxels.ixx:11-28, 192-211):
+    n_d = 1 + floor(sqrt(D) * W_d / (r_max + r_min)) voxels of length W_d / n_d per
+    dimension, enumerated with the first coordinate fastest (Loops.ixx:44-74)."""
+    o = O.run(dim, [0.0] * dim, [L] * dim, classes, seed=0, max_draws=0)
+    rmax = max(c[0] for c in classes)
+    rmin = min(c[0] for c in classes)
+    n = 1 + int(math.sqrt(dim) * L / (rmax + rmin))
+    grid = np.stack(np.meshgrid(*[np.arange(n)] * dim, indexing="ij"), axis=-1)
+    want = grid.transpose(list(range(dim))[::-1] + [dim]).reshape(-1, dim)  # coordinate 0 fastest
+    assert np.array_equal(o["vox"], want)
+    assert np.array_equal(o["vox_len"], np.full(dim, L / n))
