@@ -40,6 +40,12 @@ typedef struct rsa_gpu_stats {
   int end_reason;           /* 1 fully packed, 2 lack of new radii, 3 draw limit */
   double seconds;           /* wall time of rsa_gpu_proceed */
   double voxel_seconds;     /* GPU time in refine passes (device events) */
+  /* End-of-run checks, run on the device by every proceed (counterparts of the
+   * reference's debug asserts, operator_algorithm.ixx:269,563,
+   * rsa_data_storage.ixx:229-243, operator_check_cell.ixx:36-78): */
+  int64_t overlap_pairs;    /* pairs of placed spheres (images included) that collide */
+  int64_t cover_samples;    /* points sampled over the box for maximality */
+  int64_t cover_free;       /* of those, points farther than r + r_min from every sphere */
 } rsa_gpu_stats;
 
 /* Last error message of the calling thread ("" if none). */
@@ -88,6 +94,14 @@ int rsa_gpu_algo_init_law(rsa_gpu_ctx* ctx, double min_radius, double max_radius
  * classes, placed spheres kept. */
 int rsa_gpu_reset_classes(rsa_gpu_ctx* ctx, int n, const double* radius, const uint64_t* count,
                           const int* phase, double exclusion);
+
+/* rsa_algo<DIM>::reset_radius_generator (operator_algorithm.ixx:238-250) to a
+ * RadiusGenerator(min_radius, max_radius, RandomRadiusGenerator*, count,
+ * exclusion) (radius_generator.ixx:38-48): the custom law of rsa_gpu_algo_init_law
+ * for the next proceed, placed spheres kept; a max_radius above the current
+ * generator's fails with "Incorrect maximal radius". */
+int rsa_gpu_reset_law(rsa_gpu_ctx* ctx, double min_radius, double max_radius, rsa_gpu_radius_law law, void* user,
+                      uint64_t phase, uint64_t count, double exclusion);
 
 /* rsa_algo<DIM>::set_miss_rate (operator_algorithm.hxx:66). */
 int rsa_gpu_set_miss_rate(rsa_gpu_ctx* ctx, double miss_rate);

@@ -403,7 +403,13 @@ class rsa_algo {
   }
   void install(bool first) {
     if (const RandomRadiusGenerator* law = m_radius_generator.custom_law()) {
-      if (!first) throw std::runtime_error("reset_radius_generator with a custom radius law is not supported");
+      if (!first) {
+        rsa_mpi_amd_detail::check(rsa_gpu_reset_law(
+            m_domain.ctx(), m_radius_generator.get_min_radius(), m_radius_generator.get_max_radius(), &law_trampoline,
+            const_cast<RandomRadiusGenerator*>(law), law->phase, m_radius_generator.get_current_number(),
+            m_radius_generator.exclusion_distance()));
+        return;
+      }
       rsa_mpi_amd_detail::check(rsa_gpu_algo_init_law(
           m_domain.ctx(), m_radius_generator.get_min_radius(), m_radius_generator.get_max_radius(), &law_trampoline,
           const_cast<RandomRadiusGenerator*>(law), law->phase, m_radius_generator.get_current_number(),

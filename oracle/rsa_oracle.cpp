@@ -12,6 +12,12 @@ namespace {
 
 thread_local std::string g_err;
 thread_local int g_dump_limit = 0;
+// re-iterations with a uniform radius law instead of a single class (k -> r0, r1, phase, count)
+struct ReiterLaw {
+  double r0, r1;
+  uint64_t phase, count;
+};
+thread_local std::vector<std::pair<int, ReiterLaw>> g_reiter_law;
 
 struct Dump {
   std::vector<double> pos, rad;
@@ -69,8 +75,17 @@ Result* run_impl(const double* inf, const double* sup, int gl, double drad, int 
   algo.proceed(seed);
   // re-iteration (scripts/main_max_frac.cpp:43-52): new single-class generator, same seed
   for (int k = 0; k < n_reiter; k++) {
-    std::vector<std::tuple<double, double, int>> c = {std::make_tuple(reiter_radius[k], 1., 0)};
-    algo.reset_radius_generator(RadiusGen<D>::from_fractions(c, dom.total_volume()));
+    const ReiterLaw* law = nullptr;
+    for (const auto& e : g_reiter_law)
+      if (e.first == k) law = &e.second;
+    if (law) {  // RadiusGenerator(min, max, RandomRadiusGenerator*, count) (radius_generator.ixx:38-48)
+      const double r0 = law->r0, r1 = law->r1;
+      algo.reset_radius_generator(RadiusGen<D>::from_law(
+          r0, r1, [r0, r1](double x) { return r0 + (r1 - r0) * x; }, law->phase, law->count, 0.));
+    } else {
+      std::vector<std::tuple<double, double, int>> c = {std::make_tuple(reiter_radius[k], 1., 0)};
+      algo.reset_radius_generator(RadiusGen<D>::from_fractions(c, dom.total_volume()));
+    }
     algo.proceed(seed);
   }
   auto* r = new Result();
@@ -253,6 +268,15 @@ void orc_free(void* h) { delete static_cast<Result*>(h); }
 
 // record the candidates of the first k draws of the next orc_run on this thread
 void orc_set_dump(int k) { g_dump_limit = k; }
+// re-iteration k of the next orc_run on this thread uses the law r0 + (r1 - r0) u (count spheres
+// of `phase`); k < 0 clears every such setting
+void orc_set_reiter_law(int k, double r0, double r1, uint64_t phase, uint64_t count) {
+  if (k < 0) {
+    g_reiter_law.clear();
+    return;
+  }
+  g_reiter_law.push_back({k, ReiterLaw{r0, r1, phase, count}});
+}
 int64_t orc_num_dumps(void* h) { return (int64_t)static_cast<Result*>(h)->dumps.size(); }
 int64_t orc_dump_size(void* h, int64_t k) { return (int64_t)static_cast<Result*>(h)->dumps[k].rad.size(); }
 int orc_dump(void* h, int64_t k, double* pos, double* rad, uint64_t* prio, uint8_t* acc) {

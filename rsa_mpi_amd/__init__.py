@@ -229,6 +229,14 @@ class Algo:
         return self.last_stats
 
     def reset_radius_generator(self, radius_generator):
+        """rsa_algo::reset_radius_generator (operator_algorithm.ixx:238-250): classes or a custom law."""
+        if isinstance(radius_generator, LawRadiusGenerator):
+            g = radius_generator
+            self._law = g._callback()
+            self.domain._law = self._law
+            check(lib().rsa_gpu_reset_law(self.domain._ctx, g.min_radius, g.max_radius, self._law, None, g.phase,
+                                          g.count, g.exclusion))
+            return
         r, c, p = radius_generator._arrays(self.domain.dim)
         check(lib().rsa_gpu_reset_classes(self.domain._ctx, len(r), dptr(r), c.ctypes.data_as(_lib._U64),
                                           p.ctypes.data_as(_lib._I), radius_generator.exclusion))

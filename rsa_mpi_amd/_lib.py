@@ -18,6 +18,7 @@ EXPORTED = (
     "rsa_gpu_set_miss_rate", "rsa_gpu_proceed", "rsa_gpu_add_spheres", "rsa_gpu_count",
     "rsa_gpu_extract", "rsa_gpu_volume_of_spheres", "rsa_gpu_voxels", "rsa_gpu_mt_words",
     "rsa_gpu_profile_enable", "rsa_gpu_kernel_profile", "rsa_gpu_batch_run", "rsa_gpu_algo_init_law",
+    "rsa_gpu_reset_law",
 )
 
 # rsa_gpu_radius_law: void (*)(const double* u, double* radius, int64_t n, void* user)
@@ -30,7 +31,8 @@ class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "loop_iterations", "draws", "shots", "accepted", "refines", "rounds", "rounds_max",
         "nvox_max", "nvox_final", "words", "ghost_cell_drops", "image_cell_mismatch")] + [
-        ("end_reason", ctypes.c_int), ("seconds", ctypes.c_double), ("voxel_seconds", ctypes.c_double)]
+        ("end_reason", ctypes.c_int), ("seconds", ctypes.c_double), ("voxel_seconds", ctypes.c_double)] + [
+        (n, ctypes.c_int64) for n in ("overlap_pairs", "cover_samples", "cover_free")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -75,6 +77,8 @@ def lib():
     L.rsa_gpu_kernel_profile.argtypes = [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _I64, _D, _D]
     L.rsa_gpu_algo_init_law.argtypes = [_P, ctypes.c_double, ctypes.c_double, RADIUS_LAW, ctypes.c_void_p,
                                         ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, ctypes.c_int]
+    L.rsa_gpu_reset_law.argtypes = [_P, ctypes.c_double, ctypes.c_double, RADIUS_LAW, ctypes.c_void_p,
+                                    ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double]
     L.rsa_gpu_batch_run.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _U64, ctypes.c_int, ctypes.c_int, _U64, _D,
                                     _D, _D, ctypes.c_uint64, _D]
     _lib = L

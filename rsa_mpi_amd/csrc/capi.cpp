@@ -133,6 +133,12 @@ int rsa_gpu_reset_classes(rsa_gpu_ctx* ctx, int n, const double* radius, const u
   return guarded([&] { ctx->eng->reset_classes(make_classes(n, radius, count, phase), exclusion); });
 }
 
+int rsa_gpu_reset_law(rsa_gpu_ctx* ctx, double min_radius, double max_radius, rsa_gpu_radius_law law, void* user,
+                      uint64_t phase, uint64_t count, double exclusion) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { ctx->eng->reset_law(min_radius, max_radius, law, user, phase, count, exclusion); });
+}
+
 int rsa_gpu_set_miss_rate(rsa_gpu_ctx* ctx, double miss_rate) {
   if (need_ctx(ctx)) return 1;
   return guarded([&] { ctx->eng->set_miss_rate(miss_rate); });

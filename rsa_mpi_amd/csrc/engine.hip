@@ -245,6 +245,8 @@ enum Ctr : int {
   C_SCATTER_ERR,  // k_scatter found a workgroup's children inconsistent with the counting pass
   C_COMPACT_ERR,  // k_compact found more stored spheres than the packed copy holds
   C_MOVED,        // voxels k_move moved into holes (the compaction's traffic)
+  C_OVERLAP,      // end-of-run check: colliding (sphere, neighbour) pairs, each pair seen from both sides
+  C_UNCOVERED,    // end-of-run check: sample points a sphere of the smallest radius could still take
   C_NUM
 };
 
@@ -2562,6 +2564,112 @@ __global__ void k_insert(CellStore<D> cs, int64_t n, const int* __restrict__ cel
   if (last[i]) cs.cnt[c] = (uint8_t)(k + 1);
 }
 
+// ============================================================== kernels: end-of-run checks
+
+// A wave's count added to one counter, by one lane, only when non-zero (the
+// checks almost always count nothing, so nearly no atomics are issued).
+__device__ __forceinline__ void wave_count(int v, int* counter) {
+  if (__ballot(v != 0) == 0) return;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(counter, v);
+}
+
+// No overlap: the reference's collision predicate (is_there_collision,
+// operator_cells_and_data_storage.ixx:5-14) between every placed sphere and
+// every other sphere or periodic image of its 3^D stencil -- the pairs its
+// debug check tests (rsa_data_storage.ixx:229-243, asserted per draw at
+// operator_algorithm.ixx:269,563).  One thread per cell; a colliding pair is
+// seen from both of its spheres.
+template <int D>
+__global__ void __launch_bounds__(WG) k_check_overlap(Geo<D> g, CellStore<D> cs, int* __restrict__ ctr) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int hits = 0;
+  const int n = c < g.ncells ? cs.cnt[c] : 0;  // ghost cells hold none
+  if (n > 0) {
+    int cc[D];
+    cell_coords<D>(g, c, cc);
+    for (int k = 0; k < n; k++) {
+      const Slot<D> a = cs.sph[(size_t)c * cs.cap + k];
+      for (int st = 0; st < g.nst; st++) {
+        Nb<D> nb;
+        if (!neighbour<D>(g, cc, st, nb)) continue;
+        const int nv = cs.cnt[nb.src];
+        for (int j = 0; j < nv; j++) {
+          if (!nb.ghost && nb.src == c && j == k) continue;
+          const Slot<D> b = cs.sph[(size_t)nb.src * cs.cap + j];
+          double q[D];
+          if (nb.ghost) {
+            if (!image_of<D>(g, nb, b.c, q)) continue;
+          } else {
+#pragma unroll
+            for (int d = 0; d < D; d++) q[d] = b.c[d];
+          }
+          if (collide<D>(a.r, a.c, b.r, q)) hits++;
+        }
+      }
+    }
+  }
+  wave_count(hits, ctr + C_OVERLAP);
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {  // Steele, Lea & Flood (2014)
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Maximality: nsamp points spread over the real box (a counter-based hash, not
+// the run's word stream); a point is still free when every sphere and image of
+// its stencil is farther than r + rmin, the voxel method's coverage margin
+// (list_of_voxels.ixx:225-238).  A maximal packing leaves none free.
+template <int D>
+__global__ void __launch_bounds__(WG) k_check_cover(Geo<D> g, CellStore<D> cs, int64_t nsamp, uint64_t key,
+                                                    double rmin, int* __restrict__ ctr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int free_pt = 0;
+  if (i < nsamp) {
+    double p[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const double u = (double)(splitmix64(key ^ ((uint64_t)i * D + d)) >> 11) * 0x1p-53;
+      p[d] = g.inf[d] + (g.sup[d] - g.inf[d]) * u;
+    }
+    int cc[D];
+    const int64_t cell = cell_index<D>(g, p);
+    cell_coords<D>(g, cell, cc);
+    if (cell >= 0 && cell < g.ncells && !coords_ghost<D>(g, cc)) {
+      bool covered = false;
+      for (int st = 0; st < g.nst && !covered; st++) {
+        Nb<D> nb;
+        if (!neighbour<D>(g, cc, st, nb)) continue;
+        const int nv = cs.cnt[nb.src];
+        for (int j = 0; j < nv && !covered; j++) {
+          const Slot<D> b = cs.sph[(size_t)nb.src * cs.cap + j];
+          double q[D];
+          if (nb.ghost) {
+            if (!image_of<D>(g, nb, b.c, q)) continue;
+          } else {
+#pragma unroll
+            for (int d = 0; d < D; d++) q[d] = b.c[d];
+          }
+          double s2 = 0.;
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+            const double t = q[d] - p[d];
+            s2 += t * t;
+          }
+          const double rr = b.r + rmin;
+          covered = s2 <= rr * rr;
+        }
+      }
+      free_pt = covered ? 0 : 1;
+    }
+  }
+  wave_count(free_pt, ctr + C_UNCOVERED);
+}
+
 // ============================================================== host engine
 
 static double volume_unit_sphere(int dim) {  // auxiliary/Geometry/Sphere.ixx:31-44
@@ -2739,6 +2847,20 @@ class Engine final : public IEngine {
     }
     r_max_ = std::max(r_max_, gmax_);
     // the cell store keeps the spheres already placed: size it for every radius
+    store_rmin_ = std::min(store_rmin_, gmin_);
+    grow_cells(cell_cap(store_rmin_, r_max_ > store_rmin_ ? r_max_ : store_rmin_));
+  }
+
+  // reset_radius_generator to a RadiusGenerator with a custom law (operator_algorithm.ixx:238-250,
+  // radius_generator.ixx:38-48): same checks and kept spheres as reset_classes
+  void reset_law(double rmin, double rmax, rsa_gpu_radius_law law, void* user, uint64_t phase, uint64_t count,
+                 double exclusion) override {
+    StreamScope scope(st_, cst_);
+    if (!algo_ready_) throw Error("rsa_algo not initialised");
+    (void)exclusion;  // kept by the reference's generator, not added to the law's radii
+    if (gmax_ < rmax) throw Error("Incorrect maximal radius");
+    set_law(rmin, rmax, law, user, phase, count);  // validates its arguments before changing anything
+    r_max_ = std::max(r_max_, gmax_);
     store_rmin_ = std::min(store_rmin_, gmin_);
     grow_cells(cell_cap(store_rmin_, r_max_ > store_rmin_ ? r_max_ : store_rmin_));
   }
@@ -4306,7 +4428,22 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     cur_live = nxt_live;
   }
   if (prof_) prof_add("host_wait_mt_words", 1e-9 * (double)hs_.wait_ns.load(), 0);
+  // end-of-run checks on the device (every run): no overlap, and no free point
+  // left for a sphere of the smallest radius
+  const int64_t nsamp = std::min<int64_t>(int64_t(1) << 22, std::max<int64_t>(int64_t(1) << 16, 4 * (int64_t)nb_fast_));
+  HIP_OK(hipMemsetAsync(dctr_.p + C_OVERLAP, 0, 2 * sizeof(int), st_));
+  timed("k_check_overlap", (double)geo_.ncells + (double)nb_fast_ * sizeof(Slot<D>), [&] {
+    hipLaunchKernelGGL(k_check_overlap<D>, dim3(nblocks(geo_.ncells)), dim3(WG), 0, st_, geo_, cs(), dctr_.p);
+  }, 1);
+  timed("k_check_cover", (double)nsamp * 8.0 * D, [&] {
+    hipLaunchKernelGGL(k_check_cover<D>, dim3(nblocks(nsamp)), dim3(WG), 0, st_, geo_, cs(), nsamp,
+                       0x5eed0c0ffee5eedull ^ seed, gmin_, dctr_.p);
+  }, 1);
+  HIP_OK(hipGetLastError());
   read_ctr();  // the stream is idle after this
+  st->overlap_pairs = hctr_[C_OVERLAP] / 2;
+  st->cover_samples = nsamp;
+  st->cover_free = hctr_[C_UNCOVERED];
   check_scatter();
   resolve_prof();
   (void)hipEventDestroy(e0);

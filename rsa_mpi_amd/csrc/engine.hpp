@@ -26,6 +26,8 @@ class IEngine {
   virtual void algo_init_law(double rmin, double rmax, rsa_gpu_radius_law law, void* user, uint64_t phase,
                              uint64_t count, double exclusion, int nbshots) = 0;
   virtual void reset_classes(const std::vector<RadiusClass>& classes, double exclusion) = 0;
+  virtual void reset_law(double rmin, double rmax, rsa_gpu_radius_law law, void* user, uint64_t phase,
+                         uint64_t count, double exclusion) = 0;
   virtual void set_miss_rate(double m) = 0;
   virtual void proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) = 0;
   virtual void add_spheres(int64_t n, const double* pos, const double* rad, const uint64_t* phase) = 0;

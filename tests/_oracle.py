@@ -64,8 +64,21 @@ def run(dim, inf, sup, classes, seed=0, mode=1, nbshots=6000, max_draws=-1, drad
         compat=True, reiter=(), dump=0):
     """Full proceed<1>(seed).  classes: [(radius, amount, phase)], amount is a
     volume fraction (mode 1), a count (mode 0); mode 2 is a uniform radius law
-    with classes[0] = (r_min, count, phase), classes[1] = (r_max, ., .)."""
+    with classes[0] = (r_min, count, phase), classes[1] = (r_max, ., .).  reiter:
+    re-iterations after the first proceed (same seed), each a radius (one class at
+    volume fraction 1) or a tuple ("law", r_min, r_max, phase, count)."""
     L = lib()
+    L.orc_set_reiter_law.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                     ctypes.c_uint64]
+    L.orc_set_reiter_law(-1, 0.0, 0.0, 0, 0)
+    plain = []
+    for k, e in enumerate(reiter):
+        if isinstance(e, tuple):
+            L.orc_set_reiter_law(k, e[1], e[2], e[3], e[4])
+            plain.append(0.0)
+        else:
+            plain.append(float(e))
+    reiter = plain
     inf = np.ascontiguousarray(inf, dtype=np.float64)
     sup = np.ascontiguousarray(sup, dtype=np.float64)
     rad = np.ascontiguousarray([c[0] for c in classes], dtype=np.float64)
@@ -81,6 +94,7 @@ def run(dim, inf, sup, classes, seed=0, mode=1, nbshots=6000, max_draws=-1, drad
                    ctypes.c_uint64(seed), ctypes.c_int64(max_draws), 1 if compat else 0, len(reiter), _d(rr),
                    ctypes.byref(h))
     L.orc_set_dump(0)
+    L.orc_set_reiter_law(-1, 0.0, 0.0, 0, 0)
     if rc:
         raise RuntimeError(L.orc_last_error().decode())
     try:

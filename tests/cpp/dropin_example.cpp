@@ -34,7 +34,7 @@ static void dump(const std::string& path, const std::vector<rsa_sphere<DIM>>& s)
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s uniform3d|xyz3d|throw2d|reiter3d|law3d|pv out [args]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s uniform3d|xyz3d|throw2d|reiter3d|reiterlaw3d|law3d|pv out [args]\n", argv[0]);
     return 1;
   }
   rsa_mpi::init(&argc, &argv);
@@ -89,6 +89,26 @@ int main(int argc, char** argv) {
     sac_de_billes::RadiusGenerator<DIM> radius_generator(r_min, r_max, &random_radius_generator, nb_spheres_max, 0.);
     rsa_domain<DIM> domain(domain_inf, domain_sup, 1, radius_generator.get_max_radius());
     algorithm::uniform_generate<DIM, 1, false>(domain, radius_generator, 6000, 10, seed);
+    dump<DIM>(out, domain.extract_spheres());
+  } else if (mode == "reiterlaw3d") {  // scripts/main_max_frac.cpp:39-52 with a law generator in the loop
+    constexpr int DIM = 3;
+    const double L = std::atof(argv[3]);
+    const size_t seed = std::strtoull(argv[4], nullptr, 10);
+    rsa_domain<DIM> domain(create_array<DIM>(0.), create_array<DIM>(L), 1, 0.01);
+    sac_de_billes::RadiusGenerator<DIM> g1(std::vector<std::tuple<double, double, int>>{{0.01, 0.2, 0}},
+                                           domain.get_total_volume());
+    algorithm::rsa_algo<DIM> algo(domain, g1, 6000, 10);
+    algo.proceed<1>(seed);
+    const double r_min = 0.006, r_max = 0.008;
+    auto nonlinear_transform = [r_min, r_max](double x) { return r_min + (r_max - r_min) * x; };
+    sac_de_billes::RandomRadiusGenerator law(nonlinear_transform, 2);
+    sac_de_billes::RadiusGenerator<DIM> g2(r_min, r_max, &law, 400, 0.);
+    algo.reset_radius_generator(g2);
+    algo.proceed<1>(seed);
+    sac_de_billes::RadiusGenerator<DIM> g3(std::vector<std::tuple<double, double, int>>{{0.005, 1.0, 0}},
+                                           domain.get_total_volume());
+    algo.reset_radius_generator(g3);
+    algo.proceed<1>(seed);
     dump<DIM>(out, domain.extract_spheres());
   } else if (mode == "pv") {  // uniform_generate then rsa_paraview::paraview(domain, out) (test.hxx:45-50)
     const int dim = std::atoi(argv[3]);

@@ -74,6 +74,18 @@ def test_user_interface_reiteration_3d(tmp_path):
 
 
 @pytest.mark.gpu
+def test_reiteration_onto_a_radius_law_3d(tmp_path):
+    """rsa_algo::reset_radius_generator to a RadiusGenerator with a RandomRadiusGenerator
+    (operator_algorithm.ixx:238-250, radius_generator.ixx:38-48), then back to a class, each
+    followed by proceed(seed): the packing is the oracle's bit for bit."""
+    out, _ = run_bin(tmp_path, "reiterlaw3d", 0.2, 3)
+    pos, rad, ph = read_records(out, 3)
+    o = O.run(3, [0.0] * 3, [0.2] * 3, [(0.01, 0.2, 0)], seed=3, reiter=(("law", 0.006, 0.008, 2, 400), 0.005))
+    same_as_oracle(pos, rad, ph, o)
+    assert int((ph == 2).sum()) == 400
+
+
+@pytest.mark.gpu
 def test_write_xyz_plugin(tmp_path):
     """plugin/write_xyz.hpp:78-98: count, sup, then '0 x y z' in default stream format (6 digits)."""
     sup = (0.2, 0.3, 0.25)

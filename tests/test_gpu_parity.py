@@ -33,8 +33,21 @@ def orc_run(dim, L, classes, **kw):
     return O.run(dim, [0.0] * dim, [float(L)] * dim, classes, **kw)
 
 
+def assert_checks(st, dim=3):
+    """The device's end-of-run checks: no two spheres (images included) collide and, for a
+    packing that ended fully packed, no sampled point is left free for a smallest sphere.
+    In 4D the reference's corner table leaves a facet of every voxel untested
+    (auxiliary/Path.ixx:105-115), so its fully packed runs need not be maximal; the check
+    is reported there, not required."""
+    assert st["overlap_pairs"] == 0, st
+    assert st["cover_samples"] >= 1 << 16
+    if st["end_reason"] == 1 and dim <= 3:
+        assert st["cover_free"] == 0, st
+
+
 def assert_same(g, o):
     print("gpu n", len(g["rad"]), "oracle n", o["n"], "stats", g["stats"])
+    assert_checks(g["stats"], g["pos"].shape[1])
     assert len(g["rad"]) == o["n"]
     assert set_hash(g["pos"], g["rad"]) == set_hash(o["pos"], o["rad"])
     # extraction order (real cells by index, then in-cell order) is the reference's
@@ -158,6 +171,7 @@ def _golden(name):
 def _check_golden(g, gold, classes_phase=None):
     st = g["stats"]
     print(st)
+    assert_checks(st, g["pos"].shape[1])
     assert len(g["rad"]) == gold["n"]
     assert set_hash(g["pos"], g["rad"]) == gold["sha256"]
     assert st["words"] == gold["words"]
@@ -240,6 +254,8 @@ def test_4d_r002_matches_golden():
     gold = _golden("4d_r0.02_seed0")
     g = gpu_run(4, 1.0, [(0.02, 1.0, 0)], seed=0)
     _check_golden(g, gold)
+    # the device's maximality check finds the points that defect leaves free
+    assert g["stats"]["cover_free"] > 0
 
 
 # ------------------------------------------------------------ the reference's own test programs
@@ -395,6 +411,40 @@ def test_offset_domain_bit_exact(dim, off, L, r, seed):
     pos, rad, ph = dom.extract_arrays()
     o = O.run(dim, inf, sup, [(r, 1.0, 0)], seed=seed)
     assert st["end_reason"] == 1 and o["msg"].endswith("fully packed")
+    assert_checks(st, dim)
     assert len(rad) == o["n"] > 0
     assert np.array_equal(pos.view(np.uint64), o["pos"].view(np.uint64))
     assert st["words"] == o["words"]
+
+
+def test_device_checks_find_planted_faults():
+    """The end-of-run checks report what they are built to find: spheres handed in with
+    Domain.add that overlap (each such pair counted once), and a box left almost empty
+    (free points sampled)."""
+    dom = R.Domain([0.0] * 3, [0.2] * 3, 0.01)
+    algo = R.Algo(dom, R.RadiusGenerator([(0.01, 1.0, 0)], dom.get_total_volume(), 0.0), 6000)
+    dom.add([[0.05, 0.05, 0.05], [0.06, 0.05, 0.05], [0.15, 0.15, 0.15], [0.195, 0.15, 0.15],
+             [0.004, 0.15, 0.15]], [0.01] * 5)
+    st = algo.proceed(0, 1)  # one draw: the box stays almost empty
+    assert st["overlap_pairs"] == 2  # (0.05, 0.06) and (0.195, 0.004) across the periodic face
+    assert st["end_reason"] == 3 and st["cover_free"] > st["cover_samples"] // 2
+
+
+def test_reset_to_custom_law_python_bit_exact():
+    """Algo.reset_radius_generator(LawRadiusGenerator) between proceeds (the re-iteration of
+    scripts/main_max_frac.cpp:43-62 with a law generator), then a class again."""
+    dom = R.Domain([0.0] * 3, [0.2] * 3, 0.01)
+    algo = R.Algo(dom, R.RadiusGenerator([(0.01, 0.2, 0)], dom.get_total_volume(), 0.0), 6000)
+    algo.proceed(3)
+    algo.reset_radius_generator(R.LawRadiusGenerator(0.006, 0.008, lambda u: 0.006 + (0.008 - 0.006) * u, 400,
+                                                     phase=2))
+    st = algo.proceed(3)
+    assert st["end_reason"] == 2 and st["accepted"] == 400
+    algo.reset_radius_generator(R.RadiusGenerator([(0.005, 1.0, 0)], dom.get_total_volume(), 0.0))
+    st = algo.proceed(3)
+    assert_checks(st)
+    pos, rad, ph = dom.extract_arrays()
+    o = O.run(3, [0.0] * 3, [0.2] * 3, [(0.01, 0.2, 0)], seed=3, reiter=(("law", 0.006, 0.008, 2, 400), 0.005))
+    assert_same(dict(pos=pos, rad=rad, phase=ph, stats=st), o)
+    with pytest.raises(RuntimeError, match="Incorrect maximal radius"):
+        algo.reset_radius_generator(R.LawRadiusGenerator(0.006, 0.02, lambda u: 0.006 + 0.014 * u, 10))
