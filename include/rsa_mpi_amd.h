@@ -125,6 +125,17 @@ int rsa_gpu_count(rsa_gpu_ctx* ctx, uint64_t* n);
 int rsa_gpu_extract(rsa_gpu_ctx* ctx, double* pos, double* radius, uint64_t* phase, uint64_t cap,
                     uint64_t* n);
 
+/* rsa_gpu_extract into caller-owned DEVICE memory, allocating nothing: pos
+ * (cap*dim doubles), radius (cap doubles) and phase (cap uint64, may be NULL)
+ * are device pointers on the context's GPU; with pos NULL only *n is set.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its scratch (per-block counts) is a static device array of
+ * 4 MiB per GPU, which limits the cell grid to 2^28 cells (larger grids are
+ * refused with an error); calls that overlap on one GPU share it and race. */
+int rsa_gpu_extract_noalloc(rsa_gpu_ctx* ctx, double* pos, double* radius, uint64_t* phase, uint64_t cap,
+                            uint64_t* n);
+
 /* Total volume of the placed spheres (rsa_domain<DIM>::compute_total_volume_of_spheres,
  * rsa_domain.ixx:170-175, summed per real cell in index order). */
 int rsa_gpu_volume_of_spheres(rsa_gpu_ctx* ctx, double* volume);

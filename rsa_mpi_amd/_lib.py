@@ -18,7 +18,7 @@ EXPORTED = (
     "rsa_gpu_set_miss_rate", "rsa_gpu_proceed", "rsa_gpu_add_spheres", "rsa_gpu_count",
     "rsa_gpu_extract", "rsa_gpu_volume_of_spheres", "rsa_gpu_voxels", "rsa_gpu_mt_words",
     "rsa_gpu_profile_enable", "rsa_gpu_kernel_profile", "rsa_gpu_batch_run", "rsa_gpu_algo_init_law",
-    "rsa_gpu_reset_law",
+    "rsa_gpu_reset_law", "rsa_gpu_extract_noalloc",
 )
 
 # rsa_gpu_radius_law: void (*)(const double* u, double* radius, int64_t n, void* user)
@@ -70,6 +70,7 @@ def lib():
     L.rsa_gpu_add_spheres.argtypes = [_P, ctypes.c_int64, _D, _D, _U64]
     L.rsa_gpu_count.argtypes = [_P, _U64]
     L.rsa_gpu_extract.argtypes = [_P, _D, _D, _U64, ctypes.c_uint64, _U64]
+    L.rsa_gpu_extract_noalloc.argtypes = [_P, _P, _P, _P, ctypes.c_uint64, _U64]
     L.rsa_gpu_volume_of_spheres.argtypes = [_P, _D]
     L.rsa_gpu_voxels.argtypes = [_P, _I64, _D, ctypes.c_uint64, _U64]
     L.rsa_gpu_mt_words.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _U32]

@@ -175,6 +175,12 @@ int rsa_gpu_extract(rsa_gpu_ctx* ctx, double* pos, double* radius, uint64_t* pha
   return guarded([&] { *n = ctx->eng->extract(pos, radius, phase, cap); });
 }
 
+int rsa_gpu_extract_noalloc(rsa_gpu_ctx* ctx, double* pos, double* radius, uint64_t* phase, uint64_t cap,
+                            uint64_t* n) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { *n = ctx->eng->extract_noalloc(pos, radius, phase, cap); });
+}
+
 int rsa_gpu_volume_of_spheres(rsa_gpu_ctx* ctx, double* volume) {
   if (need_ctx(ctx)) return 1;
   return guarded([&] { *volume = ctx->eng->volume_of_spheres(); });

@@ -2499,9 +2499,9 @@ __global__ void k_cell_counts(Geo<D> g, const uint8_t* __restrict__ cnt, int* __
   if (threadIdx.x == 0) blk[blockIdx.x] = sh[0];
 }
 
-template <int D>
+template <int D, class PH>
 __global__ void k_extract(Geo<D> g, CellStore<D> cs, const int* __restrict__ blk, double* __restrict__ pos,
-                          double* __restrict__ rad, uint32_t* __restrict__ ph) {
+                          double* __restrict__ rad, PH* __restrict__ ph) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = c < g.ncells ? cs.cnt[c] : 0;
   __shared__ int sh[WG];
@@ -2519,8 +2519,34 @@ __global__ void k_extract(Geo<D> g, CellStore<D> cs, const int* __restrict__ blk
     const Slot<D> s = cs.sph[(size_t)c * cs.cap + k];
     for (int d = 0; d < D; d++) pos[(size_t)o * D + d] = s.c[d];
     rad[o] = s.r;
-    ph[o] = cs.phase[(size_t)c * cs.cap + k];
+    if (ph) ph[o] = (PH)cs.phase[(size_t)c * cs.cap + k];
   }
+}
+
+// Scratch of the allocation-free extraction (Engine::extract_noalloc): the
+// sphere count of each block of WG cells, then their total.  One copy per GPU
+// (each device holds its own instance of this array), shared by every call on
+// that GPU, so calls that overlap on one GPU race.  The host reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+constexpr int kNoAllocBlocks = 1 << 20;  // cell grids of up to 2^28 cells (4 MiB of scratch)
+static __device__ __align__(256) int scratch[kNoAllocBlocks + 64];
+
+static int* scratch_ptr() {
+  static std::mutex m;
+  static int* ptr[64] = {};
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw Error("extract_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(m);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(std::string("extract_noalloc: hipGetSymbolAddress of the scratch array failed: ") +
+                  hipGetErrorString(e));
+    ptr[dev] = static_cast<int*>(p);
+  }
+  return ptr[dev];
 }
 
 // The stencil tables (packed offsets, then linear offsets) from the launch arguments.
@@ -2930,7 +2956,8 @@ class Engine final : public IEngine {
     dp.reserve((size_t)total * D);
     dr.reserve(total);
     dph.reserve(total);
-    hipLaunchKernelGGL(k_extract<D>, dim3(nb), dim3(WG), 0, st_, geo_, cs(), blk.p, dp.p, dr.p, dph.p);
+    hipLaunchKernelGGL((k_extract<D, uint32_t>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), blk.p, dp.p, dr.p,
+                       dph.p);
     HIP_OK(hipStreamSynchronize(st_));
     HIP_OK(hipMemcpyAsync(pos, dp.p, (size_t)total * D * sizeof(double), hipMemcpyDeviceToHost, st_));
     HIP_OK(hipStreamSynchronize(st_));
@@ -2943,6 +2970,38 @@ class Engine final : public IEngine {
       for (int i = 0; i < total; i++) phase[i] = ph[i];
     }
     return (uint64_t)total;
+  }
+
+  // extract() into caller-owned device memory with no allocation: the per-block
+  // counts go to the static scratch array, the total through the engine's pinned
+  // counters (both exist before the call).  pos (cap * D), rad (cap) and phase
+  // (cap, may be null) are device pointers; with pos null only the count is returned.
+  uint64_t extract_noalloc(double* pos, double* rad, uint64_t* phase, uint64_t cap) override {
+    if (!algo_ready_) return 0;
+    const int64_t nb = nblocks(geo_.ncells);
+    if (nb > noalloc_blocks())
+      throw Error("extract_noalloc: the cell grid has " + std::to_string(geo_.ncells) + " cells; the scratch holds " +
+                  std::to_string((int64_t)noalloc_blocks() * WG) + " at most");
+    int* s = scratch_ptr();
+    hipLaunchKernelGGL(k_cell_counts<D>, dim3(nb), dim3(WG), 0, st_, geo_, cells_cnt_.p, s);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, s, (int)nb, s + kNoAllocBlocks);
+    HIP_OK(hipMemcpyAsync(hctr_ + C_CURSOR, s + kNoAllocBlocks, sizeof(int), hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipStreamSynchronize(st_));
+    const int total = hctr_[C_CURSOR];
+    if (total < 0) throw Error("extract_noalloc: sphere count exceeds 2^31");
+    if (pos == nullptr) return (uint64_t)total;
+    if ((uint64_t)total > cap) throw Error("extract: output capacity too small");
+    if (!rad) throw Error("extract_noalloc: null radius output");
+    hipLaunchKernelGGL((k_extract<D, uint64_t>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), s, pos, rad, phase);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st_));
+    return (uint64_t)total;
+  }
+  // the scratch's block limit; RSA_GPU_NOALLOC_BLOCKS lowers it (a test setting, to reach the size check)
+  static int64_t noalloc_blocks() {
+    const char* e = std::getenv("RSA_GPU_NOALLOC_BLOCKS");
+    const int64_t w = e ? std::atoll(e) : kNoAllocBlocks;
+    return std::max<int64_t>(1, std::min<int64_t>(kNoAllocBlocks, w));
   }
 
   double volume_of_spheres() override {  // rsa_grid.ixx:61-71: per real cell, cells summed in index order

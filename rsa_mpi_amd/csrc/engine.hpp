@@ -33,6 +33,7 @@ class IEngine {
   virtual void add_spheres(int64_t n, const double* pos, const double* rad, const uint64_t* phase) = 0;
   virtual uint64_t count() const = 0;
   virtual uint64_t extract(double* pos, double* rad, uint64_t* phase, uint64_t cap) = 0;
+  virtual uint64_t extract_noalloc(double* pos, double* rad, uint64_t* phase, uint64_t cap) = 0;
   virtual double volume_of_spheres() = 0;
   virtual uint64_t voxels(int64_t* coords, double* len, uint64_t cap) = 0;
   virtual double total_volume() const = 0;
