@@ -48,6 +48,27 @@ typedef struct rsa_gpu_stats {
   int64_t cover_free;       /* of those, points farther than r + r_min from every sphere */
 } rsa_gpu_stats;
 
+/* Counters of the learned predictor for the last proceed of a context (see
+ * rsa_gpu_learn_set_mode).  No counterpart in the reference. */
+typedef struct rsa_gpu_learn_stats {
+  int64_t voxels_predicted;         /* voxels whose coverage the MLP predicted (learned passes) */
+  int64_t voxels_mispredicted;      /* of those, predicted covered != covered by the exact test */
+  int64_t voxels_covered;           /* covered voxels of the measured passes (every pass in measure
+                                       or use mode) */
+  int64_t voxels_predicted_covered; /* voxels predicted covered */
+  int64_t voxels_covered_predicted; /* covered voxels of the learned passes (the constant "covered"
+                                       predictor misses voxels_predicted - this many) */
+  int64_t waves;                    /* waves of 64 voxels in the coverage kernel's walk (measured passes) */
+  int64_t waves_uniform;            /* of those, waves whose 64 voxels share one outcome */
+  int64_t draws_predicted;          /* draws whose refinement the MLP predicted (learned mode) */
+  int64_t draws_mispredicted;       /* of those, predicted refinement != the reference's decision */
+  int64_t refines;                  /* refinements of those draws */
+  int64_t refines_predicted;        /* refinements predicted */
+  int64_t speculation_skipped;      /* next draws not queued ahead because a refinement was predicted */
+  int64_t traced_voxels;            /* voxel samples appended to the trace store */
+  int64_t traced_draws;             /* draw samples appended */
+} rsa_gpu_learn_stats;
+
 /* Last error message of the calling thread ("" if none). */
 const char* rsa_gpu_last_error(void);
 
@@ -154,6 +175,48 @@ int rsa_gpu_profile_enable(rsa_gpu_ctx* ctx, int level);
  * algorithmic bytes.  Returns 2 when k is past the last entry. */
 int rsa_gpu_kernel_profile(rsa_gpu_ctx* ctx, int k, const char** name, int64_t* launches, double* seconds,
                            double* bytes);
+
+/* ---- Learned predictor (no counterpart in the reference).  A 16 -> 64 -> 64
+ * -> 2 MLP (ReLU; BF16 weights and activations, FP32 accumulation on the
+ * matrix cores) per process.  Output 0 predicts whether a voxel is covered at a
+ * refinement pass, output 1 whether a draw triggers a refinement.  It only
+ * orders work: the coverage kernel walks predicted-covered voxels first, and a
+ * predicted refinement stops the next draw being queued ahead.  Voxels are
+ * removed by the exact FP64 test and refinements follow the reference's
+ * miss-rate rule, so placed sets are unchanged. */
+
+/* Fresh parameters (seeded) and an empty trace store. */
+int rsa_gpu_learn_reset(uint64_t seed);
+
+/* Per context: trace (append voxel and draw samples of each proceed to the
+ * trace store), use (the trained model orders the coverage walk and the draw
+ * queue), measure (count outcomes and wave uniformity even without use). */
+int rsa_gpu_learn_set_mode(rsa_gpu_ctx* ctx, int trace, int use, int measure);
+
+/* The counters of the context's last proceed. */
+int rsa_gpu_learn_get_stats(rsa_gpu_ctx* ctx, rsa_gpu_learn_stats* out);
+
+/* Samples in the trace store. */
+int rsa_gpu_learn_samples(int64_t* n);
+
+/* Trains the model on the trace store on the GPU: steps of Adam (learning rate
+ * lr) on batches of `batch` samples (a multiple of 16) drawn uniformly; the
+ * loss of the first and last batch, and the samples used, are returned. */
+int rsa_gpu_learn_train(int steps, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last,
+                        int64_t* samples);
+
+/* The model's FP32 parameters (5378 floats: W1[16][64], b1[64], W2[64][64],
+ * b2[64], W3[64][2], b3[2]), read into get when non-NULL, else set from set. */
+int rsa_gpu_learn_params(float* get, const float* set);
+
+/* Test hooks: one forward and backward pass on the GPU kernels over n samples
+ * (n a multiple of 16; x is n*16 features, t is n*4 bytes y0, m0, y1, m1) with
+ * the given parameters: gradients of the mean loss into grad (5378 floats)
+ * and the loss; and the logits (n*2) of a forward pass.  The model and the
+ * trace store are left as they were. */
+int rsa_gpu_learn_gradcheck(int n, const float* x, const uint8_t* t, const float* params, float* grad,
+                            double* loss);
+int rsa_gpu_learn_forward(int n, const float* x, const float* params, float* logits);
 
 /* Raw MT19937 words of std::mt19937 seeded with seed_seq{rank, seed}
  * (rsa_random.hxx:37-42), from the library's own generator. */

@@ -16,7 +16,8 @@ from . import _lib
 from ._lib import Stats, check, dptr, f64, lib, mt_words  # noqa: F401
 
 __all__ = ["Domain", "RadiusGenerator", "LawRadiusGenerator", "Algo", "Stats", "mt_words", "uniform_generate",
-           "throw_spheres", "batch_run", "write_binary", "read_binary", "set_nb_threads"]
+           "throw_spheres", "batch_run", "write_binary", "read_binary", "set_nb_threads", "learn_reset",
+           "learn_train", "learn_samples", "learn_params", "learn_gradcheck", "learn_forward"]
 
 
 def set_nb_threads(n):
@@ -121,6 +122,17 @@ class Domain:
             out[name.value.decode()] = dict(launches=n.value, seconds=sec.value, bytes=by.value)
             k += 1
         return out
+
+    def learn_mode(self, trace=False, use=False, measure=False):
+        """The learned predictor on this domain's proceeds: trace (log voxel and draw samples
+        for training), use (order the coverage walk and the draw queue by the trained model),
+        measure (outcome and wave counters without use)."""
+        check(lib().rsa_gpu_learn_set_mode(self._ctx, int(bool(trace)), int(bool(use)), int(bool(measure))))
+
+    def learn_stats(self):
+        st = _lib.LearnStats()
+        check(lib().rsa_gpu_learn_get_stats(self._ctx, ctypes.byref(st)))
+        return st.as_dict()
 
     def voxels(self):
         """Current voxel list (int64 lattice coordinates) and the common voxel length."""
@@ -352,3 +364,61 @@ def _make_dim_aliases():
 
 
 _make_dim_aliases()
+
+
+# ------------------------------------------------------------ the learned predictor (process-wide model)
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def learn_reset(seed=1):
+    """Fresh MLP parameters and an empty trace store."""
+    check(lib().rsa_gpu_learn_reset(int(seed)))
+
+
+def learn_samples():
+    n = ctypes.c_int64()
+    check(lib().rsa_gpu_learn_samples(ctypes.byref(n)))
+    return n.value
+
+
+def learn_train(steps=400, batch=1024, lr=3e-3, seed=0):
+    """Adam on the trace store with the GPU forward/backward kernels: (first loss, last loss, samples)."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    n = ctypes.c_int64()
+    check(lib().rsa_gpu_learn_train(int(steps), int(batch), float(lr), int(seed), ctypes.byref(a), ctypes.byref(b),
+                                    ctypes.byref(n)))
+    return a.value, b.value, n.value
+
+
+def learn_params(set=None):
+    """The model's FP32 parameters (get), or install `set`."""
+    if set is not None:
+        p = np.ascontiguousarray(set, dtype=np.float32)
+        assert p.size == _lib.NPAR
+        check(lib().rsa_gpu_learn_params(None, _fptr(p)))
+        return p
+    out = np.zeros(_lib.NPAR, dtype=np.float32)
+    check(lib().rsa_gpu_learn_params(_fptr(out), None))
+    return out
+
+
+def learn_gradcheck(x, t, params):
+    """(gradients of the mean loss, loss) of one GPU forward/backward pass over the samples."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.uint8)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    g = np.zeros(_lib.NPAR, dtype=np.float32)
+    loss = ctypes.c_double()
+    check(lib().rsa_gpu_learn_gradcheck(len(x), _fptr(x), t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _fptr(p),
+                                        _fptr(g), ctypes.byref(loss)))
+    return g, loss.value
+
+
+def learn_forward(x, params):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    out = np.zeros((len(x), 2), dtype=np.float32)
+    check(lib().rsa_gpu_learn_forward(len(x), _fptr(x), _fptr(p), _fptr(out)))
+    return out

@@ -19,6 +19,8 @@ EXPORTED = (
     "rsa_gpu_extract", "rsa_gpu_volume_of_spheres", "rsa_gpu_voxels", "rsa_gpu_mt_words",
     "rsa_gpu_profile_enable", "rsa_gpu_kernel_profile", "rsa_gpu_batch_run", "rsa_gpu_algo_init_law",
     "rsa_gpu_reset_law", "rsa_gpu_extract_noalloc",
+    "rsa_gpu_learn_reset", "rsa_gpu_learn_set_mode", "rsa_gpu_learn_get_stats", "rsa_gpu_learn_samples",
+    "rsa_gpu_learn_train", "rsa_gpu_learn_params", "rsa_gpu_learn_gradcheck", "rsa_gpu_learn_forward",
 )
 
 # rsa_gpu_radius_law: void (*)(const double* u, double* radius, int64_t n, void* user)
@@ -37,6 +39,20 @@ class Stats(ctypes.Structure):
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
+
+class LearnStats(ctypes.Structure):
+    """rsa_gpu_learn_stats"""
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "voxels_predicted", "voxels_mispredicted", "voxels_covered", "voxels_predicted_covered",
+        "voxels_covered_predicted", "waves",
+        "waves_uniform", "draws_predicted", "draws_mispredicted", "refines", "refines_predicted",
+        "speculation_skipped", "traced_voxels", "traced_draws")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+NPAR = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2  # the MLP's FP32 parameters
 
 _lib = None
 
@@ -80,6 +96,16 @@ def lib():
                                         ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, ctypes.c_int]
     L.rsa_gpu_reset_law.argtypes = [_P, ctypes.c_double, ctypes.c_double, RADIUS_LAW, ctypes.c_void_p,
                                     ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double]
+    _F = ctypes.POINTER(ctypes.c_float)
+    _U8 = ctypes.POINTER(ctypes.c_uint8)
+    L.rsa_gpu_learn_reset.argtypes = [ctypes.c_uint64]
+    L.rsa_gpu_learn_set_mode.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.rsa_gpu_learn_get_stats.argtypes = [_P, ctypes.POINTER(LearnStats)]
+    L.rsa_gpu_learn_samples.argtypes = [_I64]
+    L.rsa_gpu_learn_train.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D, _D, _I64]
+    L.rsa_gpu_learn_params.argtypes = [_F, _F]
+    L.rsa_gpu_learn_gradcheck.argtypes = [ctypes.c_int, _F, _U8, _F, _F, _D]
+    L.rsa_gpu_learn_forward.argtypes = [ctypes.c_int, _F, _F, _F]
     L.rsa_gpu_batch_run.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _U64, ctypes.c_int, ctypes.c_int, _U64, _D,
                                     _D, _D, ctypes.c_uint64, _D]
     _lib = L

@@ -14,6 +14,7 @@
 
 #include "../../include/rsa_mpi_amd.h"
 #include "engine.hpp"
+#include "learn.hpp"
 #include "mt19937.hpp"
 
 struct rsa_gpu_ctx {
@@ -207,6 +208,55 @@ int rsa_gpu_kernel_profile(rsa_gpu_ctx* ctx, int k, const char** name, int64_t* 
     return 2;
   }
   return 0;
+}
+
+int rsa_gpu_learn_reset(uint64_t seed) {
+  return guarded([&] {
+    rsa_amd::learn::Model::get().ensure();
+    rsa_amd::learn::Model::get().reset(seed);
+  });
+}
+
+int rsa_gpu_learn_set_mode(rsa_gpu_ctx* ctx, int trace, int use, int measure) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { ctx->eng->learn_mode(trace, use, measure); });
+}
+
+int rsa_gpu_learn_get_stats(rsa_gpu_ctx* ctx, rsa_gpu_learn_stats* out) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] { ctx->eng->learn_stats(out); });
+}
+
+int rsa_gpu_learn_samples(int64_t* n) {
+  return guarded([&] { *n = rsa_amd::learn::Model::get().samples(); });
+}
+
+int rsa_gpu_learn_train(int steps, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last,
+                        int64_t* samples) {
+  return guarded([&] {
+    const int64_t n = rsa_amd::learn::Model::get().train(steps, batch, lr, seed, loss_first, loss_last);
+    if (samples) *samples = n;
+  });
+}
+
+int rsa_gpu_learn_params(float* get, const float* set) {
+  return guarded([&] {
+    if (get)
+      rsa_amd::learn::Model::get().get_params(get);
+    else if (set)
+      rsa_amd::learn::Model::get().set_params(set);
+    else
+      throw rsa_amd::Error("learn: get or set must be given");
+  });
+}
+
+int rsa_gpu_learn_gradcheck(int n, const float* x, const uint8_t* t, const float* params, float* grad,
+                            double* loss) {
+  return guarded([&] { rsa_amd::learn::Model::get().gradcheck(n, x, t, params, grad, loss); });
+}
+
+int rsa_gpu_learn_forward(int n, const float* x, const float* params, float* logits) {
+  return guarded([&] { rsa_amd::learn::Model::get().forward(n, x, params, logits); });
 }
 
 int rsa_gpu_mt_words(uint64_t seed, uint64_t rank, uint64_t n, uint32_t* out) {

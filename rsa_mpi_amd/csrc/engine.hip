@@ -39,6 +39,8 @@
 
 #include "device_common.hpp"
 #include "engine.hpp"
+#include "learn.hpp"
+#include "mlp.hpp"
 #include "mt19937.hpp"
 
 namespace rsa_amd {
@@ -247,6 +249,8 @@ enum Ctr : int {
   C_MOVED,        // voxels k_move moved into holes (the compaction's traffic)
   C_OVERLAP,      // end-of-run check: colliding (sphere, neighbour) pairs, each pair seen from both sides
   C_UNCOVERED,    // end-of-run check: sample points a sphere of the smallest radius could still take
+  C_L0,           // k_outcomes: predicted, mispredicted, covered, predicted covered, waves, uniform waves
+  C_L5 = C_L0 + 5,
   C_NUM
 };
 
@@ -2169,10 +2173,14 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
 // (a counting sort), so that the lanes of a wave walk the same stencil cells.
 // The flags do not depend on the order; the grouping only makes the walk
 // uniform across lanes and the cell reads shared.
+// With a prediction (pred not null, one byte per voxel, 1 = predicted covered)
+// the key also puts every predicted-uncovered voxel after the predicted-covered
+// ones (keys ncells .. 2 ncells - 1), so that a wave's voxels tend to share their
+// outcome as well as their cells.
 template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) k_bin(Geo<D> g, VoxView<D> vx, int64_t nvox, VoxParams<D> vp,
                                             int* __restrict__ ccount, int* __restrict__ vcell,
-                                            int* __restrict__ vrank) {
+                                            int* __restrict__ vrank, const uint8_t* __restrict__ pred) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool valid = i < nvox;
@@ -2187,6 +2195,7 @@ __global__ void __launch_bounds__(WG) k_bin(Geo<D> g, VoxView<D> vx, int64_t nvo
     }
     const int cell = cell_of_point<D>(g, p, cc);
     key = (cell >= 0 && cell < g.ncells) ? cell : 0;  // a key only: any value is correct
+    if (pred && !pred[i]) key += g.ncells;
   }
   // one atomic per run of equal keys in adjacent lanes (siblings share a cell)
   const int prev = __shfl_up(key, 1, 64);
@@ -2213,6 +2222,182 @@ __global__ void __launch_bounds__(WG) k_place(int64_t nvox, const int* __restric
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nvox) return;
   perm[cstart[vcell[i]] + vrank[i]] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------- learned predictor: voxels
+//
+// The 16 features of a voxel (the predictor's inputs), from its coordinates and
+// the cell store only: the position of its centre in its cell, its size
+// relative to the cell, the occupancy of its cell and of the cell's face
+// neighbours, the distance to the nearest sphere of its own cell, the
+// refinement level and two per-pass constants (placed spheres per cell, the
+// last miss rate); the last feature is 0 for voxel samples.
+template <int D, bool WIDE>
+__device__ __forceinline__ void voxel_features(const Geo<D>& g, const CellStore<D>& cs, const VoxView<D>& vx,
+                                               const VoxParams<D>& vp, int64_t i, float level, float density,
+                                               float miss, float* f) {
+  double p[D];
+  int cc[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int64_t c = WIDE ? vx.w[d][i] : (int64_t)vx.c[d][i];
+    p[d] = vp.origin[d] + ((double)c + 0.5) * vp.len[d];
+  }
+  int cell = cell_of_point<D>(g, p, cc);
+  if (cell < 0 || cell >= g.ncells) cell = 0;
+  f[0] = f[1] = f[2] = 0.5f;
+#pragma unroll
+  for (int d = 0; d < (D < 3 ? D : 3); d++) f[d] = (float)((p[d] - g.cshift[d]) * g.cinv[d] - cc[d]);
+  f[3] = (float)(vp.len[0] * g.cinv[0]);
+  const int own = cs.cnt[cell];
+  f[4] = own * 0.125f;
+  int stride = 1, k = 5;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int lo = cc[d] > 0 ? cell - stride : cell, hi = cc[d] < g.n[d] - 1 ? cell + stride : cell;
+    if (k < 11) f[k++] = cs.cnt[lo] * 0.125f;
+    if (k < 11) f[k++] = cs.cnt[hi] * 0.125f;
+    stride *= g.n[d];
+  }
+  while (k < 11) f[k++] = 0.f;
+  double best = 4.;
+  for (int q = 0; q < own; q++) {
+    const Slot<D> sl = cs.sph[(size_t)cell * cs.cap + q];
+    double s2 = 0.;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const double t = sl.c[d] - p[d];
+      s2 += t * t;
+    }
+    const double rr = sl.r + vp.rmin;
+    best = fmin(best, s2 / (rr * rr));
+  }
+  f[11] = (float)(best < 4. ? sqrt(best) : 2.);
+  f[12] = level * (1.f / 32.f);
+  f[13] = density;
+  f[14] = miss;
+  f[15] = 0.f;
+}
+
+// Trace of a refinement pass: every stride-th voxel's features and its
+// outcome (covered: code 0), appended to the model's trace store.
+template <int D, bool WIDE>
+__global__ void __launch_bounds__(WG) k_trace_voxels(Geo<D> g, CellStore<D> cs, VoxView<D> vx, VoxParams<D> vp,
+                                                     int64_t nvox, int64_t stride, const uint32_t* __restrict__ code,
+                                                     float level, float density, float miss,
+                                                     uint16_t* __restrict__ tx, uint8_t* __restrict__ tt,
+                                                     int* __restrict__ cursor, int cap) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = k * stride;
+  const bool valid = i < nvox;
+  const uint64_t b = __ballot(valid);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0 && b) base = atomicAdd(cursor, (int)__popcll(b));
+  base = __shfl(base, 0, 64);
+  if (!valid) return;
+  const int slot = base + (int)__popcll(b & ((1ull << lane) - 1ull));
+  if (slot >= cap) return;
+  float f[mlp::NF];
+  voxel_features<D, WIDE>(g, cs, vx, vp, i, level, density, miss, f);
+#pragma unroll
+  for (int q = 0; q < mlp::NF; q++) tx[(size_t)slot * mlp::NF + q] = mlp::f2bf(f[q]);
+  tt[(size_t)slot * 4 + 0] = code[i] == 0 ? 1 : 0;
+  tt[(size_t)slot * 4 + 1] = 1;
+  tt[(size_t)slot * 4 + 2] = 0;
+  tt[(size_t)slot * 4 + 3] = 0;
+}
+
+// The prediction pass: each wave takes 64 voxels (4 tiles of 16), computes their
+// features lane by lane into LDS and runs the MLP on the matrix cores;
+// pred[i] = 1 when the coverage logit is positive.
+constexpr int PW = WG / 64;  // waves per workgroup
+template <int D, bool WIDE>
+__global__ void __launch_bounds__(WG) k_predict(Geo<D> g, CellStore<D> cs, VoxView<D> vx, VoxParams<D> vp,
+                                                int64_t nvox, float level, float density, float miss,
+                                                const float* __restrict__ P, const uint16_t* __restrict__ Wq,
+                                                uint8_t* __restrict__ pred) {
+  using namespace mlp;
+  __shared__ uint16_t sX[PW][64 * LDS_STRIDE];
+  __shared__ uint16_t sH1[PW][16 * LDS_STRIDE], sH2[PW][16 * LDS_STRIDE];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t base = ((int64_t)blockIdx.x * PW + w) * 64;
+  if (base >= nvox) return;  // whole waves only: the MFMA passes below need all 64 lanes
+  Frags fr;
+  load_frags(Wq, fr, lane);
+  {
+    float f[NF];
+    const int64_t i = base + lane;
+    if (i < nvox) {
+      voxel_features<D, WIDE>(g, cs, vx, vp, i, level, density, miss, f);
+    } else {
+#pragma unroll
+      for (int q = 0; q < NF; q++) f[q] = 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < NF; q++) sX[w][lane * LDS_STRIDE + q] = f2bf(f[q]);
+  }
+  wave_lds_sync();
+  const int gq = lane >> 4, c = lane & 15;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const floatx4 z3 = forward_tile(fr, P, sX[w] + t * 16 * LDS_STRIDE, sH1[w], sH2[w], lane);
+    if (c == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int64_t i = base + t * 16 + 4 * gq + j;
+        if (i < nvox) pred[i] = z3[j] > 0.f ? 1 : 0;
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+// Outcome counters of a pass (learned mode or measured passes): per workgroup of
+// 1024 threads x 16 voxels, the confusion counts of pred against the outcome
+// (when pred is not null) and, over k_cover's processing order (perm, or the
+// list order), the waves of 64 voxels and those whose voxels all share one
+// outcome.  ctr: [0] predicted, [1] mispredicted, [2] covered, [3] predicted
+// covered, [4] waves, [5] uniform waves.
+__global__ void __launch_bounds__(1024) k_outcomes(int64_t nvox, const uint32_t* __restrict__ code,
+                                                   const uint8_t* __restrict__ pred,
+                                                   const uint32_t* __restrict__ perm, int* __restrict__ ctr) {
+  int wrong = 0, cov = 0, pcov = 0, n = 0, waves = 0, uniform = 0;
+  const int lane = threadIdx.x & 63;
+  for (int r = 0; r < 16; r++) {
+    const int64_t t = ((int64_t)blockIdx.x * 16 + r) * 1024 + threadIdx.x;
+    const bool valid = t < nvox;
+    bool covered = false;
+    if (valid) {
+      const int64_t i = perm ? (int64_t)perm[t] : t;
+      covered = code[i] == 0;
+      n++;
+      cov += covered;
+      if (pred) {
+        pcov += pred[i] != 0;
+        wrong += (pred[i] != 0) != covered;
+      }
+    }
+    const uint64_t vb = __ballot(valid), cb = __ballot(covered);
+    if (lane == 0 && vb == ~0ull) {
+      waves++;
+      uniform += (cb == 0 || cb == ~0ull);
+    }
+  }
+  // the workgroup's sums, then one atomic per counter and workgroup
+  __shared__ int s_sum[6];
+  if (threadIdx.x < 6) s_sum[threadIdx.x] = 0;
+  __syncthreads();
+  const int v[6] = {pred ? n : 0, wrong, cov, pcov, waves, uniform};
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    int x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0 && x) atomicAdd(&s_sum[k], x);
+  }
+  __syncthreads();
+  if (threadIdx.x < 6 && s_sum[threadIdx.x]) atomicAdd(&ctr[threadIdx.x], s_sum[threadIdx.x]);
 }
 
 // uncovered voxels per block of WG (the blocks of the list order)
@@ -3054,6 +3239,15 @@ class Engine final : public IEngine {
 
   void proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* out) override;
 
+  // learned predictor: trace logging, use (ordering and speculation), outcome measurement
+  void learn_mode(int trace, int use, int measure) override {
+    learn_trace_ = trace != 0;
+    learn_use_ = use != 0;
+    learn_measure_ = measure != 0;
+    if (learn_trace_ || learn_use_) learn::Model::get().ensure();
+  }
+  void learn_stats(rsa_gpu_learn_stats* out) override { *out = lst_; }
+
   void profile_enable(int level) override {
     const bool on = level > 0;
     StreamScope scope(st_, cst_);
@@ -3325,6 +3519,8 @@ class Engine final : public IEngine {
     for (int d = 0; d < D; d++) nc *= geo_.n[d];
     if (nc > (int64_t)1 << 30) throw Error("cell grid too large");
     geo_.ncells = (int)nc;
+    real_cells_ = 1;
+    for (int d = 0; d < D; d++) real_cells_ *= geo_.n[d] - 2 * gl_;
     // stencil order (rsa_grid_traversal.ixx:30-66): libstdc++ std::sort by squared norm
     const int n3 = ipow_c(3, D);
     std::vector<int> order(n3);
@@ -3688,21 +3884,41 @@ class Engine final : public IEngine {
                                (double)nb_fast_ * sizeof(Slot<D>) + geo_.ncells;
     // long lists are walked in cell-grouped order
     const bool grouped = nvox_ >= kGroupMin;
+    // learned mode: the MLP predicts each voxel's coverage, and the walk order puts the
+    // predicted-covered voxels first (within each group, the cell-grouped order)
+    const bool predicted = grouped && learn_use_ && learn::Model::get().trained();
+    const float density = (float)((double)(nb_fast_ + added_) / (double)std::max<int64_t>(1, real_cells_)),
+                miss = (float)last_miss_, level = (float)level_;
+    if (predicted) {
+      pred_.reserve(nvox_);
+      const learn::Model& m = learn::Model::get();
+      timed("k_predict", (double)nvox_ * ((wide_[vcur_] ? 8.0 : 4.0) * D + 1.0), [&] {
+        const unsigned nbp = nblocks(nvox_);
+        if (wide_[vcur_])
+          hipLaunchKernelGGL((k_predict<D, true>), dim3(nbp), dim3(WG), 0, st_, geo_, cs(), vx, vp, nvox_, level,
+                             density, miss, m.P(), m.Wq(), pred_.p);
+        else
+          hipLaunchKernelGGL((k_predict<D, false>), dim3(nbp), dim3(WG), 0, st_, geo_, cs(), vx, vp, nvox_, level,
+                             density, miss, m.P(), m.Wq(), pred_.p);
+      }, 1);
+    }
     if (grouped) {
+      const int64_t nkeys = (predicted ? 2 : 1) * (int64_t)geo_.ncells;
       perm_.reserve(nvox_);
       vcell_.reserve(nvox_);
       vrank_.reserve(nvox_);
-      ccount_.reserve(geo_.ncells);
-      HIP_OK(hipMemsetAsync(ccount_.p, 0, sizeof(int) * (size_t)geo_.ncells, st_));
+      ccount_.reserve(nkeys);
+      HIP_OK(hipMemsetAsync(ccount_.p, 0, sizeof(int) * (size_t)nkeys, st_));
+      const uint8_t* pr = predicted ? pred_.p : nullptr;
       timed("k_bin", (double)nvox_ * ((wide_[vcur_] ? 8.0 : 4.0) * D + 8.0), [&] {
         if (wide_[vcur_])
           hipLaunchKernelGGL((k_bin<D, true>), dim3(nb), dim3(WG), 0, st_, geo_, vx, nvox_, vp, ccount_.p, vcell_.p,
-                             vrank_.p);
+                             vrank_.p, pr);
         else
           hipLaunchKernelGGL((k_bin<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, vx, nvox_, vp, ccount_.p, vcell_.p,
-                             vrank_.p);
+                             vrank_.p, pr);
       }, 1);
-      scan_excl(ccount_.p, geo_.ncells, dctr_.p + C_GROUPED);
+      scan_excl(ccount_.p, nkeys, dctr_.p + C_GROUPED);
       timed("k_place", (double)nvox_ * 12.0, [&] {
         hipLaunchKernelGGL(k_place, dim3(nb), dim3(WG), 0, st_, nvox_, vcell_.p, vrank_.p, ccount_.p, perm_.p);
       }, 1);
@@ -3736,6 +3952,27 @@ class Engine final : public IEngine {
                            csph, ncs, code_.p, blk_.p);
     }, 1);
     if (grouped) hipLaunchKernelGGL(k_block_count, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p);
+    // the predictor's outcome counters, and its traces
+    const bool measured = learn_use_ || learn_measure_;
+    if (measured) {
+      HIP_OK(hipMemsetAsync(dctr_.p + C_L0, 0, 6 * sizeof(int), st_));
+      hipLaunchKernelGGL(k_outcomes, dim3(nblocks(nvox_, 16384)), dim3(1024), 0, st_, nvox_, code_.p,
+                         predicted ? pred_.p : (const uint8_t*)nullptr, grouped ? perm_.p : (const uint32_t*)nullptr,
+                         dctr_.p + C_L0);
+    }
+    if (learn_trace_) {
+      learn::Model& m = learn::Model::get();
+      const int64_t stride = std::max<int64_t>(1, nvox_ / kTraceVoxels);
+      const int64_t nt = (nvox_ + stride - 1) / stride;
+      if (wide_[vcur_])
+        hipLaunchKernelGGL((k_trace_voxels<D, true>), dim3(nblocks(nt)), dim3(WG), 0, st_, geo_, cs(), vx, vp, nvox_,
+                           stride, code_.p, level, density, miss, m.tx(), m.tt(), m.cursor(), learn::kTraceCap);
+      else
+        hipLaunchKernelGGL((k_trace_voxels<D, false>), dim3(nblocks(nt)), dim3(WG), 0, st_, geo_, cs(), vx, vp,
+                           nvox_, stride, code_.p, level, density, miss, m.tx(), m.tt(), m.cursor(),
+                           learn::kTraceCap);
+      lst_.traced_voxels += nt;
+    }
     scan_excl(blk_.p, nb, dctr_.p + C_KEEP);
     HIP_OK(hipMemsetAsync(dctr_.p + C_MOVED, 0, sizeof(int), st_));
     // the compaction's bytes depend on the number of moved voxels (left by k_holes):
@@ -3754,6 +3991,16 @@ class Engine final : public IEngine {
     scan_excl(blk_.p, nb, dctr_.p + C_NCHILD);
     read_ctr();
     check_scatter();  // the previous refinement's children
+    if (measured) {
+      lst_.voxels_predicted += hctr_[C_L0];
+      lst_.voxels_mispredicted += hctr_[C_L0 + 1];
+      lst_.voxels_covered += hctr_[C_L0 + 2];
+      lst_.voxels_predicted_covered += hctr_[C_L0 + 3];
+      if (predicted) lst_.voxels_covered_predicted += hctr_[C_L0 + 2];
+      lst_.waves += hctr_[C_L0 + 4];
+      lst_.waves_uniform += hctr_[C_L0 + 5];
+    }
+    level_++;
     if (pend_.size() == holes_at + 2) {
       // k_holes: codes and block offsets read, one index per hole written; k_move: the
       // same reads, then per moved voxel its coordinates read and written and its code
@@ -3859,6 +4106,36 @@ class Engine final : public IEngine {
   DevBuf<URec> urec_;
   DevBuf<uint8_t> owner_;
   DevBuf<int> stab_;  // packed stencil offsets (Geo::stab), then linear ones (Geo::slin)
+
+  // learned predictor
+  static constexpr int64_t kTraceVoxels = 8192;  // voxels traced per refinement pass
+  bool learn_trace_ = false, learn_use_ = false, learn_measure_ = false;
+  DevBuf<uint8_t> pred_;
+  rsa_gpu_learn_stats lst_{};
+  int level_ = 0;            // refinements so far in this proceed
+  double last_miss_ = 0.;    // the last draw's miss rate
+  int since_refine_ = 0;     // draws since the last refinement
+  int64_t real_cells_ = 1;
+  std::vector<double> miss_hist_;
+  std::vector<float> draw_x_;
+  std::vector<uint8_t> draw_t_;
+  // the draw head's inputs for the coming draw (its refinement is what is predicted):
+  // the recent miss rates and their trend, draws since the last refinement, the shot
+  // count and Poisson intensity relative to nbshots, the voxel count, the refinements so
+  // far and the margin below the aimed miss rate; the last feature marks a draw sample
+  void draw_features(float* x, int64_t nshots) const {
+    for (int k = 0; k < mlp::NF; k++) x[k] = 0.f;
+    const int n = (int)miss_hist_.size();
+    for (int k = 0; k < 4; k++) x[k] = k < n ? (float)miss_hist_[n - 1 - k] : 0.f;
+    x[4] = x[0] - x[1];
+    x[5] = since_refine_ / 64.f;
+    x[6] = (float)nshots / (float)std::max(1, nbshots_);
+    x[7] = (float)(0.01 * (double)nvox_ / (double)std::max(1, nbshots_));
+    x[8] = (float)(std::log2((double)nvox_ + 1.0) / 32.0);
+    x[9] = level_ / 32.f;
+    x[10] = (float)aimed_ - x[0];
+    x[15] = 1.f;
+  }
 };
 
 // median_of_medians::find_pivot_for_first_elements on one rank (median_of_medians.ixx:8-178)
@@ -4379,6 +4656,15 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
   Prep cur;               // the next draw's inputs (valid while cur.nv == nvox_ and cur.w_start == w)
   bool cur_live = false;  // its first round is already queued (behind the previous draw)
   uint64_t w = 0;
+  lst_ = rsa_gpu_learn_stats{};
+  level_ = 0;
+  last_miss_ = 0.;
+  since_refine_ = 0;
+  miss_hist_.clear();
+  draw_x_.clear();
+  draw_t_.clear();
+  // the draw head predicts refinements only with a trained model
+  const bool draw_head = learn_use_ && learn::Model::get().trained();
   init_voxels(r_max_ + gmin_);
   st->nvox_max = nvox_;
   uint64_t old_nb = 0;
@@ -4407,6 +4693,11 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     const uint64_t nshots = (uint64_t)cur.nshots;
     const uint64_t maximal = current_number();
     const bool may_outreach = nshots >= maximal;
+    // the draw head: will this draw trigger a refinement? (its inputs are known before the draw)
+    float dx[mlp::NF];
+    draw_features(dx, (int64_t)nshots);
+    bool predicted_refine = false;
+    if (draw_head) predicted_refine = learn::host_logit(learn::Model::get().host_params(), dx, 1) > 0.f;
     int64_t rounds = 0;
     uint64_t nb_new = 0;
     bool fast = false, gate_open = false;
@@ -4442,7 +4733,10 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
       // Queue the next draw behind this one.  Its inputs hold unless this draw
       // triggers a refinement (the device then keeps its gate closed); it cannot
       // change the class or overshoot its target: shots(cur) + shots(nxt) < maximal.
-      if (max_draws < 0 || st->draws + 2 <= max_draws) {
+      // With the draw head predicting a refinement, the next draw is not queued ahead
+      // (its gate would stay closed); the device still decides, as always.
+      if (predicted_refine) lst_.speculation_skipped++;
+      if ((max_draws < 0 || st->draws + 2 <= max_draws) && !predicted_refine) {
         nxt = host_timed("host_prepare_next", [&] { return prep_get(cur.w_end(), nvox_); });
         if (nxt.nshots <= 48000 && (size_t)nxt.nshots <= cand_cap_ && nshots + (uint64_t)nxt.nshots < maximal) {
           enqueue_first(nxt, false, 0);  // the miss count restarts after each draw
@@ -4466,6 +4760,21 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     const double miss_rate = nb_miss / (0.000001 + nshots);
     const bool refine_now = miss_rate > aimed_;
     if (fast && gate_open == refine_now) throw Error("internal: device and host refinement decisions differ");
+    if (draw_head) {
+      lst_.draws_predicted++;
+      lst_.draws_mispredicted += predicted_refine != refine_now;
+      lst_.refines += refine_now;
+      lst_.refines_predicted += predicted_refine;
+    }
+    if (learn_trace_) {
+      draw_x_.insert(draw_x_.end(), dx, dx + mlp::NF);
+      const uint8_t t[4] = {0, 0, (uint8_t)(refine_now ? 1 : 0), 1};
+      draw_t_.insert(draw_t_.end(), t, t + 4);
+      lst_.traced_draws++;
+    }
+    miss_hist_.push_back(miss_rate);
+    last_miss_ = miss_rate;
+    since_refine_ = refine_now ? 0 : since_refine_ + 1;
     if (refine_now) {
       nxt_live = false;  // its gate stayed closed
       HIP_OK(hipEventRecord(e0, st_));
@@ -4487,6 +4796,10 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     cur_live = nxt_live;
   }
   if (prof_) prof_add("host_wait_mt_words", 1e-9 * (double)hs_.wait_ns.load(), 0);
+  if (learn_trace_ && !draw_t_.empty()) {
+    HIP_OK(hipStreamSynchronize(st_));  // the voxel traces of this run are in before the draw samples
+    learn::Model::get().add_samples((int64_t)draw_t_.size() / 4, draw_x_.data(), draw_t_.data());
+  }
   // end-of-run checks on the device (every run): no overlap, and no free point
   // left for a sphere of the smallest radius
   const int64_t nsamp = std::min<int64_t>(int64_t(1) << 22, std::max<int64_t>(int64_t(1) << 16, 4 * (int64_t)nb_fast_));

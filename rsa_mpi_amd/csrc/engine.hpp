@@ -37,6 +37,9 @@ class IEngine {
   virtual double volume_of_spheres() = 0;
   virtual uint64_t voxels(int64_t* coords, double* len, uint64_t cap) = 0;
   virtual double total_volume() const = 0;
+  // learned predictor (learn.hip): trace logging, use for ordering and speculation, outcome counts
+  virtual void learn_mode(int trace, int use, int measure) = 0;
+  virtual void learn_stats(rsa_gpu_learn_stats* out) = 0;
   // per-kernel device timings (HIP events on the context's stream)
   virtual void profile_enable(int level) = 0;  // 0 off, 1 refine kernels, 2 all
   virtual bool profile_get(int k, const char** name, int64_t* launches, double* seconds, double* bytes) = 0;
