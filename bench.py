@@ -7,7 +7,9 @@ cube, seed 0, nbshots_singledraw 6000 (11 463 041 spheres, 772 M voxels at the
 peak).  Inputs are synthetic and generated from the seed; the timed region is
 proceed() only (domain and buffers are set up before it).  The placed set of
 the last timed step is checked by SHA-256 against the oracle's fixture.  A
-secondary line runs the r=0.005 cube the same way.  Prints one JSON line.
+secondary line runs the r=0.005 cube the same way, and the learned form (an
+MLP trained on the GPU orders the work; the placed set is unchanged) runs the
+headline beside the plain one.  Prints one JSON line.
 """
 import argparse
 import hashlib
@@ -66,12 +68,14 @@ def profile_figures(name):
     return out
 
 
-def run_gpu_step(R, w, profile_level=1):
+def run_gpu_step(R, w, profile_level=1, learn=None):
     dom = R.Domain([0.0] * w["dim"], [w["L"]] * w["dim"], w["r"])
     gen = R.RadiusGenerator([(w["r"], 1.0, 0)], dom.get_total_volume(), 0.0)
     algo = R.Algo(dom, gen, w["nbshots"])
     if profile_level:
         dom.profile(profile_level)
+    if learn:
+        dom.learn_mode(**learn)
     t0 = time.perf_counter()
     st = algo.proceed(w["seed"])
     dt = time.perf_counter() - t0
@@ -86,14 +90,15 @@ def parity(dom, w):
     return "sha256 match" if ok else "MISMATCH"
 
 
-def measure(R, w, steps, warmup):
+def measure(R, w, steps, warmup, learn=None):
     """warmup untimed steps, then `steps` timed ones; per-kernel event profile (refine
-    kernels) of the timed steps and the parity of the last one."""
+    kernels) of the timed steps, the parity of the last one and, in a learned mode, its
+    predictor counters."""
     for _ in range(warmup):
-        run_gpu_step(R, w, profile_level=0)
+        run_gpu_step(R, w, profile_level=0, learn=learn)
     total_n, total_t, prof, stats, dom = 0, 0.0, {}, None, None
     for s in range(steps):
-        dom, st, dt = run_gpu_step(R, w, profile_level=1)
+        dom, st, dt = run_gpu_step(R, w, profile_level=1, learn=learn)
         print(f"#   step {s}: {st['accepted']} spheres in {dt:.3f} s", file=sys.stderr, flush=True)
         total_n += st["accepted"]
         total_t += dt
@@ -103,8 +108,60 @@ def measure(R, w, steps, warmup):
             for f in a:
                 a[f] += v[f]
     par = parity(dom, w)
+    ls = dom.learn_stats() if learn else None
     del dom
-    return dict(value=total_n / total_t, ms=1e3 * total_t / steps, prof=prof, stats=stats, parity=par)
+    return dict(value=total_n / total_t, ms=1e3 * total_t / steps, prof=prof, stats=stats, parity=par, learn=ls)
+
+
+# FLOP of one sample through the 16 -> 64 -> 64 -> 2 MLP (forward); forward + backward is 3x
+MLP_FLOP = 2 * (16 * 64 + 64 * 64 + 64 * 2)
+BF16_PEAK_TFLOPS = 2500.0
+TRACE_RADII = (0.01, 0.007, 0.005)
+
+
+def learned(R, steps, plain):
+    """The learned form on the headline workload: the model is trained on the GPU from the
+    traces of three smaller runs (r = 0.01, 0.007, 0.005 in the unit cube), then the headline
+    runs with the model ordering the coverage walk and the draw queue.  Reported beside the
+    plain form: spheres/s, the parity of the placed set, the misprediction rates (and the
+    constant predictor's), the uniform-wave fractions, k_predict's matrix-core rate."""
+    R.learn_reset(5)
+    t0 = time.perf_counter()
+    for r in TRACE_RADII:
+        run_gpu_step(R, dict(HEADLINE, r=r, golden=None), profile_level=0, learn=dict(trace=True))
+    t_trace = time.perf_counter() - t0
+    nsamp = R.learn_samples()
+    train_steps, batch = 600, 2048
+    t0 = time.perf_counter()
+    first, last, _ = R.learn_train(steps=train_steps, batch=batch, lr=3e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    print(f"#   trained on {nsamp} samples: loss {first:.4f} -> {last:.4f} in {t_train:.2f} s",
+          file=sys.stderr, flush=True)
+    base = measure(R, HEADLINE, 1, 0, learn=dict(measure=True))["learn"]
+    m = measure(R, HEADLINE, steps, 0, learn=dict(use=True))
+    ls, prof = m["learn"], m["prof"]
+    kp = prof.get("k_predict", {})
+    vp = ls["voxels_predicted"]
+    out = {"value": round(m["value"], 1), "unit": "spheres/s", "ms_per_step": round(m["ms"], 2),
+           "vs_plain": round(m["value"] / plain, 3) if plain else None, "parity_vs_oracle": m["parity"],
+           "trained_on": f"{nsamp} samples traced from r={list(TRACE_RADII)} ({t_trace:.2f} s)",
+           "train": dict(steps=train_steps, batch=batch, seconds=round(t_train, 3), loss_first=round(first, 4),
+                         loss_last=round(last, 4),
+                         tflops=round(3 * MLP_FLOP * train_steps * batch / t_train / 1e12, 3)),
+           "voxel_misprediction": round(ls["voxels_mispredicted"] / max(1, vp), 4),
+           "constant_predictor_misprediction": round((vp - ls["voxels_covered_predicted"]) / max(1, vp), 4),
+           "draw_misprediction": round(ls["draws_mispredicted"] / max(1, ls["draws_predicted"]), 4),
+           "refines": ls["refines"], "refines_predicted": ls["refines_predicted"],
+           "speculation_skipped": ls["speculation_skipped"],
+           "uniform_waves_plain": round(base["waves_uniform"] / max(1, base["waves"]), 4),
+           "uniform_waves_learned": round(ls["waves_uniform"] / max(1, ls["waves"]), 4),
+           "profile": per_kernel(prof, steps)}
+    if kp.get("seconds"):
+        per_step = kp["seconds"] / steps  # every step predicts the same voxels
+        tf = vp * MLP_FLOP / per_step / 1e12
+        out["k_predict"] = dict(ms=round(1e3 * per_step, 3), voxels=vp, flop_per_voxel=MLP_FLOP,
+                                tflops=round(tf, 2), mfma_frac=round(tf / BF16_PEAK_TFLOPS, 4))
+    return out
 
 
 def phases(stats):
@@ -214,7 +271,8 @@ def main():
                     help="realizations run at once in the batched measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
     ap.add_argument("--no-secondary", action="store_true", help="skip the r=0.005 line")
-    ap.add_argument("--only", choices=["headline", "secondary"], default=None,
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned form of the headline")
+    ap.add_argument("--only", choices=["headline", "secondary", "learned"], default=None,
                     help="run one workload's timed steps and nothing else (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (reference reads for rocprofv3 "
@@ -236,6 +294,9 @@ def main():
 
     import rsa_mpi_amd as R
     R.lib()
+    if args.only == "learned":
+        print(json.dumps(learned(R, args.steps, None)), flush=True)
+        return
     if args.only:
         w = HEADLINE if args.only == "headline" else SECONDARY
         m = measure(R, w, args.steps, args.warmup)
@@ -257,6 +318,10 @@ def main():
                "refines": m["stats"]["refines"], "parity_vs_oracle": m["parity"], "phases_ms": phases(m["stats"]),
                "roofline_k_cover_frac": (roofline(m["prof"], args.steps, {}) or {}).get("frac"),
                "profile": per_kernel(m["prof"], args.steps)}
+    lrn = None
+    if not args.no_learned:
+        print("# learned", file=sys.stderr, flush=True)
+        lrn = learned(R, args.steps, head["value"])
     bat = None
     if not args.no_batch:
         print("# batched", file=sys.stderr, flush=True)
@@ -278,6 +343,7 @@ def main():
         "phases_ms": phases(stats),
         "profile": per_kernel(head["prof"], args.steps),
         "secondary": sec,
+        "learned": lrn,
         "batched": bat,
         "cpu_baseline": cpu,
     }
