@@ -301,7 +301,8 @@ def main():
         w = HEADLINE if args.only == "headline" else SECONDARY
         m = measure(R, w, args.steps, args.warmup)
         print(json.dumps({"workload": w["name"], "value": round(m["value"], 1), "ms_per_step": round(m["ms"], 2),
-                          "parity_vs_oracle": m["parity"], "profile": per_kernel(m["prof"], args.steps)}),
+                          "parity_vs_oracle": m["parity"], "phases_ms": phases(m["stats"]),
+                          "profile": per_kernel(m["prof"], args.steps)}),
               flush=True)
         return
 

@@ -4070,7 +4070,13 @@ class Engine final : public IEngine {
   int host_chain(int nshots, uint64_t w0, uint32_t N, int b, RejList* rej);
   Prep prepare(uint64_t w, int64_t nvox, int b);
   // draw preparation on a helper thread (prep_start / prep_get / prep_halt)
-  static constexpr int kPrepAhead = 4, kPrepBufs = 8, kFastShots = 48000;
+  // kQueueDepth draws queued behind the current one (deeper queues measured no
+  // faster: the draw kernels keep the device busy already, and every conflict
+  // round or refinement wastes the launches of all queued draws); pinned offset
+  // buffers for the current draw, the queued ones and the kPrepAhead the helper
+  // prepares, with room to spare
+  static constexpr int kPrepAhead = 4, kQueueDepth = 1, kPrepBufs = 12, kFastShots = 48000;
+  static_assert(kPrepBufs > 1 + kQueueDepth + kPrepAhead + 1, "offset buffers in use at once");
   void prep_start();
   void prep_halt();
   Prep prep_get(uint64_t w, int64_t nv);
@@ -4655,6 +4661,10 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
   } halt_at_exit{*this};
   Prep cur;               // the next draw's inputs (valid while cur.nv == nvox_ and cur.w_start == w)
   bool cur_live = false;  // its first round is already queued (behind the previous draw)
+  // the draws prepared after cur, in stream order; the first `live` of them have
+  // their first round queued behind cur (each gated on the one before it)
+  std::deque<Prep> ahead;
+  int live = 0;
   uint64_t w = 0;
   lst_ = rsa_gpu_learn_stats{};
   level_ = 0;
@@ -4687,6 +4697,8 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     hs_.set_low(w);
     if (cur.nv != nvox_ || cur.w_start != w) {
       if (cur_live) throw Error("internal: a queued draw lost its inputs");
+      ahead.clear();  // prepared for another voxel list
+      live = 0;
       cur = host_timed("host_prepare", [&] { return prep_get(w, nvox_); });
     }
     st->loop_iterations += cur.zero_iters;  // the iterations whose Poisson draw gave no shot
@@ -4701,10 +4713,11 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     int64_t rounds = 0;
     uint64_t nb_new = 0;
     bool fast = false, gate_open = false;
-    Prep nxt;
-    bool nxt_live = false;
+    // whether the draws queued behind cur are still good (no kernel of cur came after them)
+    bool spec_ok = true;
     if (may_outreach || nshots > 48000) {
-      if (cur_live) throw Error("internal: a queued draw needs the host-driven form");
+      if (cur_live || live) throw Error("internal: a queued draw needs the host-driven form");
+      ahead.clear();
       uint64_t w_next = 0;
       const double* hrad = nullptr;
       if (cur.law) {
@@ -4730,20 +4743,30 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
         ensure_cand(cur.nshots);
         enqueue_first(cur, true, (uint64_t)(int)nb_fast_ - old_nb);
       }
-      // Queue the next draw behind this one.  Its inputs hold unless this draw
-      // triggers a refinement (the device then keeps its gate closed); it cannot
-      // change the class or overshoot its target: shots(cur) + shots(nxt) < maximal.
-      // With the draw head predicting a refinement, the next draw is not queued ahead
-      // (its gate would stay closed); the device still decides, as always.
+      // Queue up to kQueueDepth draws behind this one, each gated on the one
+      // before it.  Their inputs hold unless a draw before them triggers a
+      // refinement (the device then keeps the gate closed and every later queued
+      // draw returns at once); they cannot change the class or overshoot its
+      // target: the shots of cur and of every queued draw sum below maximal.
+      // With the draw head predicting a refinement, nothing more is queued ahead
+      // (the gates would stay closed); the device still decides, as always.
       if (predicted_refine) lst_.speculation_skipped++;
-      if ((max_draws < 0 || st->draws + 2 <= max_draws) && !predicted_refine) {
-        nxt = host_timed("host_prepare_next", [&] { return prep_get(cur.w_end(), nvox_); });
-        if (nxt.nshots <= 48000 && (size_t)nxt.nshots <= cand_cap_ && nshots + (uint64_t)nxt.nshots < maximal) {
-          enqueue_first(nxt, false, 0);  // the miss count restarts after each draw
-          nxt_live = true;
+      uint64_t queued_shots = nshots;
+      for (int k = 0; k < live; k++) queued_shots += (uint64_t)ahead[k].nshots;
+      while (live < kQueueDepth && !predicted_refine && (max_draws < 0 || st->draws + 2 + live <= max_draws)) {
+        if ((int)ahead.size() == live) {
+          const uint64_t wn = live == 0 ? cur.w_end() : ahead[live - 1].w_end();
+          ahead.push_back(host_timed("host_prepare_next", [&] { return prep_get(wn, nvox_); }));
         }
+        Prep& n = ahead[live];
+        if (!(n.nshots <= 48000 && (size_t)n.nshots <= cand_cap_ && queued_shots + (uint64_t)n.nshots < maximal))
+          break;  // it runs as a forced draw later
+        enqueue_first(n, false, 0);  // the miss count restarts after each draw
+        queued_shots += (uint64_t)n.nshots;
+        live++;
       }
-      nb_new = (uint64_t)finish_draw(cur, rounds, &nxt_live, gate_open, st);
+      nb_new = (uint64_t)finish_draw(cur, rounds, &spec_ok, gate_open, st);
+      if (!spec_ok) live = 0;  // their gates stayed closed: they are queued again, forced
     }
     w = cur.w_end();
     st->draws++;
@@ -4776,7 +4799,8 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     last_miss_ = miss_rate;
     since_refine_ = refine_now ? 0 : since_refine_ + 1;
     if (refine_now) {
-      nxt_live = false;  // its gate stayed closed
+      ahead.clear();  // their gates stayed closed, and the voxel list changes
+      live = 0;
       HIP_OK(hipEventRecord(e0, st_));
       refine(gmin_);
       HIP_OK(hipEventRecord(e1, st_));
@@ -4788,12 +4812,19 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
       st->nvox_max = std::max<int64_t>(st->nvox_max, nvox_);
     }
     if (!cont) {
-      if (nxt_live) throw Error("internal: a draw was queued past the last class");
+      if (live) throw Error("internal: a draw was queued past the last class");
       st->end_reason = 2;
       break;
     }
-    cur = nxt;
-    cur_live = nxt_live;
+    if (ahead.empty()) {
+      cur = Prep{};  // prepared at the top of the loop
+      cur_live = false;
+    } else {
+      cur = ahead.front();
+      ahead.pop_front();
+      cur_live = live > 0;
+      if (live) live--;
+    }
   }
   if (prof_) prof_add("host_wait_mt_words", 1e-9 * (double)hs_.wait_ns.load(), 0);
   if (learn_trace_ && !draw_t_.empty()) {
