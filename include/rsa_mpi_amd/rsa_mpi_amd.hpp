@@ -545,65 +545,58 @@ inline int& output_step() {
   return step;
 }
 
+// a file for writing, or a message on std::cerr naming it
+inline bool open_for_writing(std::ofstream& out, const std::string& path) {
+  out.open(path);
+  if (!out) std::cerr << "rsa_paraview: cannot write " << path << std::endl;
+  return bool(out);
+}
+
 inline void write_pvtp(const std::string& basedir, const std::string& basename, size_t number_of_files) {
-  const std::string name = basedir + "/" + basename + ".pvtp";
-  std::ofstream out(name);
-  if (!out) {
-    std::cerr << "Erreur : impossible de créer le fichier de sortie suivant: " << name << std::endl;
-    return;
-  }
-  out << " <VTKFile type=\"PPolyData\"> " << std::endl;
-  out << "   <PPolyData GhostLevel=\"0\">" << std::endl;
-  out << "    <PPointData Scalar=\"id, tag\">" << std::endl;
-  out << "      <PDataArray type=\"Int64\" Name=\"Id\"/>" << std::endl;
-  out << "      <PDataArray type=\"Int32\" Name=\"Tag\"/>" << std::endl;
-  out << "      <PDataArray type=\"Float64\" Name=\"Radius\"/>" << std::endl;
-  out << "    </PPointData>" << std::endl;
-  out << "     <PPoints>" << std::endl;
-  out << "       <PDataArray type=\"Float64\" NumberOfComponents=\"3\"/>" << std::endl;
-  out << "     </PPoints> " << std::endl;
-  for (size_t i = 0; i < number_of_files; i++)
-    out << "     <Piece Source=\"" << basename << "/" << basename << "_" << i << ".vtp\"/>" << std::endl;
-  out << "   </PPolyData>" << std::endl;
-  out << " </VTKFile>" << std::endl;
+  std::ofstream out;
+  if (!open_for_writing(out, basedir + "/" + basename + ".pvtp")) return;
+  // the index file's fixed part, byte for byte as the reference writes it
+  static const char* const kHead =
+      " <VTKFile type=\"PPolyData\"> \n"
+      "   <PPolyData GhostLevel=\"0\">\n"
+      "    <PPointData Scalar=\"id, tag\">\n"
+      "      <PDataArray type=\"Int64\" Name=\"Id\"/>\n"
+      "      <PDataArray type=\"Int32\" Name=\"Tag\"/>\n"
+      "      <PDataArray type=\"Float64\" Name=\"Radius\"/>\n"
+      "    </PPointData>\n"
+      "     <PPoints>\n"
+      "       <PDataArray type=\"Float64\" NumberOfComponents=\"3\"/>\n"
+      "     </PPoints> \n";
+  out << kHead;
+  for (size_t piece = 0; piece < number_of_files; piece++)
+    out << "     <Piece Source=\"" << basename << "/" << basename << "_" << piece << ".vtp\"/>\n";
+  out << "   </PPolyData>\n </VTKFile>" << std::endl;
 }
 
 template <int DIM>
 void write_vtp(const std::string& name, const std::vector<rsa_sphere<DIM>>& spheres) {
-  std::ofstream out(name + ".vtp");
-  if (!out) {
-    std::cerr << "Erreur : impossible de créer le fichier de sortie suivant: " << name << ".vtp" << std::endl;
-    return;
+  std::ofstream out;
+  if (!open_for_writing(out, name + ".vtp")) return;
+  // one ASCII value list per point attribute, and the centres (z = 0 in 2D)
+  std::ostringstream centres, phases, tags, radii;
+  for (const rsa_sphere<DIM>& sph : spheres) {
+    for (int d = 0; d < 3; d++) centres << (d < DIM ? sph.center[d] : 0.) << ' ';
+    phases << sph.phase << ' ';
+    tags << int(TypeTag::Validated) << ' ';
+    radii << sph.radius << ' ';
   }
-  std::stringstream pts, ids, tags, rads;
-  for (const auto& s : spheres) {
-    for (int d = 0; d < 3; d++) pts << (d < DIM ? s.center[d] : 0.) << " ";
-    ids << s.phase << " ";
-    tags << int(TypeTag::Validated) << " ";
-    rads << s.radius << " ";
-  }
-  out << "<VTKFile type=\"PolyData\">" << std::endl;
-  out << " <PolyData>" << std::endl;
-  out << "   <Piece NumberOfPoints=\"" << spheres.size() << "\" NumberOfPolys=\"" << 0 << "\">" << std::endl;
-  out << "   <PointData>" << std::endl;
-  out << "     <DataArray type=\"Int64\" Name=\"Id\" format=\"ascii\">" << std::endl;
-  out << ids.str() << std::endl;
-  out << "     </DataArray>" << std::endl;
-  out << "     <DataArray type=\"Int32\" Name=\"Tag\" format=\"ascii\">" << std::endl;
-  out << tags.str() << std::endl;
-  out << "     </DataArray>" << std::endl;
-  out << "     <DataArray type=\"Float64\" Name=\"Radius\" format=\"ascii\">" << std::endl;
-  out << rads.str() << std::endl;
-  out << "     </DataArray>" << std::endl;
-  out << "   </PointData>" << std::endl;
-  out << "   <Points>" << std::endl;
-  out << "     <DataArray type=\"Float64\" Name=\"\"  NumberOfComponents=\"3\" format=\"ascii\">" << std::endl;
-  out << pts.str() << std::endl;
-  out << "     </DataArray>" << std::endl;
-  out << "   </Points>" << std::endl;
-  out << "  </Piece>" << std::endl;
-  out << " </PolyData>" << std::endl;
-  out << "</VTKFile>" << std::endl;
+  auto data_array = [&out](const char* type, const char* label, const std::ostringstream& values) {
+    out << "     <DataArray type=\"" << type << "\" Name=\"" << label << "\" format=\"ascii\">\n"
+        << values.str() << "\n     </DataArray>\n";
+  };
+  out << "<VTKFile type=\"PolyData\">\n <PolyData>\n"
+      << "   <Piece NumberOfPoints=\"" << spheres.size() << "\" NumberOfPolys=\"0\">\n   <PointData>\n";
+  data_array("Int64", "Id", phases);
+  data_array("Int32", "Tag", tags);
+  data_array("Float64", "Radius", radii);
+  out << "   </PointData>\n   <Points>\n"
+      << "     <DataArray type=\"Float64\" Name=\"\"  NumberOfComponents=\"3\" format=\"ascii\">\n"
+      << centres.str() << "\n     </DataArray>\n   </Points>\n  </Piece>\n </PolyData>\n</VTKFile>" << std::endl;
 }
 
 //! operator_paraview.ixx:98-136 on one rank
@@ -653,35 +646,35 @@ template <int DIM>
 class rsa_domain {
  public:
   rsa_domain(const vec_d<DIM>& global_inf, const vec_d<DIM>& global_sup, const double a_rad)
-      : rsa_domain_ptr_{new ::rsa_domain<DIM>(global_inf, global_sup, 1, a_rad)} {}
-  const vec_d<DIM>& get_inf() const { return rsa_domain_ptr_->get_inf(); }
-  const vec_d<DIM>& get_sup() const { return rsa_domain_ptr_->get_sup(); }
-  double get_total_volume() const { return rsa_domain_ptr_->get_total_volume(); }
-  double compute_total_volume_of_spheres() const { return rsa_domain_ptr_->compute_total_volume_of_spheres(); }
-  void domain_log() const { rsa_domain_ptr_->domain_log(); }
-  std::vector<rsa_sphere<DIM>> extract_spheres() const { return rsa_domain_ptr_->extract_spheres(); }
+      : impl_{new ::rsa_domain<DIM>(global_inf, global_sup, 1, a_rad)} {}
+  const vec_d<DIM>& get_inf() const { return impl_->get_inf(); }
+  const vec_d<DIM>& get_sup() const { return impl_->get_sup(); }
+  double get_total_volume() const { return impl_->get_total_volume(); }
+  double compute_total_volume_of_spheres() const { return impl_->compute_total_volume_of_spheres(); }
+  void domain_log() const { impl_->domain_log(); }
+  std::vector<rsa_sphere<DIM>> extract_spheres() const { return impl_->extract_spheres(); }
   //! user_interface.hxx:40
-  void paraview() const { rsa_paraview::paraview(*rsa_domain_ptr_); }
-  const ::rsa_domain<DIM>& get_pointed_to() const { return *rsa_domain_ptr_; }
-  ::rsa_domain<DIM>& get_pointed_to() { return *rsa_domain_ptr_; }
+  void paraview() const { rsa_paraview::paraview(*impl_); }
+  const ::rsa_domain<DIM>& get_pointed_to() const { return *impl_; }
+  ::rsa_domain<DIM>& get_pointed_to() { return *impl_; }
 
  private:
-  std::shared_ptr<::rsa_domain<DIM>> rsa_domain_ptr_;
+  std::shared_ptr<::rsa_domain<DIM>> impl_;
 };
 
 template <int DIM>
 class RadiusGenerator {
  public:
-  RadiusGenerator(std::vector<std::tuple<double, uint64_t, int>> desired_radius_nb_phase, double exclusion_distance)
-      : radius_generator_prt_{new sac_de_billes::RadiusGenerator<DIM>(desired_radius_nb_phase, exclusion_distance)} {}
-  RadiusGenerator(std::vector<std::tuple<double, double, int>> desired_radius_volumeFraction_phase, double volume,
-                  double exclusion_distance)
-      : radius_generator_prt_{new sac_de_billes::RadiusGenerator<DIM>(desired_radius_volumeFraction_phase, volume,
-                                                                      exclusion_distance)} {}
-  sac_de_billes::RadiusGenerator<DIM>& get_pointed_to() { return *radius_generator_prt_; }
+  // classes as (radius, count, phase)
+  RadiusGenerator(std::vector<std::tuple<double, uint64_t, int>> classes, double exclusion)
+      : impl_{std::make_shared<sac_de_billes::RadiusGenerator<DIM>>(classes, exclusion)} {}
+  // classes as (radius, volume fraction, phase) of a domain of the given volume
+  RadiusGenerator(std::vector<std::tuple<double, double, int>> classes, double volume, double exclusion)
+      : impl_{std::make_shared<sac_de_billes::RadiusGenerator<DIM>>(classes, volume, exclusion)} {}
+  sac_de_billes::RadiusGenerator<DIM>& get_pointed_to() { return *impl_; }
 
  private:
-  std::shared_ptr<sac_de_billes::RadiusGenerator<DIM>> radius_generator_prt_;
+  std::shared_ptr<sac_de_billes::RadiusGenerator<DIM>> impl_;
 };
 
 template <int DIM>
@@ -689,15 +682,15 @@ class rsa_algo {
  public:
   rsa_algo(rsa_domain<DIM>& a_domain, user_interface::RadiusGenerator<DIM>& a_radius_generator,
            int a_nbshots_singledraw)
-      : rsa_algo_ptr_{new algorithm::rsa_algo<DIM>(a_domain.get_pointed_to(), a_radius_generator.get_pointed_to(),
+      : impl_{new algorithm::rsa_algo<DIM>(a_domain.get_pointed_to(), a_radius_generator.get_pointed_to(),
                                                    a_nbshots_singledraw)} {}
-  void proceed(size_t seed) { rsa_algo_ptr_->template proceed<1>(seed); }
+  void proceed(size_t seed) { impl_->template proceed<1>(seed); }
   void reset_radius_generator(RadiusGenerator<DIM>& a_radius_generator) {
-    rsa_algo_ptr_->reset_radius_generator(a_radius_generator.get_pointed_to());
+    impl_->reset_radius_generator(a_radius_generator.get_pointed_to());
   }
 
  private:
-  std::shared_ptr<algorithm::rsa_algo<DIM>> rsa_algo_ptr_;
+  std::shared_ptr<algorithm::rsa_algo<DIM>> impl_;
 };
 
 template <int DIM>
@@ -720,18 +713,20 @@ rsa_data_storage<DIM> gather_data(rsa_data_storage<DIM>& a_data) {
 //! reference reads three coordinates whatever DIM is; here D = 2 writes z = 0.
 template <int DIM>
 void write_xyz(std::string a_name, rsa_data_storage<DIM>& spheres, std::array<double, DIM>& a_sup) {
-  const int type = 0;
-  auto all_spheres = gather_data(spheres);
-  std::ofstream outFile(a_name);
-  if (!outFile) {
-    std::cerr << "Erreur : impossible de créer le fichier de sortie !" << std::endl;
+  const rsa_data_storage<DIM> all = gather_data(spheres);
+  std::ofstream out(a_name);
+  if (!out) {
+    std::cerr << "write_xyz: cannot write " << a_name << std::endl;
     return;
   }
-  auto comp = [](const auto& a, int d) { return d < DIM ? a[d] : 0.; };
-  outFile << all_spheres.size() << std::endl;
-  outFile << comp(a_sup, 0) << " " << comp(a_sup, 1) << " " << comp(a_sup, 2) << std::endl;
-  for (size_t i = 0; i < all_spheres.size(); i++) {
-    const auto center = all_spheres.get_center((int)i);
-    outFile << type << " " << comp(center, 0) << " " << comp(center, 1) << " " << comp(center, 2) << std::endl;
+  auto xyz = [&out](const auto& v) {  // three coordinates, z = 0 in 2D
+    for (int d = 0; d < 3; d++) out << (d ? " " : "") << (d < DIM ? v[d] : 0.);
+    out << std::endl;
+  };
+  out << all.size() << std::endl;
+  xyz(a_sup);
+  for (size_t k = 0; k < all.size(); k++) {
+    out << 0 << " ";  // the sphere type the format carries (always 0)
+    xyz(all.get_center((int)k));
   }
 }

@@ -152,3 +152,39 @@ def test_write_xyz_plugin_reference_args(tmp_path):
     assert lines[0] == str(o["n"])
     assert lines[1] == "10 10 10"
     assert lines[2:] == ["0 " + " ".join("%g" % v for v in p) for p in o["pos"]]
+
+
+def test_writers_byte_for_byte(tmp_path):
+    """The Paraview piece/index writers (operator_paraview.ixx:12-77) and write_xyz
+    (plugin/write_xyz.hpp:78-98) on fixed spheres, host only, against the reference's
+    output restated line by line (C++ default stream format: %g with 6 digits)."""
+    exe = str(tmp_path / "format_check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include", "rsa_mpi_amd"),
+                    os.path.join(ROOT, "tests", "cpp", "format_check.cpp"), "-o", exe], check=True)
+    subprocess.run([exe, str(tmp_path)], check=True)
+    g = lambda v: "%g" % v
+
+    def vtp(spheres, dim):
+        pts = "".join(" ".join(g(c[d] if d < dim else 0.0) for d in range(3)) + " " for c, _, _ in spheres)
+        lines = ['<VTKFile type="PolyData">', " <PolyData>",
+                 '   <Piece NumberOfPoints="%d" NumberOfPolys="0">' % len(spheres), "   <PointData>"]
+        for typ, name, vals in (("Int64", "Id", [str(p) for _, _, p in spheres]), ("Int32", "Tag", ["0"] * len(spheres)),
+                                ("Float64", "Radius", [g(r) for _, r, _ in spheres])):
+            lines += ['     <DataArray type="%s" Name="%s" format="ascii">' % (typ, name),
+                      "".join(v + " " for v in vals), "     </DataArray>"]
+        lines += ["   </PointData>", "   <Points>",
+                  '     <DataArray type="Float64" Name=""  NumberOfComponents="3" format="ascii">', pts,
+                  "     </DataArray>", "   </Points>", "  </Piece>", " </PolyData>", "</VTKFile>"]
+        return "\n".join(lines) + "\n"
+
+    s3 = [((0.125, 0.5, 0.75), 0.0625, 2), ((1.0 / 3.0, 2.5e-7, 12345.678901), 0.01, 0)]
+    assert open(tmp_path / "p3.vtp").read() == vtp(s3, 3)
+    assert open(tmp_path / "p2.vtp").read() == vtp([((0.25, 0.375), 0.125, 1)], 2)
+    idx = [' <VTKFile type="PPolyData"> ', '   <PPolyData GhostLevel="0">', '    <PPointData Scalar="id, tag">',
+           '      <PDataArray type="Int64" Name="Id"/>', '      <PDataArray type="Int32" Name="Tag"/>',
+           '      <PDataArray type="Float64" Name="Radius"/>', "    </PPointData>", "     <PPoints>",
+           '       <PDataArray type="Float64" NumberOfComponents="3"/>', "     </PPoints> "]
+    idx += ['     <Piece Source="idx/idx_%d.vtp"/>' % k for k in range(3)] + ["   </PPolyData>", " </VTKFile>"]
+    assert open(tmp_path / "idx.pvtp").read() == "\n".join(idx) + "\n"
+    xyz = ["2", "1 2 3.5"] + ["0 " + " ".join(g(v) for v in c) for c, _, _ in s3]
+    assert open(tmp_path / "out.xyz").read() == "\n".join(xyz) + "\n"
