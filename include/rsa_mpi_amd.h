@@ -188,6 +188,18 @@ int rsa_gpu_kernel_profile(rsa_gpu_ctx* ctx, int k, const char** name, int64_t* 
 /* Fresh parameters (seeded) and an empty trace store. */
 int rsa_gpu_learn_reset(uint64_t seed);
 
+/* Refinement thresholds of a context (no counterpart in the reference: they
+ * choose how the device walks a voxel list, never what it computes).  Lists of
+ * at least group_min voxels are walked in cell-grouped order; lists of at least
+ * pack_min read the placed spheres from a packed copy in cell order (D <= 3).
+ * Both default to 2^20; both must be >= 1 (INT64_MAX switches one off). */
+typedef struct rsa_gpu_tuning {
+  int64_t group_min;
+  int64_t pack_min;
+} rsa_gpu_tuning;
+int rsa_gpu_set_tuning(rsa_gpu_ctx* ctx, const rsa_gpu_tuning* t);
+int rsa_gpu_get_tuning(rsa_gpu_ctx* ctx, rsa_gpu_tuning* out);
+
 /* Per context: trace (append voxel and draw samples of each proceed to the
  * trace store), use (the trained model orders the coverage walk and the draw
  * queue), measure (count outcomes and wave uniformity even without use). */

@@ -13,6 +13,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from . import tuning as _tuning
+from .tuning import autotune, autotune_batch  # noqa: F401
 from ._lib import Stats, check, dptr, f64, lib, mt_words  # noqa: F401
 
 __all__ = ["Domain", "RadiusGenerator", "LawRadiusGenerator", "Algo", "Stats", "mt_words", "uniform_generate",
@@ -40,7 +42,9 @@ class Sphere:
 class Domain:
     """rsa_domain<DIM> on one GPU (rsa_domain.hxx:53-93): periodic box [inf, sup]."""
 
-    def __init__(self, inf, sup, rad, ghost_layer=1):
+    def __init__(self, inf, sup, rad, ghost_layer=1, tuning="auto"):
+        """tuning: "auto" applies the thresholds autotune() stored for this dimension and
+        box-to-radius ratio (if any), None keeps the defaults, a dict sets them."""
         self.inf = [float(v) for v in inf]
         self.sup = [float(v) for v in sup]
         self.dim = len(self.inf)
@@ -50,6 +54,24 @@ class Domain:
         self._ctx = ctypes.c_void_p()
         a, b = f64(self.inf), f64(self.sup)
         check(lib().rsa_gpu_create(self.dim, dptr(a), dptr(b), int(ghost_layer), self.rad, ctypes.byref(self._ctx)))
+        if tuning == "auto":
+            side = min(b_ - a_ for a_, b_ in zip(self.inf, self.sup))
+            tuning = _tuning.lookup(self.dim, side, self.rad) if self.rad > 0 else None
+        if tuning:
+            self.set_tuning(**{k: tuning[k] for k in ("group_min", "pack_min") if k in tuning})
+
+    def set_tuning(self, group_min=None, pack_min=None):
+        """The refinement thresholds (rsa_gpu_set_tuning): they choose how voxel lists are
+        walked, never the result."""
+        t = self.tuning()
+        t = _lib.Tuning(int(group_min if group_min is not None else t["group_min"]),
+                        int(pack_min if pack_min is not None else t["pack_min"]))
+        check(lib().rsa_gpu_set_tuning(self._ctx, ctypes.byref(t)))
+
+    def tuning(self):
+        t = _lib.Tuning()
+        check(lib().rsa_gpu_get_tuning(self._ctx, ctypes.byref(t)))
+        return {"group_min": t.group_min, "pack_min": t.pack_min}
 
     def __del__(self):
         ctx = getattr(self, "_ctx", None)
@@ -308,14 +330,17 @@ def read_binary(path):
     return rec["c"].copy(), rec["r"].copy(), rec["p"].copy()
 
 
-def batch_run(dim, sides, radii, seeds, nbshots=6000, workers=4, extract=False):
+def batch_run(dim, sides, radii, seeds, nbshots=6000, workers=None, extract=False):
     """Independent realizations, realization k being uniform_generate<dim, 1> on
     the periodic box [0, sides[k]]^dim with radius radii[k] and seed seeds[k]
-    (operator_algorithm.ixx:275-292), `workers` of them at once on the GPU.
-    Returns a dict: counts, volumes, seconds and, with extract, per realization
-    the (pos, rad) arrays in extraction order."""
+    (operator_algorithm.ixx:275-292), `workers` of them at once on the GPU (None:
+    the count autotune_batch() stored for this dimension and median box-to-radius
+    ratio, else 4).  Returns a dict: counts, volumes, seconds and, with extract,
+    per realization the (pos, rad) arrays in extraction order."""
     sides = f64(sides)
     radii = f64(radii)
+    if workers is None:
+        workers = _tuning.lookup_batch(dim, sides, radii) or 4
     seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
     n = len(sides)
     if len(radii) != n or len(seeds) != n:

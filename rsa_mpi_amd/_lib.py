@@ -18,7 +18,7 @@ EXPORTED = (
     "rsa_gpu_set_miss_rate", "rsa_gpu_proceed", "rsa_gpu_add_spheres", "rsa_gpu_count",
     "rsa_gpu_extract", "rsa_gpu_volume_of_spheres", "rsa_gpu_voxels", "rsa_gpu_mt_words",
     "rsa_gpu_profile_enable", "rsa_gpu_kernel_profile", "rsa_gpu_batch_run", "rsa_gpu_algo_init_law",
-    "rsa_gpu_reset_law", "rsa_gpu_extract_noalloc",
+    "rsa_gpu_reset_law", "rsa_gpu_extract_noalloc", "rsa_gpu_set_tuning", "rsa_gpu_get_tuning",
     "rsa_gpu_learn_reset", "rsa_gpu_learn_set_mode", "rsa_gpu_learn_get_stats", "rsa_gpu_learn_samples",
     "rsa_gpu_learn_train", "rsa_gpu_learn_params", "rsa_gpu_learn_gradcheck", "rsa_gpu_learn_forward",
 )
@@ -38,6 +38,11 @@ class Stats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    """rsa_gpu_tuning"""
+    _fields_ = [("group_min", ctypes.c_int64), ("pack_min", ctypes.c_int64)]
 
 
 class LearnStats(ctypes.Structure):
@@ -99,6 +104,8 @@ def lib():
     _F = ctypes.POINTER(ctypes.c_float)
     _U8 = ctypes.POINTER(ctypes.c_uint8)
     L.rsa_gpu_learn_reset.argtypes = [ctypes.c_uint64]
+    L.rsa_gpu_set_tuning.argtypes = [_P, ctypes.POINTER(Tuning)]
+    L.rsa_gpu_get_tuning.argtypes = [_P, ctypes.POINTER(Tuning)]
     L.rsa_gpu_learn_set_mode.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.rsa_gpu_learn_get_stats.argtypes = [_P, ctypes.POINTER(LearnStats)]
     L.rsa_gpu_learn_samples.argtypes = [_I64]

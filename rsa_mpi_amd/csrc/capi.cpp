@@ -217,6 +217,22 @@ int rsa_gpu_learn_reset(uint64_t seed) {
   });
 }
 
+int rsa_gpu_set_tuning(rsa_gpu_ctx* ctx, const rsa_gpu_tuning* t) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] {
+    if (!t) throw rsa_amd::Error("rsa_gpu_set_tuning: null tuning");
+    ctx->eng->set_tuning(*t);
+  });
+}
+
+int rsa_gpu_get_tuning(rsa_gpu_ctx* ctx, rsa_gpu_tuning* out) {
+  if (need_ctx(ctx)) return 1;
+  return guarded([&] {
+    if (!out) throw rsa_amd::Error("rsa_gpu_get_tuning: null output");
+    *out = ctx->eng->tuning();
+  });
+}
+
 int rsa_gpu_learn_set_mode(rsa_gpu_ctx* ctx, int trace, int use, int measure) {
   if (need_ctx(ctx)) return 1;
   return guarded([&] { ctx->eng->learn_mode(trace, use, measure); });

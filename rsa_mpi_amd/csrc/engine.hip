@@ -410,6 +410,13 @@ class Engine final : public IEngine {
 
   void proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* out) override;
 
+  void set_tuning(const rsa_gpu_tuning& t) override {
+    if (t.group_min < 1 || t.pack_min < 1) throw Error("tuning: thresholds must be >= 1");
+    group_min_ = t.group_min;
+    pack_min_ = t.pack_min;
+  }
+  rsa_gpu_tuning tuning() const override { return rsa_gpu_tuning{group_min_, pack_min_}; }
+
   // learned predictor: trace logging, use (ordering and speculation), outcome measurement
   void learn_mode(int trace, int use, int measure) override {
     learn_trace_ = trace != 0;
@@ -1038,7 +1045,12 @@ class Engine final : public IEngine {
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, tsum_.p, (int)nt, total);
     hipLaunchKernelGGL(k_tile_scan, dim3(nt), dim3(1024), 0, st_, a, n, tsum_.p);
   }
+  // Refinement thresholds (rsa_gpu_tuning): lists of at least group_min_ voxels
+  // are walked in cell-grouped order, lists of at least pack_min_ read the
+  // placed spheres from a packed copy in cell order (D <= 3).  Neither changes
+  // any result; the defaults were measured on the 3D cubes of r = 0.002-0.005.
   static constexpr int64_t kGroupMin = int64_t(1) << 20;
+  int64_t group_min_ = kGroupMin, pack_min_ = kGroupMin;
 
   // remove_covered + subdivide_uncovered (operator_algorithm.ixx:190-197)
   void refine(double rmin) {
@@ -1054,7 +1066,7 @@ class Engine final : public IEngine {
     const double cover_bytes = (double)nvox_ * ((wide_[vcur_] ? 8.0 : 4.0) * D + 4.0) +
                                (double)nb_fast_ * sizeof(Slot<D>) + geo_.ncells;
     // long lists are walked in cell-grouped order
-    const bool grouped = nvox_ >= kGroupMin;
+    const bool grouped = nvox_ >= group_min_;
     // learned mode: the MLP predicts each voxel's coverage, and the walk order puts the
     // predicted-covered voxels first (within each group, the cell-grouped order)
     const bool predicted = grouped && learn_use_ && learn::Model::get().trained();
@@ -1096,11 +1108,11 @@ class Engine final : public IEngine {
     }
     const uint32_t* perm = grouped ? perm_.p : nullptr;
     // the placed spheres packed in cell order (k_cover's stencil rows, D <= 3),
-    // for long lists only: below kGroupMin voxels the copy costs more than it saves
+    // for long lists only: below pack_min_ voxels the copy costs more than it saves
     const int* coff = nullptr;
     const Slot<D>* csph = nullptr;
     int ncs = 0;
-    if (ipow_c(3, D) <= 32 && nvox_ >= kGroupMin) {
+    if (ipow_c(3, D) <= 32 && nvox_ >= pack_min_) {
       const int64_t nc = geo_.ncells;
       coff_.reserve((size_t)nc + 1);
       csph_.reserve((size_t)(nb_fast_ + added_) + 1);

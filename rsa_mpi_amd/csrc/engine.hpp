@@ -38,6 +38,8 @@ class IEngine {
   virtual uint64_t voxels(int64_t* coords, double* len, uint64_t cap) = 0;
   virtual double total_volume() const = 0;
   // learned predictor (learn.hip): trace logging, use for ordering and speculation, outcome counts
+  virtual void set_tuning(const rsa_gpu_tuning& t) = 0;
+  virtual rsa_gpu_tuning tuning() const = 0;
   virtual void learn_mode(int trace, int use, int measure) = 0;
   virtual void learn_stats(rsa_gpu_learn_stats* out) = 0;
   // per-kernel device timings (HIP events on the context's stream)
