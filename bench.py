@@ -156,6 +156,9 @@ def learned(R, steps, plain):
            "uniform_waves_plain": round(base["waves_uniform"] / max(1, base["waves"]), 4),
            "uniform_waves_learned": round(ls["waves_uniform"] / max(1, ls["waves"]), 4),
            "profile": per_kernel(prof, steps)}
+    mlp = profile_figures("mlp_kernels_3d_r0.002.md")
+    if mlp:
+        out["mfma_profile"] = {k: v for k, v in mlp.items() if k.endswith(("_mfma_busy_frac", "_ms"))}
     if kp.get("seconds"):
         per_step = kp["seconds"] / steps  # every step predicts the same voxels
         tf = vp * MLP_FLOP / per_step / 1e12
@@ -200,6 +203,17 @@ def roofline(prof, steps, figures):
         busy_s = valu * 4 / (1024 * 2.4e9)
         out["valu"] = dict(wave_instructions_per_run=valu, issue_seconds_per_run=round(busy_s, 6),
                            busy_frac=round(busy_s / (k["seconds"] / steps), 3))
+    # FP64 share of that issue, and the FP64 rate against the 78.6 TFLOP/s vector peak
+    # (1024 SIMDs x 16 lanes x 2 FLOP x 2.4 GHz): FP64 wave-instructions x 64 lanes x the
+    # measured active-lane fraction, one FLOP each (the build contracts no FMA)
+    f64 = figures.get("fp64_wave_instructions_per_run")
+    util = figures.get("valu_lane_utilization")
+    if f64 and util:
+        t = k["seconds"] / steps
+        flops = f64 * 64 * util / t
+        out["fp64_valu"] = dict(wave_instructions_per_run=f64, issue_frac=round(f64 * 4 / (1024 * 2.4e9) / t, 3),
+                                active_lane_frac=round(util, 3), tflops=round(flops / 1e12, 2), peak_tflops=78.6,
+                                frac=round(flops / 78.6e12, 4))
     return out
 
 
@@ -246,7 +260,11 @@ def batched(R, workers):
     res = {}
     for name, w in (("batched", workers), ("plain", 1)):
         t0 = time.perf_counter()
-        out = R.batch_run(3, L, r, s, workers=w, extract=True)
+        try:
+            out = R.batch_run(3, L, r, s, workers=w, extract=True)
+        except RuntimeError as e:  # a failed device check or extraction checksum: reported, not a value
+            res[name] = dict(seconds=None, value=None, sets_match_oracle="error", error=str(e)[:300])
+            continue
         dt = time.perf_counter() - t0
         bad = [k for k, (p, rr) in enumerate(out["spheres"])
                if len(rr) != fx["oracle"][k]["n"] or set_hash(p, rr) != fx["oracle"][k]["sha256"]]
@@ -256,7 +274,8 @@ def batched(R, workers):
                         "r~U[0.004,0.008], seeds 0..255, each to maximal packing (extraction in the timed region)",
             "realizations": len(real), "spheres": int(sum(c["n"] for c in fx["oracle"])), "workers": workers,
             "unit": "spheres/s", "value": res["batched"]["value"], "plain_value": res["plain"]["value"],
-            "speedup_vs_plain": round(res["batched"]["value"] / res["plain"]["value"], 3), **{
+            "speedup_vs_plain": (round(res["batched"]["value"] / res["plain"]["value"], 3)
+                                 if res["batched"]["value"] and res["plain"]["value"] else None), **{
                 k + "_" + kk: v for k, d in res.items() for kk, v in d.items() if kk != "value"}}
 
 

@@ -322,6 +322,13 @@ int rsa_gpu_batch_run(int dim, int n, const double* side, const double* radius, 
             throw rsa_amd::Error(g_err);
           rsa_gpu_stats st;
           tmp.eng->proceed(seed[k], -1, &st);
+          // the device's end-of-run checks: a packing that is not maximal and free of
+          // overlaps is an error here, not a result (the reference's 4D voxel path
+          // need not be maximal, so 4D checks overlaps only)
+          if (st.end_reason != 1 || st.nvox_final != 0 || st.overlap_pairs != 0 || (dim <= 3 && st.cover_free != 0))
+            throw rsa_amd::Error("batch: realization " + std::to_string(k) + " failed the end-of-run checks (" +
+                                 std::to_string(st.overlap_pairs) + " overlapping pairs, " +
+                                 std::to_string(st.cover_free) + " free sample points)");
           Out& o = res[k];
           o.count = tmp.eng->count();
           if (volume) o.volume = tmp.eng->volume_of_spheres();

@@ -12,6 +12,23 @@ namespace rsa_amd {
 
 constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
 
+// Order-independent checksum of a sphere set (extraction check): the sum mod
+// 2^64 over spheres and values (D coordinates, then the radius) of word_mix of
+// each value's bits.  word_mix(., k) is a bijection of 64-bit words (an
+// xor-shift, then a product with an odd constant), so changing any one value
+// always changes the sum.
+__host__ __device__ __forceinline__ uint64_t word_mix(uint64_t x, int k) {
+  x ^= x >> 29;
+  return x * ((0x9E3779B97F4A7C15ull ^ (0xC2B2AE3D27D4EB4Full * (uint64_t)(k + 1))) | 1ull);
+}
+__host__ __device__ __forceinline__ uint64_t bits_of(double v) {
+  union {
+    double d;
+    uint64_t u;
+  } w{v};
+  return w.u;
+}
+
 // Tags of rsa_data_storage_with_tags (rsa_data_storage.hxx:141-151) for the
 // candidates of the current draw; DROPPED marks a centre that rounded into a
 // ghost cell (never placed).

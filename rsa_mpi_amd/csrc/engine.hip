@@ -127,7 +127,7 @@ class Engine final : public IEngine {
       (void)hipStreamDestroy(st_);
       throw;
     }
-    hctr_ = static_cast<int*>(PinnedPool::get().alloc(sizeof(int) * C_NUM, hipHostMallocDefault));
+    hctr_ = static_cast<int*>(PinnedPool::get().alloc(sizeof(int) * (C_NUM + 4), hipHostMallocDefault));
     dctr_.reserve(C_NUM);
     reset_ctr();  // pool memory: counters start at zero
     ring_.reserve(kRing);
@@ -294,6 +294,13 @@ class Engine final : public IEngine {
 
   uint64_t count() const override { return nb_fast_; }
 
+  // 64-bit slot after the counters in pinned memory (8-byte aligned)
+  uint64_t* hsum() { return reinterpret_cast<uint64_t*>(hctr_ + ((C_NUM + 1) & ~1)); }
+
+  // The placed spheres in extraction order.  Device to host copies go through a
+  // pinned staging block (never straight into the caller's pageable memory), and
+  // the received values are checked against the set checksum k_extract forms
+  // on the device: a transfer that changed any value is an error, not a result.
   uint64_t extract(double* pos, double* rad, uint64_t* phase, uint64_t cap) override {
     StreamScope scope(st_, cst_);
     if (!algo_ready_) return 0;
@@ -302,28 +309,50 @@ class Engine final : public IEngine {
     blk.reserve(nb);
     hipLaunchKernelGGL(k_cell_counts<D>, dim3(nb), dim3(WG), 0, st_, geo_, cells_cnt_.p, blk.p);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st_, blk.p, (int)nb, dctr_.p + C_CURSOR);
-    int total = 0;
-    HIP_OK(hipMemcpyAsync(&total, dctr_.p + C_CURSOR, sizeof(int), hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipMemcpyAsync(hctr_ + C_CURSOR, dctr_.p + C_CURSOR, sizeof(int), hipMemcpyDeviceToHost, st_));
     HIP_OK(hipStreamSynchronize(st_));
+    const int total = hctr_[C_CURSOR];
     if (pos == nullptr) return (uint64_t)total;
     if ((uint64_t)total > cap) throw Error("extract: output capacity too small");
+    if (total == 0) return 0;
     DevBuf<double> dp, dr;
     DevBuf<uint32_t> dph;
+    DevBuf<unsigned long long> xs;
     dp.reserve((size_t)total * D);
     dr.reserve(total);
     dph.reserve(total);
+    xs.reserve(1);
+    HIP_OK(hipMemsetAsync(xs.p, 0, sizeof(unsigned long long), st_));
     hipLaunchKernelGGL((k_extract<D, uint32_t>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), blk.p, dp.p, dr.p,
-                       dph.p);
+                       dph.p, xs.p);
+    HIP_OK(hipGetLastError());
+    const size_t bpos = (size_t)total * D * sizeof(double), brad = (size_t)total * sizeof(double),
+                 bph = (size_t)total * sizeof(uint32_t);
+    char* stage = static_cast<char*>(PinnedPool::get().alloc(bpos + brad + bph, hipHostMallocDefault));
+    struct Give {  // the staging block goes back to the pool on every path
+      void* p;
+      ~Give() { PinnedPool::get().release(p); }
+    } give{stage};
+    HIP_OK(hipMemcpyAsync(hsum(), xs.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipMemcpyAsync(stage, dp.p, bpos, hipMemcpyDeviceToHost, st_));
+    HIP_OK(hipMemcpyAsync(stage + bpos, dr.p, brad, hipMemcpyDeviceToHost, st_));
+    if (phase) HIP_OK(hipMemcpyAsync(stage + bpos + brad, dph.p, bph, hipMemcpyDeviceToHost, st_));
     HIP_OK(hipStreamSynchronize(st_));
-    HIP_OK(hipMemcpyAsync(pos, dp.p, (size_t)total * D * sizeof(double), hipMemcpyDeviceToHost, st_));
-    HIP_OK(hipStreamSynchronize(st_));
-    HIP_OK(hipMemcpyAsync(rad, dr.p, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, st_));
-    HIP_OK(hipStreamSynchronize(st_));
+    const double* sp = reinterpret_cast<const double*>(stage);
+    const double* sr = reinterpret_cast<const double*>(stage + bpos);
+    uint64_t h = 0;
+    for (int i = 0; i < total; i++) {
+      for (int d = 0; d < D; d++) h += word_mix(bits_of(sp[(size_t)i * D + d]), d);
+      h += word_mix(bits_of(sr[i]), D);
+    }
+    if (h != *hsum())
+      throw Error("extract: the spheres received on the host do not match their device checksum (" +
+                  std::to_string(total) + " spheres)");
+    std::memcpy(pos, sp, bpos);
+    std::memcpy(rad, sr, brad);
     if (phase) {
-      std::vector<uint32_t> ph(total);
-      HIP_OK(hipMemcpyAsync(ph.data(), dph.p, (size_t)total * sizeof(uint32_t), hipMemcpyDeviceToHost, st_));
-      HIP_OK(hipStreamSynchronize(st_));
-      for (int i = 0; i < total; i++) phase[i] = ph[i];
+      const uint32_t* sph = reinterpret_cast<const uint32_t*>(stage + bpos + brad);
+      for (int i = 0; i < total; i++) phase[i] = sph[i];
     }
     return (uint64_t)total;
   }
@@ -348,7 +377,8 @@ class Engine final : public IEngine {
     if (pos == nullptr) return (uint64_t)total;
     if ((uint64_t)total > cap) throw Error("extract: output capacity too small");
     if (!rad) throw Error("extract_noalloc: null radius output");
-    hipLaunchKernelGGL((k_extract<D, uint64_t>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), s, pos, rad, phase);
+    hipLaunchKernelGGL((k_extract<D, uint64_t>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), s, pos, rad, phase,
+                       (unsigned long long*)nullptr);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st_));
     return (uint64_t)total;

@@ -25,9 +25,10 @@ __global__ void k_cell_counts(Geo<D> g, const uint8_t* __restrict__ cnt, int* __
   if (threadIdx.x == 0) blk[blockIdx.x] = sh[0];
 }
 
+// xsum (may be null): the set's checksum (word_mix sum over the spheres written)
 template <int D, class PH>
 __global__ void k_extract(Geo<D> g, CellStore<D> cs, const int* __restrict__ blk, double* __restrict__ pos,
-                          double* __restrict__ rad, PH* __restrict__ ph) {
+                          double* __restrict__ rad, PH* __restrict__ ph, unsigned long long* __restrict__ xsum) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = c < g.ncells ? cs.cnt[c] : 0;
   __shared__ int sh[WG];
@@ -39,14 +40,22 @@ __global__ void k_extract(Geo<D> g, CellStore<D> cs, const int* __restrict__ blk
     sh[threadIdx.x] += add;
     __syncthreads();
   }
-  if (v == 0) return;
+  uint64_t h = 0;
   int o = blk[blockIdx.x] + sh[threadIdx.x] - v;
   for (int k = 0; k < v; k++, o++) {
     const Slot<D> s = cs.sph[(size_t)c * cs.cap + k];
-    for (int d = 0; d < D; d++) pos[(size_t)o * D + d] = s.c[d];
+    for (int d = 0; d < D; d++) {
+      pos[(size_t)o * D + d] = s.c[d];
+      h += word_mix(bits_of(s.c[d]), d);
+    }
     rad[o] = s.r;
+    h += word_mix(bits_of(s.r), D);
     if (ph) ph[o] = (PH)cs.phase[(size_t)c * cs.cap + k];
   }
+  if (xsum == nullptr) return;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) h += (uint64_t)__shfl_xor((unsigned long long)h, off, 64);
+  if ((threadIdx.x & 63) == 0 && h) atomicAdd(xsum, (unsigned long long)h);
 }
 
 // Scratch of the allocation-free extraction (Engine::extract_noalloc): the
