@@ -448,3 +448,41 @@ def test_reset_to_custom_law_python_bit_exact():
     assert_same(dict(pos=pos, rad=rad, phase=ph, stats=st), o)
     with pytest.raises(RuntimeError, match="Incorrect maximal radius"):
         algo.reset_radius_generator(R.LawRadiusGenerator(0.006, 0.02, lambda u: 0.006 + 0.014 * u, 10))
+
+
+@pytest.mark.parametrize("inf,sup,r,seed,nbshots", [
+    ([0.0, 0.0], [1.0, 0.4], 0.01, 2, 6000),
+    ([-0.1, 0.2, 0.0], [0.2, 0.45, 0.18], 0.006, 1, 6000),
+    ([0.0, 0.0, 0.0], [0.5, 0.1, 0.3], 0.008, 4, 6000),
+    ([0.0, 0.0, 0.0, 0.0], [0.3, 0.4, 0.3, 0.35], 0.04, 0, 6000),
+    ([0.0, 0.0], [0.6, 0.6], 0.012, 5, 100),  # many small draws
+])
+def test_non_cubic_boxes_bit_exact(inf, sup, r, seed, nbshots):
+    """Boxes with a different side per dimension (and offsets per dimension), and a small
+    nbshots_singledraw, against the oracle: same spheres, same order, same words."""
+    dim = len(inf)
+    dom = R.Domain(inf, sup, r)
+    gen = R.RadiusGenerator([(r, 1.0, 0)], dom.get_total_volume(), 0.0)
+    st = R.Algo(dom, gen, nbshots).proceed(seed)
+    pos, rad, ph = dom.extract_arrays()
+    o = O.run(dim, inf, sup, [(r, 1.0, 0)], seed=seed, nbshots=nbshots)
+    assert st["words"] == o["words"] and st["draws"] == len(o["trace"])
+    assert_same(dict(pos=pos, rad=rad, phase=ph, stats=st), o)
+
+
+def test_reference_error_messages():
+    """The reference's own errors: a box narrower than twice the domain radius (the grid
+    is built first and refuses it, rsa_grid_traversal.ixx:18-23, before
+    rsa_domain.ixx:135-142 could), an empty radius generator, a generator whose largest
+    radius exceeds the domain's; a zero-draw proceed places nothing."""
+    with pytest.raises(RuntimeError, match="Too small domain wrt ghost layer"):
+        R.Domain([0.0, 0.0, 0.0], [0.1, 0.5, 0.5], 0.06)
+    dom = R.Domain([0.0] * 3, [0.2] * 3, 0.01)
+    with pytest.raises(RuntimeError, match="Empty radius generator!"):
+        R.Algo(dom, R.RadiusGenerator([], 0.0), 6000)
+    with pytest.raises(RuntimeError, match="Impossible to have the maximal radius of radius generator larger than "
+                                           "domain implicit radius"):  # operator_algorithm.ixx:26-28
+        R.Algo(dom, R.RadiusGenerator([(0.02, 10, 0)], 0.0), 6000)
+    algo = R.Algo(dom, R.RadiusGenerator([(0.01, 1.0, 0)], dom.get_total_volume(), 0.0), 6000)
+    st = algo.proceed(0, 0)
+    assert st["draws"] == 0 and st["end_reason"] == 3 and dom.extract_arrays()[1].size == 0
