@@ -1164,7 +1164,9 @@ class Engine final : public IEngine {
         hipLaunchKernelGGL((k_cover<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, perm, coff,
                            csph, ncs, code_.p, blk_.p);
     }, 1);
-    if (grouped) hipLaunchKernelGGL(k_block_count, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p);
+    // the streaming passes: one wave per block of WG voxels, four blocks per workgroup
+    const unsigned nbw = nblocks(nvox_, 4 * WG);
+    if (grouped) hipLaunchKernelGGL(k_block_count, dim3(nbw), dim3(WG), 0, st_, nvox_, code_.p, blk_.p);
     // the predictor's outcome counters, and its traces
     const bool measured = learn_use_ || learn_measure_;
     if (measured) {
@@ -1192,15 +1194,15 @@ class Engine final : public IEngine {
     // filled in below, once the counters are read
     const size_t holes_at = pend_.size();
     timed("k_holes", 0., [&] {
-      hipLaunchKernelGGL(k_holes, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
+      hipLaunchKernelGGL(k_holes, dim3(nbw), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
     }, 1);
     timed("k_move", 0., [&] {
-      hipLaunchKernelGGL(k_move<D>, dim3(nb), dim3(WG), 0, st_, nvox_, code_.p, moved_.p, blk_.p, dctr_.p,
+      hipLaunchKernelGGL(k_move<D>, dim3(nbw), dim3(WG), 0, st_, nvox_, code_.p, moved_.p, blk_.p, dctr_.p,
                          holes_.p, vx);
     }, 1);
     // children per block of the kept prefix, counted against the device's keep
     // (blocks past it count 0), so that one host round trip returns both totals
-    hipLaunchKernelGGL(k_child_count, dim3(nb), dim3(WG), 0, st_, nvox_, dctr_.p, code_.p, moved_.p, blk_.p);
+    hipLaunchKernelGGL(k_child_count, dim3(nbw), dim3(WG), 0, st_, nvox_, dctr_.p, code_.p, moved_.p, blk_.p);
     scan_excl(blk_.p, nb, dctr_.p + C_NCHILD);
     read_ctr();
     check_scatter();  // the previous refinement's children
