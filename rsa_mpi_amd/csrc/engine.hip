@@ -1125,15 +1125,16 @@ class Engine final : public IEngine {
       const uint8_t* pr = predicted ? pred_.p : nullptr;
       timed("k_bin", (double)nvox_ * ((wide_[vcur_] ? 8.0 : 4.0) * D + 8.0), [&] {
         if (wide_[vcur_])
-          hipLaunchKernelGGL((k_bin<D, true>), dim3(nb), dim3(WG), 0, st_, geo_, vx, nvox_, vp, ccount_.p, vcell_.p,
-                             vrank_.p, pr);
+          hipLaunchKernelGGL((k_bin<D, true>), dim3(nblocks(nvox_, 4 * WG)), dim3(WG), 0, st_, geo_, vx, nvox_, vp,
+                             ccount_.p, vcell_.p, vrank_.p, pr);
         else
-          hipLaunchKernelGGL((k_bin<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, vx, nvox_, vp, ccount_.p, vcell_.p,
-                             vrank_.p, pr);
+          hipLaunchKernelGGL((k_bin<D, false>), dim3(nblocks(nvox_, 4 * WG)), dim3(WG), 0, st_, geo_, vx, nvox_, vp,
+                             ccount_.p, vcell_.p, vrank_.p, pr);
       }, 1);
       scan_excl(ccount_.p, nkeys, dctr_.p + C_GROUPED);
       timed("k_place", (double)nvox_ * 12.0, [&] {
-        hipLaunchKernelGGL(k_place, dim3(nb), dim3(WG), 0, st_, nvox_, vcell_.p, vrank_.p, ccount_.p, perm_.p);
+        hipLaunchKernelGGL(k_place, dim3(nblocks(nvox_, 4 * WG)), dim3(WG), 0, st_, nvox_, vcell_.p, vrank_.p,
+                           ccount_.p, perm_.p);
       }, 1);
     }
     const uint32_t* perm = grouped ? perm_.p : nullptr;

@@ -324,49 +324,75 @@ template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) k_bin(Geo<D> g, VoxView<D> vx, int64_t nvox, VoxParams<D> vp,
                                             int* __restrict__ ccount, int* __restrict__ vcell,
                                             int* __restrict__ vrank, const uint8_t* __restrict__ pred) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // each wave takes 4 x 64 consecutive voxels, in four passes of 64 whose loads
+  // are all issued first (launch nblocks(n, 4 * WG))
+  constexpr int NP = 4;
   const int lane = threadIdx.x & 63;
-  const bool valid = i < nvox;
-  int key = -1 - lane;  // distinct for lanes past the end
-  if (valid) {
-    double p[D];
-    int cc[D];
+  const int64_t w0 = ((int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6)) * (64 * NP);
+  int64_t cv[NP][D];
 #pragma unroll
-    for (int d = 0; d < D; d++) {
-      const int64_t c = WIDE ? vx.w[d][i] : (int64_t)vx.c[d][i];
-      p[d] = vp.origin[d] + ((double)c + 0.5) * vp.len[d];
+  for (int q = 0; q < NP; q++) {
+    const int64_t i = w0 + q * 64 + lane;
+    const int64_t ic = i < nvox ? i : 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) cv[q][d] = WIDE ? vx.w[d][ic] : (int64_t)vx.c[d][ic];
+  }
+#pragma unroll
+  for (int q = 0; q < NP; q++) {
+    const int64_t i = w0 + q * 64 + lane;
+    const bool valid = i < nvox;
+    int key = -1 - lane;  // distinct for lanes past the end
+    if (valid) {
+      double p[D];
+      int cc[D];
+#pragma unroll
+      for (int d = 0; d < D; d++) p[d] = vp.origin[d] + ((double)cv[q][d] + 0.5) * vp.len[d];
+      const int cell = cell_of_point<D>(g, p, cc);
+      key = (cell >= 0 && cell < g.ncells) ? cell : 0;  // a key only: any value is correct
+      if (pred && !pred[i]) key += g.ncells;
     }
-    const int cell = cell_of_point<D>(g, p, cc);
-    key = (cell >= 0 && cell < g.ncells) ? cell : 0;  // a key only: any value is correct
-    if (pred && !pred[i]) key += g.ncells;
-  }
-  // one atomic per run of equal keys in adjacent lanes (siblings share a cell)
-  const int prev = __shfl_up(key, 1, 64);
-  const bool head = lane == 0 || prev != key;
-  const uint64_t heads = __ballot(head);
-  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  const int start = 63 - __clzll(heads & le);
-  const uint64_t above = heads & ~le;
-  int base = 0;
-  if (head && valid) {
-    const int end = above ? __ffsll((unsigned long long)above) - 1 : 64;
-    base = atomicAdd(&ccount[key], end - lane);
-  }
-  base = __shfl(base, start, 64);
-  if (valid) {
-    vcell[i] = key;
-    vrank[i] = base + lane - start;
+    // one atomic per run of equal keys in adjacent lanes (siblings share a cell)
+    const int prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || prev != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int start = 63 - __clzll(heads & le);
+    const uint64_t above = heads & ~le;
+    int base = 0;
+    if (head && valid) {
+      const int end = above ? __ffsll((unsigned long long)above) - 1 : 64;
+      base = atomicAdd(&ccount[key], end - lane);
+    }
+    base = __shfl(base, start, 64);
+    if (valid) {
+      vcell[i] = key;
+      vrank[i] = base + lane - start;
+    }
   }
 }
 
 __global__ void __launch_bounds__(WG) k_place(int64_t nvox, const int* __restrict__ vcell,
                                               const int* __restrict__ vrank, const int* __restrict__ cstart,
                                               uint32_t* __restrict__ perm) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nvox) return;
-  perm[cstart[vcell[i]] + vrank[i]] = (uint32_t)i;
+  // four voxels per thread, 64 apart within the wave's 256 (launch nblocks(n, 4 * WG))
+  constexpr int NP = 4;
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = ((int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6)) * (64 * NP);
+  int c[NP], r[NP], o[NP];
+#pragma unroll
+  for (int q = 0; q < NP; q++) {
+    const int64_t i = w0 + q * 64 + lane;
+    c[q] = i < nvox ? vcell[i] : 0;
+    r[q] = i < nvox ? vrank[i] : 0;
+  }
+#pragma unroll
+  for (int q = 0; q < NP; q++) o[q] = cstart[c[q]];
+#pragma unroll
+  for (int q = 0; q < NP; q++) {
+    const int64_t i = w0 + q * 64 + lane;
+    if (i < nvox) perm[o[q] + r[q]] = (uint32_t)i;
+  }
 }
-
 
 // The streaming passes over the list (k_block_count, k_holes, k_move,
 // k_child_count) keep the blocks of WG voxels as their unit, but one wave takes
