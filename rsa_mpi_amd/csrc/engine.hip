@@ -1198,8 +1198,12 @@ class Engine final : public IEngine {
       hipLaunchKernelGGL(k_holes, dim3(nbw), dim3(WG), 0, st_, nvox_, code_.p, blk_.p, dctr_.p, holes_.p);
     }, 1);
     timed("k_move", 0., [&] {
-      hipLaunchKernelGGL(k_move<D>, dim3(nbw), dim3(WG), 0, st_, nvox_, code_.p, moved_.p, blk_.p, dctr_.p,
-                         holes_.p, vx);
+      if (wide_[vcur_])
+        hipLaunchKernelGGL((k_move<D, true>), dim3(nbw), dim3(WG), 0, st_, nvox_, code_.p, moved_.p, blk_.p, dctr_.p,
+                           holes_.p, vx);
+      else
+        hipLaunchKernelGGL((k_move<D, false>), dim3(nbw), dim3(WG), 0, st_, nvox_, code_.p, moved_.p, blk_.p, dctr_.p,
+                           holes_.p, vx);
     }, 1);
     // children per block of the kept prefix, counted against the device's keep
     // (blocks past it count 0), so that one host round trip returns both totals

@@ -542,7 +542,7 @@ __global__ void __launch_bounds__(WG) k_holes(int64_t nvox, const uint32_t* __re
 // workgroups fill holes, so code[] must not change during this kernel.
 // Afterwards the code of voxel i < keep is code[i], or moved[i] where code[i]
 // is 0 (every hole is filled).
-template <int D>
+template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) k_move(int64_t nvox, const uint32_t* __restrict__ code,
                                              uint32_t* __restrict__ moved, const int* __restrict__ blk,
                                              const int* __restrict__ ctr, const int* __restrict__ holes,
@@ -557,16 +557,33 @@ __global__ void __launch_bounds__(WG) k_move(int64_t nvox, const uint32_t* __res
   for (int k = 0; k < VPL; k++) n += c[k] != 0;
   int U = blk[b] + wave_incl_scan(n) - n;
   const int keep = ctr[C_KEEP];
+  // the lane's moves: destinations first, then every coordinate load, then the stores
+  bool mv[VPL];
+  int dst[VPL];
 #pragma unroll
   for (int k = 0; k < VPL; k++) {
     const int64_t i = i0 + k;
     const bool unc = c[k] != 0;
-    if (i < nvox && unc && i >= keep) {
-      const int dst = holes[keep - U - 1];
-      for (int d = 0; d < D; d++) vx.set(d, dst, vx.coord(d, i));
-      moved[dst] = c[k];
-    }
+    mv[k] = i < nvox && unc && i >= keep;
+    dst[k] = mv[k] ? holes[keep - U - 1] : 0;
     U += unc;
+  }
+  int64_t cv[VPL][D];
+#pragma unroll
+  for (int k = 0; k < VPL; k++)
+#pragma unroll
+    for (int d = 0; d < D; d++) cv[k][d] = !mv[k] ? 0 : (WIDE ? vx.w[d][i0 + k] : (int64_t)vx.c[d][i0 + k]);
+#pragma unroll
+  for (int k = 0; k < VPL; k++) {
+    if (!mv[k]) continue;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if constexpr (WIDE)
+        const_cast<int64_t*>(vx.w[d])[dst[k]] = cv[k][d];
+      else
+        const_cast<int32_t*>(vx.c[d])[dst[k]] = (int32_t)cv[k][d];
+    }
+    moved[dst[k]] = c[k];
   }
 }
 
