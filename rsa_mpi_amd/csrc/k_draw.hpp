@@ -739,6 +739,11 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
       ok = !coords_ghost<D>(g, cc);
     }
   }
+  // The arrival slot does not depend on the filter's outcome, so it is taken
+  // before the stencil walk and the atomic's round trip overlaps the walk's
+  // loads.  With a closed gate nothing is written (the test below returns).
+  int a = -1;
+  if (valid && ok && sub == 0 && gate == draw_id) a = atomicAdd(&cellcc[cell], 1);
   bool hit = false;
   if (valid && ok) {
     int cc[D];
@@ -786,7 +791,6 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
     } else {
       ccell[i] = (int)cell;
       ctag[i] = anyhit ? T_SKIP : T_UNDECIDED;
-      const int a = atomicAdd(&cellcc[cell], 1);
       if (a < CSLOT)
         cslot[(size_t)cell * CSLOT + a] = i;
       else
