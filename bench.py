@@ -245,10 +245,12 @@ def cpu_baseline(side):
                        f"({side ** 3:.1%} of the workload volume) to maximal packing: {o['n']} spheres in {dt:.2f} s")
 
 
-def batched(R, workers):
+def batched(R, workers, active):
     """The batched workload: 256 independent 3D realizations (box edge L ~ U[0.2, 1],
-    r ~ U[0.004, 0.008], seeds 0..255, each to maximal packing), timed once `workers`
-    realizations at a time and once one after another; every placed set (extracted
+    r ~ U[0.004, 0.008], seeds 0..255, each to maximal packing), timed three ways:
+    as a device work queue (`active` realizations in flight on one stream, one
+    launch per step for all of them), as `workers` concurrent contexts on host
+    threads, and one realization after another.  Every placed set (extracted
     inside the timed region) is checked against the oracle's SHA-256."""
     from _common import set_hash
     fx = json.load(open(os.path.join(ROOT, "tests", "golden", "batch_3d_256.json")))
@@ -257,11 +259,16 @@ def batched(R, workers):
     r = [c["r"] for c in real]
     s = [c["seed"] for c in real]
     R.batch_run(3, L[:8], r[:8], s[:8], workers=workers)  # warm-up: contexts, pools, threads
+    R.batch_queue(3, L[:8], r[:8], s[:8], active=8)
     res = {}
-    for name, w in (("batched", workers), ("plain", 1)):
+    runs = (("queue", lambda: R.batch_queue(3, L, r, s, active=active, extract=True)),
+            ("threads", lambda: R.batch_run(3, L, r, s, workers=workers, extract=True)),
+            ("plain", lambda: R.batch_run(3, L, r, s, workers=1, extract=True)))
+    for name, fn in runs:
+        print(f"#   batched: {name}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         try:
-            out = R.batch_run(3, L, r, s, workers=w, extract=True)
+            out = fn()
         except RuntimeError as e:  # a failed device check or extraction checksum: reported, not a value
             res[name] = dict(seconds=None, value=None, sets_match_oracle="error", error=str(e)[:300])
             continue
@@ -270,12 +277,17 @@ def batched(R, workers):
                if len(rr) != fx["oracle"][k]["n"] or set_hash(p, rr) != fx["oracle"][k]["sha256"]]
         res[name] = dict(seconds=round(dt, 3), value=round(float(out["counts"].sum()) / dt, 1),
                          sets_match_oracle=f"{256 - len(bad)}/256", mismatched=bad[:8])
+        if bad:
+            res[name]["value"] = None  # a wrong set is not a result
+        if "stats" in out:
+            res[name]["stats"] = out["stats"]
+    q, p = res["queue"]["value"], res["plain"]["value"]
     return {"workload": "256 independent 3D monodisperse realizations, periodic box [0,L]^3, L~U[0.2,1], "
                         "r~U[0.004,0.008], seeds 0..255, each to maximal packing (extraction in the timed region)",
-            "realizations": len(real), "spheres": int(sum(c["n"] for c in fx["oracle"])), "workers": workers,
-            "unit": "spheres/s", "value": res["batched"]["value"], "plain_value": res["plain"]["value"],
-            "speedup_vs_plain": (round(res["batched"]["value"] / res["plain"]["value"], 3)
-                                 if res["batched"]["value"] and res["plain"]["value"] else None), **{
+            "realizations": len(real), "spheres": int(sum(c["n"] for c in fx["oracle"])), "active": active,
+            "workers": workers, "unit": "spheres/s", "value": q, "plain_value": p,
+            "speedup_vs_plain": round(q / p, 3) if q and p else None,
+            "threads_value": res["threads"]["value"], **{
                 k + "_" + kk: v for k, d in res.items() for kk, v in d.items() if kk != "value"}}
 
 
@@ -287,11 +299,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-side", type=float, default=0.4)
     ap.add_argument("--batch-workers", type=int, default=6,
-                    help="realizations run at once in the batched measurement")
+                    help="concurrent contexts of the host-thread form of the batched measurement")
+    ap.add_argument("--batch-active", type=int, default=64,
+                    help="realizations in flight in the device work queue form of the batched measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
     ap.add_argument("--no-secondary", action="store_true", help="skip the r=0.005 line")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned form of the headline")
-    ap.add_argument("--only", choices=["headline", "secondary", "learned"], default=None,
+    ap.add_argument("--only", choices=["headline", "secondary", "learned", "batched"], default=None,
                     help="run one workload's timed steps and nothing else (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (reference reads for rocprofv3 "
@@ -315,6 +329,9 @@ def main():
     R.lib()
     if args.only == "learned":
         print(json.dumps(learned(R, args.steps, None)), flush=True)
+        return
+    if args.only == "batched":
+        print(json.dumps(batched(R, args.batch_workers, args.batch_active)), flush=True)
         return
     if args.only:
         w = HEADLINE if args.only == "headline" else SECONDARY
@@ -345,7 +362,7 @@ def main():
     bat = None
     if not args.no_batch:
         print("# batched", file=sys.stderr, flush=True)
-        bat = batched(R, args.batch_workers)
+        bat = batched(R, args.batch_workers, args.batch_active)
     cpu = None
     if not args.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)

@@ -251,6 +251,37 @@ int rsa_gpu_batch_run(int dim, int n, const double* side, const double* radius, 
                       int nbshots, int workers, uint64_t* count, double* volume, double* pos,
                       double* radius_out, uint64_t cap, double* seconds);
 
+/* The batched form as a device work queue: realization k is the same
+ * uniform_generate<DIM, 1> run as in rsa_gpu_batch_run (operator_algorithm.ixx:275-292),
+ * but up to `active` realizations (at most 256) are in flight together on ONE
+ * stream driven by one host thread: every draw round and every refinement step
+ * of all the realizations that need it is one kernel launch over a table of
+ * per-realization descriptors, and a finished realization's place is taken by
+ * the next one.  Each realization's sequence of words, draws, rounds and
+ * refinements is the one rsa_gpu_proceed runs, so its placed set is the same
+ * (a refinement may wait for other realizations' draws: nothing it reads
+ * changes meanwhile).  Arguments and outputs as rsa_gpu_batch_run; stats (may
+ * be NULL) receives the launch-group counts.  No counterpart in the reference,
+ * which runs one realization per process. */
+typedef struct rsa_gpu_batch_stats {
+  double seconds;               /* wall time of the call */
+  int64_t ticks;                /* launch groups: draw rounds plus refinement groups */
+  int64_t draw_launch_groups;   /* draw rounds launched together */
+  int64_t refine_launch_groups; /* refinements launched together */
+  int64_t max_active;           /* realizations in flight at most */
+  int64_t draws;                /* draws of all realizations */
+  int64_t refines;              /* refinements of all realizations */
+  /* host time of the driver thread by phase (seconds): starting realizations,
+   * preparing draws (prepared words, word copies, descriptors), waiting for the
+   * rounds' results, refinement groups (launches and their one wait), finishing
+   * realizations (end-of-run checks, extraction) */
+  double t_admit, t_plan, t_wait, t_refine, t_finish;
+} rsa_gpu_batch_stats;
+
+int rsa_gpu_batch_queue(int dim, int n, const double* side, const double* radius, const uint64_t* seed,
+                        int nbshots, int active, uint64_t* count, double* volume, double* pos,
+                        double* radius_out, uint64_t cap, rsa_gpu_batch_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,7 @@ from .tuning import autotune, autotune_batch  # noqa: F401
 from ._lib import Stats, check, dptr, f64, lib, mt_words  # noqa: F401
 
 __all__ = ["Domain", "RadiusGenerator", "LawRadiusGenerator", "Algo", "Stats", "mt_words", "uniform_generate",
-           "throw_spheres", "batch_run", "write_binary", "read_binary", "set_nb_threads", "learn_reset",
+           "throw_spheres", "batch_run", "batch_queue", "write_binary", "read_binary", "set_nb_threads", "learn_reset",
            "learn_train", "learn_samples", "learn_params", "learn_gradcheck", "learn_forward"]
 
 
@@ -361,6 +361,41 @@ def batch_run(dim, sides, radii, seeds, nbshots=6000, workers=None, extract=Fals
                                   int(workers), counts.ctypes.data_as(_lib._U64), dptr(volumes), pp, rp, cap,
                                   dptr(secs)))
     out = {"counts": counts, "volumes": volumes, "seconds": float(secs[0])}
+    if extract:
+        res, off = [], 0
+        for c in counts.astype(np.int64):
+            res.append((pos[off * dim:(off + c) * dim].reshape(c, dim).copy(), rad[off:off + c].copy()))
+            off += c
+        out["spheres"] = res
+    return out
+
+
+def batch_queue(dim, sides, radii, seeds, nbshots=6000, active=64, extract=False, volumes=False):
+    """The same realizations as batch_run, driven as a device work queue: up to
+    `active` of them in flight on one stream, every draw round and refinement step
+    of all of them one launch (rsa_gpu_batch_queue).  Returns batch_run's dict
+    (volumes only when asked for) plus "stats" (launch-group counts, host time by
+    phase)."""
+    sides = f64(sides)
+    radii = f64(radii)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    n = len(sides)
+    if len(radii) != n or len(seeds) != n:
+        raise ValueError("sides, radii and seeds must have the same length")
+    counts = np.zeros(n, dtype=np.uint64)
+    vols = np.zeros(n, dtype=np.float64)
+    if extract:
+        cap = int(sum(float(s) ** dim / (2 * float(r) / np.sqrt(dim)) ** dim for s, r in zip(sides, radii))) + 16
+        pos = np.zeros(cap * dim, dtype=np.float64)
+        rad = np.zeros(cap, dtype=np.float64)
+        pp, rp = dptr(pos), dptr(rad)
+    else:
+        cap, pp, rp = 0, None, None
+    st = _lib.BatchStats()
+    check(lib().rsa_gpu_batch_queue(dim, n, dptr(sides), dptr(radii), seeds.ctypes.data_as(_lib._U64), int(nbshots),
+                                    int(active), counts.ctypes.data_as(_lib._U64), dptr(vols) if volumes else None,
+                                    pp, rp, cap, ctypes.byref(st)))
+    out = {"counts": counts, "volumes": vols, "seconds": st.seconds, "stats": st.as_dict()}
     if extract:
         res, off = [], 0
         for c in counts.astype(np.int64):

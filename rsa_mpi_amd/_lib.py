@@ -21,6 +21,7 @@ EXPORTED = (
     "rsa_gpu_reset_law", "rsa_gpu_extract_noalloc", "rsa_gpu_set_tuning", "rsa_gpu_get_tuning",
     "rsa_gpu_learn_reset", "rsa_gpu_learn_set_mode", "rsa_gpu_learn_get_stats", "rsa_gpu_learn_samples",
     "rsa_gpu_learn_train", "rsa_gpu_learn_params", "rsa_gpu_learn_gradcheck", "rsa_gpu_learn_forward",
+    "rsa_gpu_batch_queue",
 )
 
 # rsa_gpu_radius_law: void (*)(const double* u, double* radius, int64_t n, void* user)
@@ -52,6 +53,16 @@ class LearnStats(ctypes.Structure):
         "voxels_covered_predicted", "waves",
         "waves_uniform", "draws_predicted", "draws_mispredicted", "refines", "refines_predicted",
         "speculation_skipped", "traced_voxels", "traced_draws")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchStats(ctypes.Structure):
+    """rsa_gpu_batch_stats"""
+    _fields_ = [("seconds", ctypes.c_double)] + [(n, ctypes.c_int64) for n in (
+        "ticks", "draw_launch_groups", "refine_launch_groups", "max_active", "draws", "refines")] + [
+        (n, ctypes.c_double) for n in ("t_admit", "t_plan", "t_wait", "t_refine", "t_finish")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -115,6 +126,8 @@ def lib():
     L.rsa_gpu_learn_forward.argtypes = [ctypes.c_int, _F, _F, _F]
     L.rsa_gpu_batch_run.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _U64, ctypes.c_int, ctypes.c_int, _U64, _D,
                                     _D, _D, ctypes.c_uint64, _D]
+    L.rsa_gpu_batch_queue.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _U64, ctypes.c_int, ctypes.c_int, _U64, _D,
+                                      _D, _D, ctypes.c_uint64, ctypes.POINTER(BatchStats)]
     _lib = L
     return L
 

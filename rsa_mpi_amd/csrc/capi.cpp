@@ -363,4 +363,19 @@ int rsa_gpu_batch_run(int dim, int n, const double* side, const double* radius, 
   });
 }
 
+int rsa_gpu_batch_queue(int dim, int n, const double* side, const double* radius, const uint64_t* seed, int nbshots,
+                        int active, uint64_t* count, double* volume, double* pos, double* radius_out, uint64_t cap,
+                        rsa_gpu_batch_stats* stats) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!side || !radius || !seed || !count)))
+      throw rsa_amd::Error("batch: null or negative arguments");
+    if (dim < 2 || dim > 4) throw rsa_amd::Error("batch: dim must be 2, 3 or 4");
+    if ((pos == nullptr) != (radius_out == nullptr)) throw rsa_amd::Error("batch: pos and radius_out go together");
+    if (active < 1) throw rsa_amd::Error("batch: active must be at least 1");
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (n == 0) return;
+    rsa_amd::batch_queue_run(dim, n, side, radius, seed, nbshots, active, count, volume, pos, radius_out, cap, stats);
+  });
+}
+
 }  // extern "C"

@@ -49,6 +49,7 @@
 #include "k_learn.hpp"
 #include "k_extract.hpp"
 #include "k_checks.hpp"
+#include "k_batch.hpp"
 
 namespace rsa_amd {
 
@@ -94,7 +95,9 @@ static std::vector<std::array<int, D>> tab_corner_host() {  // auxiliary/Path.ix
 template <int D>
 class Engine final : public IEngine {
  public:
-  Engine(const double* ginf_, const double* gsup_, int ghost_layer, double domain_rad) {
+  // ext: a stream to run on instead of one of the engine's own (the batched form
+  // drives many engines on one stream)
+  Engine(const double* ginf_, const double* gsup_, int ghost_layer, double domain_rad, hipStream_t ext = nullptr) {
     if (ghost_layer < 1) throw Error("ghost_layer must be >= 1 for a periodic domain");
     gl_ = ghost_layer;
     drad_ = domain_rad;
@@ -111,7 +114,11 @@ class Engine final : public IEngine {
     // non-blocking: no implicit ordering with the null stream (every copy, fill
     // and launch of the engine is on its own streams, the stencil table of the
     // first define_grid included), so engines run side by side
-    HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    own_st_ = ext == nullptr;
+    if (own_st_)
+      HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    else
+      st_ = ext;
     HIP_OK(hipStreamCreateWithFlags(&cst_, hipStreamNonBlocking));
     // Cells of more than list_thr_ candidates send a draw's first round to the
     // host-built lists.  RSA_GPU_LIST_THRESHOLD lowers it (1 .. CSLOT) so that tests
@@ -124,7 +131,7 @@ class Engine final : public IEngine {
     } catch (...) {  // no destructor runs for a constructor that throws
       (void)hipStreamSynchronize(st_);
       (void)hipStreamDestroy(cst_);
-      (void)hipStreamDestroy(st_);
+      if (own_st_) (void)hipStreamDestroy(st_);
       throw;
     }
     hctr_ = static_cast<int*>(PinnedPool::get().alloc(sizeof(int) * (C_NUM + 4), hipHostMallocDefault));
@@ -165,7 +172,7 @@ class Engine final : public IEngine {
     for (auto& c : chunks_) (void)hipEventDestroy(c.ev);
     for (auto e : free_ev_) (void)hipEventDestroy(e);
     if (cst_) (void)hipStreamDestroy(cst_);
-    if (st_) (void)hipStreamDestroy(st_);
+    if (st_ && own_st_) (void)hipStreamDestroy(st_);
     PinnedPool::get().release(hctr_);
   }
 
@@ -501,6 +508,9 @@ class Engine final : public IEngine {
   Geo<D> geo_{};
   hipStream_t st_ = nullptr;
   hipStream_t cst_ = nullptr;  // copies of MT words into the device ring
+  bool own_st_ = true;         // st_ was created by the engine (not the batch driver's)
+  template <int DD>
+  friend class BatchDriver;
   int* hctr_ = nullptr;
   DevBuf<int> dctr_;
   int list_thr_ = CSLOT;  // see the constructor
@@ -1368,6 +1378,8 @@ class Engine final : public IEngine {
 
 // the host side of a draw: Engine<D> member functions
 #include "engine_draw.inl"
+// the batched form: many engines driven in lockstep on one stream
+#include "batch.inl"
 
 namespace rsa_amd {
 

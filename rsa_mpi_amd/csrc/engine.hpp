@@ -49,6 +49,11 @@ class IEngine {
 
 IEngine* make_engine(int dim, const double* inf, const double* sup, int ghost_layer, double domain_rad);
 
+// rsa_gpu_batch_queue: realizations in lockstep on one stream (engine.hip, batch.inl)
+void batch_queue_run(int dim, int n, const double* side, const double* radius, const uint64_t* seed, int nbshots,
+                     int active, uint64_t* count, double* volume, double* pos, double* radius_out, uint64_t cap,
+                     rsa_gpu_batch_stats* stats);
+
 // reference error messages (rsa_grid_traversal.ixx:22, operator_algorithm.ixx:28,246,
 // radius_generator.ixx:18, list_of_voxels.ixx:113, rsa_domain.ixx:140)
 struct Error : std::runtime_error {

@@ -663,7 +663,7 @@ constexpr int kPubSlots = 8;
 // and their images are validated); each candidate takes an arrival slot in its
 // cell (shot order is restored in k_lists).
 template <int D, bool WIDE>
-__global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
+__device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
                                                       int nshots, uint32_t N, const uint32_t* __restrict__ vpos,
                                                       const double* __restrict__ hrad, RejList rl,
                                                       int end_rel, VoxView<D> vx, ClassDev cls, Geo<D> g,
@@ -674,13 +674,13 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
                                                       int* __restrict__ cellcc, int* __restrict__ cslot,
                                                       int* __restrict__ ovhead, int* __restrict__ next,
                                                       DrawCtl* __restrict__ ctl, int draw_id, int forced,
-                                                      int list_thr) {
+                                                      int list_thr, const int bid) {
   using LG = LaneGroup<D>;
   // The gate does not change during this kernel.  It is read first but tested
   // only before the first store, so that its load overlaps the reads below
   // (all of which stay in bounds whatever the gate says).
   const int gate = forced ? draw_id : ctl->gate;
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gtid = bid * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
   const int sub = gtid % LG::LPC;
@@ -803,6 +803,22 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
   }
 }
 
+template <int D, bool WIDE>
+__global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
+                                                      int nshots, uint32_t N, const uint32_t* __restrict__ vpos,
+                                                      const double* __restrict__ hrad, RejList rl,
+                                                      int end_rel, VoxView<D> vx, ClassDev cls, Geo<D> g,
+                                                      CellStore<D> cs, double* __restrict__ cpos,
+                                                      double* __restrict__ crad, uint32_t* __restrict__ cprio,
+                                                      uint32_t* __restrict__ cphase, int* __restrict__ ccell,
+                                                      uint8_t* __restrict__ ctag, uint8_t* __restrict__ owner,
+                                                      int* __restrict__ cellcc, int* __restrict__ cslot,
+                                                      int* __restrict__ ovhead, int* __restrict__ next,
+                                                      DrawCtl* __restrict__ ctl, int draw_id, int forced,
+                                                      int list_thr) {
+  k_decode_filter_body<D, WIDE>(ring, mask, w0, nshots, N, vpos, hrad, rl, end_rel, vx, cls, g, cs, cpos, crad, cprio, cphase, ccell, ctag, owner, cellcc, cslot, ovhead, next, ctl, draw_id, forced, list_thr, (int)blockIdx.x);
+}
+
 // The list of a cell of n <= CSLOT candidates as k_lists builds it: the arrival
 // slots v (n entries) in shot order (a sorting network), then the round-0 Skips
 // removed from the back with swap-with-last (rsa_data_storage.ixx:108-116,
@@ -847,15 +863,15 @@ __device__ __forceinline__ int short_list(int n, int (&v)[CSLOT], const uint8_t*
 // shot order (build_grid appends in candidate order,
 // operator_cells_and_data_storage.ixx:153-159), then the round-0 Skip removal
 // from the back with swap-with-last (rsa_data_storage.ixx:108-116, 305-315).
-__global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict__ ccell,
+__device__ __forceinline__ void k_lists_body(int nshots, const int* __restrict__ ccell,
                                               const uint8_t* __restrict__ owner, const uint8_t* __restrict__ ctag,
                                               int* __restrict__ cellcc, const int* __restrict__ cslot,
                                               int* __restrict__ ovhead, const int* __restrict__ next,
                                               int* __restrict__ seg, int* __restrict__ len, int* __restrict__ ord,
                                               int* __restrict__ cell2t, DrawCtl* __restrict__ ctl, int draw_id,
-                                              int ncells) {
+                                              int ncells, const int bid) {
   const int gate = ctl->gate;  // tested before the first store (loads overlap)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bid * blockDim.x + threadIdx.x;
   const bool in = i < nshots;
   // Every thread reads, without a branch (index clamped to candidate 0), its
   // owner flag, cell, the cell's count and fixed slots, so that these loads are
@@ -913,16 +929,26 @@ __global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict_
   ovhead[c] = -1;
 }
 
+__global__ void __launch_bounds__(WG) k_lists(int nshots, const int* __restrict__ ccell,
+                                              const uint8_t* __restrict__ owner, const uint8_t* __restrict__ ctag,
+                                              int* __restrict__ cellcc, const int* __restrict__ cslot,
+                                              int* __restrict__ ovhead, const int* __restrict__ next,
+                                              int* __restrict__ seg, int* __restrict__ len, int* __restrict__ ord,
+                                              int* __restrict__ cell2t, DrawCtl* __restrict__ ctl, int draw_id,
+                                              int ncells) {
+  k_lists_body(nshots, ccell, owner, ctag, cellcc, cslot, ovhead, next, seg, len, ord, cell2t, ctl, draw_id, ncells, (int)blockIdx.x);
+}
+
 // Rounds >= 1 filter (operator_cells_and_data_storage.ixx:54-76): placed
 // spheres, spheres validated by earlier rounds of this draw, and their images.
 template <int D>
-__global__ void __launch_bounds__(WG) k_refilter(Geo<D> g, CellStore<D> cs, int nshots,
+__device__ __forceinline__ void k_refilter_body(Geo<D> g, CellStore<D> cs, int nshots,
                                                  const double* __restrict__ cpos, const double* __restrict__ crad,
                                                  const int* __restrict__ ccell, uint8_t* __restrict__ ctag,
                                                  const int* __restrict__ cell2t, const int* __restrict__ seg,
-                                                 const int* __restrict__ len, const int* __restrict__ ord) {
+                                                 const int* __restrict__ len, const int* __restrict__ ord, const int bid) {
   using LG = LaneGroup<D>;
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gtid = bid * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
   const int sub = gtid % LG::LPC;
@@ -980,9 +1006,18 @@ __global__ void __launch_bounds__(WG) k_refilter(Geo<D> g, CellStore<D> cs, int 
   if (active && sub == 0 && anyhit) ctag[i] = T_SKIP;
 }
 
-__global__ void k_remove_skips(int nshots, const uint8_t* __restrict__ owner, const int* __restrict__ seg,
-                               int* __restrict__ len, int* __restrict__ ord, const uint8_t* __restrict__ ctag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+template <int D>
+__global__ void __launch_bounds__(WG) k_refilter(Geo<D> g, CellStore<D> cs, int nshots,
+                                                 const double* __restrict__ cpos, const double* __restrict__ crad,
+                                                 const int* __restrict__ ccell, uint8_t* __restrict__ ctag,
+                                                 const int* __restrict__ cell2t, const int* __restrict__ seg,
+                                                 const int* __restrict__ len, const int* __restrict__ ord) {
+  k_refilter_body<D>(g, cs, nshots, cpos, crad, ccell, ctag, cell2t, seg, len, ord, (int)blockIdx.x);
+}
+
+__device__ __forceinline__ void k_remove_skips_body(int nshots, const uint8_t* __restrict__ owner, const int* __restrict__ seg,
+                               int* __restrict__ len, int* __restrict__ ord, const uint8_t* __restrict__ ctag, const int bid) {
+  const int i = bid * blockDim.x + threadIdx.x;
   if (i >= nshots || !owner[i]) return;
   int* a = ord + seg[i];
   int n = len[i];
@@ -993,6 +1028,11 @@ __global__ void k_remove_skips(int nshots, const uint8_t* __restrict__ owner, co
     }
   }
   len[i] = n;
+}
+
+__global__ void k_remove_skips(int nshots, const uint8_t* __restrict__ owner, const int* __restrict__ seg,
+                               int* __restrict__ len, int* __restrict__ ord, const uint8_t* __restrict__ ctag) {
+  k_remove_skips_body(nshots, owner, seg, len, ord, ctag, (int)blockIdx.x);
 }
 
 // Partners of one candidate held by one lane's stencil cells, in stencil then
@@ -1096,7 +1136,7 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
 // Conflict; the others get their ordered partner list (slab of KMAX entries,
 // longer lists from a pool) and are left unresolved for k_sweep_labels.
 template <int D>
-__global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, const double* __restrict__ cpos,
+__device__ __forceinline__ void k_partner_lists_body(Geo<D> g, int nshots, const double* __restrict__ cpos,
                                                  const double* __restrict__ crad, const uint32_t* __restrict__ cprio,
                                                  const int* __restrict__ ccell, const uint8_t* __restrict__ ctag,
                                                  uint8_t* __restrict__ lab, const int* __restrict__ cell2t,
@@ -1104,14 +1144,14 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
                                                  const int* __restrict__ ord, int* __restrict__ slab,
                                                  int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int gate_id,
                                                  URec* __restrict__ urec, const int* __restrict__ cellcc,
-                                                 const int* __restrict__ cslot) {
+                                                 const int* __restrict__ cslot, const int bid) {
   using LG = LaneGroup<D>;
   // A draw's first round (gate_id >= 0) builds the cell lists from the arrival
   // slots; the later ones read the lists k_update_commit kept.
   const bool first = gate_id >= 0;
   const int gate = first ? ctl->gate : gate_id;  // tested before the first store (loads overlap)
   const int long_list = first ? ctl->long_list : 0;
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gtid = bid * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
   const int sub = gtid % LG::LPC;
@@ -1201,6 +1241,19 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
   }
 }
 
+template <int D>
+__global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, const double* __restrict__ cpos,
+                                                 const double* __restrict__ crad, const uint32_t* __restrict__ cprio,
+                                                 const int* __restrict__ ccell, const uint8_t* __restrict__ ctag,
+                                                 uint8_t* __restrict__ lab, const int* __restrict__ cell2t,
+                                                 const int* __restrict__ seg, const int* __restrict__ len,
+                                                 const int* __restrict__ ord, int* __restrict__ slab,
+                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int gate_id,
+                                                 URec* __restrict__ urec, const int* __restrict__ cellcc,
+                                                 const int* __restrict__ cslot) {
+  k_partner_lists_body<D>(g, nshots, cpos, crad, cprio, ccell, ctag, lab, cell2t, seg, len, ord, slab, pool, pool_cap, ctl, gate_id, urec, cellcc, cslot, (int)blockIdx.x);
+}
+
 // The sequential priority sweep (find_solve_conflicts,
 // operator_cells_and_data_storage.ixx:127-142) as a dependency evaluation: a
 // candidate's label is fixed by its first partner (detection order) that is not
@@ -1210,10 +1263,10 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
 // passes reach the same labels.  One workgroup, labels in LDS (nshots bytes of
 // dynamic shared memory).  The round's counts are left in DrawCtl::pubv, and
 // k_update_commit writes them to pinned host memory.
-__global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
+__device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
                                                 const int* __restrict__ slab, const int* __restrict__ pool,
                                                 DrawCtl* ctl, int seq, int gate_id, int draw_id,
-                                                double aimed, uint64_t miss_base, const URec* __restrict__ urec) {
+                                                double aimed, uint64_t miss_base, const URec* __restrict__ urec, const int bid) {
   extern __shared__ __align__(16) uint8_t s_lab[];
   __shared__ int s_prog, s_left, s_acc, s_con, s_stuck;
   // everything below reads first and stores only once the gate is known open
@@ -1420,12 +1473,19 @@ __global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t
   for (int k = 0; k < kPubFields; k++) ctl->pubv[k] = v[k];  // published by k_update_commit
 }
 
+__global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
+                                                const int* __restrict__ slab, const int* __restrict__ pool,
+                                                DrawCtl* ctl, int seq, int gate_id, int draw_id,
+                                                double aimed, uint64_t miss_base, const URec* __restrict__ urec) {
+  k_sweep_labels_body(nshots, ctag, lab, slab, pool, ctl, seq, gate_id, draw_id, aimed, miss_base, urec, (int)blockIdx.x);
+}
+
 // update_sphere_tags on the touched real cells (operator_algorithm.ixx:446-473),
 // one thread per cell owner: Skip removed from the back, Undecided ->
 // Validated, Conflict -> Undecided.  When the round left no Conflict the draw
 // is over and the survivors are appended to their cells in tail order.
 template <int D>
-__global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshots, const double* __restrict__ cpos,
+__device__ __forceinline__ void k_update_commit_body(CellStore<D> cs, int nshots, const double* __restrict__ cpos,
                                                       const double* __restrict__ crad,
                                                       const uint32_t* __restrict__ cphase,
                                                       const int* __restrict__ ccell, const uint8_t* __restrict__ owner,
@@ -1434,8 +1494,8 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
                                                       int* __restrict__ ord, int* __restrict__ cell2t,
                                                       DrawCtl* __restrict__ ctl, int seq, DrawPub* host, int first,
                                                       int* __restrict__ cellcc, const int* __restrict__ cslot,
-                                                      int ncells) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                      int ncells, const int bid) {
+  const int i = bid * blockDim.x + threadIdx.x;
   const int swept = ctl->swept;  // all reads are issued before the tests
   if (i == 0 && swept == seq) {
     // the round's results to the host, as the sweep left them (seq last)
@@ -1622,6 +1682,20 @@ __global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshot
   }
   cs.cnt[c] = (uint8_t)k0;
   cell2t[c] = -1;
+}
+
+template <int D>
+__global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshots, const double* __restrict__ cpos,
+                                                      const double* __restrict__ crad,
+                                                      const uint32_t* __restrict__ cphase,
+                                                      const int* __restrict__ ccell, const uint8_t* __restrict__ owner,
+                                                      uint8_t* __restrict__ ctag, const uint8_t* __restrict__ lab,
+                                                      int* __restrict__ seg, int* __restrict__ len,
+                                                      int* __restrict__ ord, int* __restrict__ cell2t,
+                                                      DrawCtl* __restrict__ ctl, int seq, DrawPub* host, int first,
+                                                      int* __restrict__ cellcc, const int* __restrict__ cslot,
+                                                      int ncells) {
+  k_update_commit_body<D>(cs, nshots, cpos, crad, cphase, ccell, owner, ctag, lab, seg, len, ord, cell2t, ctl, seq, host, first, cellcc, cslot, ncells, (int)blockIdx.x);
 }
 
 }  // namespace rsa_amd

@@ -107,16 +107,16 @@ __global__ void __launch_bounds__(WG) k_compact(int64_t ncells, CellStore<D> cs,
 // Interior voxels of D <= 3 read the packed copy (coff, csph) by stencil rows
 // when the host made one (coff not null); the others read the cell store.
 template <int D, bool WIDE>
-__global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView<D> vx, int64_t nvox,
+__device__ __forceinline__ void k_cover_body(Geo<D> g, CellStore<D> cs, VoxView<D> vx, int64_t nvox,
                                               VoxParams<D> vp, const uint32_t* __restrict__ perm,
                                               const int* __restrict__ coff, const Slot<D>* __restrict__ csph,
-                                              int ncs, uint32_t* __restrict__ out, int* __restrict__ blk) {
+                                              int ncs, uint32_t* __restrict__ out, int* __restrict__ blk, const int bid) {
   using TC = TabCorner<D>;
   constexpr int NCH = ipow_c(2, D);
   constexpr int NST = ipow_c(3, D);
   constexpr uint32_t FULL = (NCH == 32) ? 0xffffffffu : ((1u << NCH) - 1u);
   // thread t takes voxel perm[t] (the cell-grouped order) or, without perm, voxel t
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)bid * blockDim.x + threadIdx.x;
   const int64_t i = perm == nullptr ? t : (t < nvox ? (int64_t)perm[t] : nvox);
   // The stencil as NR rows of three cells along dimension 0 (any order:
   // coverage is an OR): the linear offset of each row's first cell.
@@ -308,8 +308,16 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
   }
   if (perm == nullptr) {  // uniform: per-block counts of uncovered voxels
     const int u = __syncthreads_count(code != 0);
-    if (threadIdx.x == 0) blk[blockIdx.x] = u;
+    if (threadIdx.x == 0) blk[bid] = u;
   }
+}
+
+template <int D, bool WIDE>
+__global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView<D> vx, int64_t nvox,
+                                              VoxParams<D> vp, const uint32_t* __restrict__ perm,
+                                              const int* __restrict__ coff, const Slot<D>* __restrict__ csph,
+                                              int ncs, uint32_t* __restrict__ out, int* __restrict__ blk) {
+  k_cover_body<D, WIDE>(g, cs, vx, nvox, vp, perm, coff, csph, ncs, out, blk, (int)blockIdx.x);
 }
 
 // k_cover's processing order: the voxels grouped by the cell of their centre
@@ -446,8 +454,8 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
   return x;
 }
 
-__global__ void __launch_bounds__(1024) k_tile_sums(const int* __restrict__ a, int64_t n, int* __restrict__ sums) {
-  const int64_t base = (int64_t)blockIdx.x * SCAN_T + threadIdx.x * 4;
+__device__ __forceinline__ void k_tile_sums_body(const int* __restrict__ a, int64_t n, int* __restrict__ sums, const int bid) {
+  const int64_t base = (int64_t)bid * SCAN_T + threadIdx.x * 4;
   int s = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++) s += base + k < n ? a[base + k] : 0;
@@ -458,12 +466,16 @@ __global__ void __launch_bounds__(1024) k_tile_sums(const int* __restrict__ a, i
   if (threadIdx.x == 0) {
     int tot = 0;
     for (int w = 0; w < 16; w++) tot += sh[w];
-    sums[blockIdx.x] = tot;
+    sums[bid] = tot;
   }
 }
 
-__global__ void __launch_bounds__(1024) k_tile_scan(int* __restrict__ a, int64_t n, const int* __restrict__ sums) {
-  const int64_t base = (int64_t)blockIdx.x * SCAN_T + threadIdx.x * 4;
+__global__ void __launch_bounds__(1024) k_tile_sums(const int* __restrict__ a, int64_t n, int* __restrict__ sums) {
+  k_tile_sums_body(a, n, sums, (int)blockIdx.x);
+}
+
+__device__ __forceinline__ void k_tile_scan_body(int* __restrict__ a, int64_t n, const int* __restrict__ sums, const int bid) {
+  const int64_t base = (int64_t)bid * SCAN_T + threadIdx.x * 4;
   int v[4], s = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -474,7 +486,7 @@ __global__ void __launch_bounds__(1024) k_tile_scan(int* __restrict__ a, int64_t
   const int x = wave_incl_scan(s);
   if ((threadIdx.x & 63) == 63) sh[threadIdx.x >> 6] = x;
   __syncthreads();
-  int run = sums[blockIdx.x] + x - s;
+  int run = sums[bid] + x - s;
   for (int w = 0; w < (int)(threadIdx.x >> 6); w++) run += sh[w];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -483,9 +495,13 @@ __global__ void __launch_bounds__(1024) k_tile_scan(int* __restrict__ a, int64_t
   }
 }
 
+__global__ void __launch_bounds__(1024) k_tile_scan(int* __restrict__ a, int64_t n, const int* __restrict__ sums) {
+  k_tile_scan_body(a, n, sums, (int)blockIdx.x);
+}
+
 // single workgroup exclusive scan over block counts; writes the total to *total
 // (*total = -1 when the sum exceeds INT_MAX: the offsets would not fit)
-__global__ void __launch_bounds__(1024) k_scan_blocks(int* __restrict__ blk, int nb, int* __restrict__ total) {
+__device__ __forceinline__ void k_scan_blocks_body(int* __restrict__ blk, int nb, int* __restrict__ total, const int bid) {
   __shared__ int sh[1024];
   long long carry = 0;
   for (int base = 0; base < nb; base += 1024) {
@@ -507,15 +523,19 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(int* __restrict__ blk, int
   if (threadIdx.x == 0) *total = carry > INT_MAX ? -1 : (int)carry;
 }
 
+__global__ void __launch_bounds__(1024) k_scan_blocks(int* __restrict__ blk, int nb, int* __restrict__ total) {
+  k_scan_blocks_body(blk, nb, total, (int)blockIdx.x);
+}
+
 // swap-with-last compaction of remove_covered (list_of_voxels.ixx:42-53) in
 // closed form: with keep uncovered voxels, the covered positions below keep,
 // in increasing order, receive the uncovered voxels at or above keep in
 // decreasing order.  Step 1 lists the holes.
 // The voxel at keep - 1 also leaves the number of holes (= voxels step 2 moves).
-__global__ void __launch_bounds__(WG) k_holes(int64_t nvox, const uint32_t* __restrict__ code,
+__device__ __forceinline__ void k_holes_body(int64_t nvox, const uint32_t* __restrict__ code,
                                               const int* __restrict__ blk, int* __restrict__ ctr,
-                                              int* __restrict__ holes) {
-  const int64_t b = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+                                              int* __restrict__ holes, const int bid) {
+  const int64_t b = (int64_t)bid * (WG / 64) + (threadIdx.x >> 6);
   if (b * WG >= nvox) return;  // whole waves past the list
   const int64_t i0 = b * WG + (threadIdx.x & 63) * VPL;
   uint32_t c[VPL];
@@ -536,6 +556,12 @@ __global__ void __launch_bounds__(WG) k_holes(int64_t nvox, const uint32_t* __re
   }
 }
 
+__global__ void __launch_bounds__(WG) k_holes(int64_t nvox, const uint32_t* __restrict__ code,
+                                              const int* __restrict__ blk, int* __restrict__ ctr,
+                                              int* __restrict__ holes) {
+  k_holes_body(nvox, code, blk, ctr, holes, (int)blockIdx.x);
+}
+
 // Step 2: the uncovered voxels at or above keep move into the holes.  Their
 // codes go to `moved` (at the hole's index), not into `code`: the workgroup
 // that straddles keep reads code[] below keep for its ranks while other
@@ -543,11 +569,11 @@ __global__ void __launch_bounds__(WG) k_holes(int64_t nvox, const uint32_t* __re
 // Afterwards the code of voxel i < keep is code[i], or moved[i] where code[i]
 // is 0 (every hole is filled).
 template <int D, bool WIDE>
-__global__ void __launch_bounds__(WG) k_move(int64_t nvox, const uint32_t* __restrict__ code,
+__device__ __forceinline__ void k_move_body(int64_t nvox, const uint32_t* __restrict__ code,
                                              uint32_t* __restrict__ moved, const int* __restrict__ blk,
                                              const int* __restrict__ ctr, const int* __restrict__ holes,
-                                             VoxView<D> vx) {
-  const int64_t b = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+                                             VoxView<D> vx, const int bid) {
+  const int64_t b = (int64_t)bid * (WG / 64) + (threadIdx.x >> 6);
   if (b * WG >= nvox) return;  // whole waves past the list
   const int64_t i0 = b * WG + (threadIdx.x & 63) * VPL;
   uint32_t c[VPL];
@@ -587,6 +613,14 @@ __global__ void __launch_bounds__(WG) k_move(int64_t nvox, const uint32_t* __res
   }
 }
 
+template <int D, bool WIDE>
+__global__ void __launch_bounds__(WG) k_move(int64_t nvox, const uint32_t* __restrict__ code,
+                                             uint32_t* __restrict__ moved, const int* __restrict__ blk,
+                                             const int* __restrict__ ctr, const int* __restrict__ holes,
+                                             VoxView<D> vx) {
+  k_move_body<D, WIDE>(nvox, code, moved, blk, ctr, holes, vx, (int)blockIdx.x);
+}
+
 __device__ __forceinline__ uint32_t kept_code(const uint32_t* code, const uint32_t* moved, int64_t i) {
   const uint32_t c = code[i];
   return c != 0 ? c : moved[i];
@@ -623,10 +657,10 @@ __device__ __forceinline__ uint32_t kept_code_now(const uint32_t* code, const ui
   return c != 0 ? c : m;
 }
 
-__global__ void __launch_bounds__(WG) k_child_count(int64_t n, const int* __restrict__ ctr,
+__device__ __forceinline__ void k_child_count_body(int64_t n, const int* __restrict__ ctr,
                                                     const uint32_t* __restrict__ code,
-                                                    const uint32_t* __restrict__ moved, int* __restrict__ blk) {
-  const int64_t b = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+                                                    const uint32_t* __restrict__ moved, int* __restrict__ blk, const int bid) {
+  const int64_t b = (int64_t)bid * (WG / 64) + (threadIdx.x >> 6);
   const int64_t i0 = b * WG + (threadIdx.x & 63) * VPL;
   const int64_t keep = ctr[C_KEEP];
   const int64_t lim = n < keep ? n : keep;
@@ -640,16 +674,22 @@ __global__ void __launch_bounds__(WG) k_child_count(int64_t n, const int* __rest
   if ((threadIdx.x & 63) == 0 && b * WG < n) blk[b] = t;
 }
 
+__global__ void __launch_bounds__(WG) k_child_count(int64_t n, const int* __restrict__ ctr,
+                                                    const uint32_t* __restrict__ code,
+                                                    const uint32_t* __restrict__ moved, int* __restrict__ blk) {
+  k_child_count_body(n, ctr, code, moved, blk, (int)blockIdx.x);
+}
+
 // stable append of the kept children, parents in list order, children in
 // TabCorner order (list_of_voxels.ixx:76-99): coordinates 2c + tab[k]
 // SW / DW: the source and destination lattices hold 64-bit coordinates (the
 // launch picks the instance, so coordinate loads and stores carry no branch).
 template <int D, bool SW, bool DW>
-__global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __restrict__ code,
+__device__ __forceinline__ void k_scatter_body(int64_t n, const uint32_t* __restrict__ code,
                                                 const uint32_t* __restrict__ moved, const int* __restrict__ blk,
                                                 VoxView<D> src, VoxView<D> dst, int64_t nchild,
-                                                int* __restrict__ err) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                int* __restrict__ err, const int bid) {
+  const int64_t i = (int64_t)bid * blockDim.x + threadIdx.x;
   const bool in = i < n;
   const int64_t ic = in ? i : 0;
   const uint32_t m = in ? (kept_code_now(code, moved, ic) & 0xffffu) : 0u;
@@ -659,7 +699,7 @@ __global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __res
   const int c = __popc(m);
   int total;
   const int excl = block_excl_scan(c, total);
-  const int base = blk[blockIdx.x];  // the workgroup's children are base .. base + total - 1
+  const int base = blk[bid];  // the workgroup's children are base .. base + total - 1
   constexpr int NCH = ipow_c(2, D);
   constexpr uint32_t FULLM = (1u << NCH) - 1u;  // NCH <= 16
   // The children must agree with the counting pass (k_child_count + scan): the
@@ -705,6 +745,14 @@ __global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __res
       o++;
     }
   }
+}
+
+template <int D, bool SW, bool DW>
+__global__ void __launch_bounds__(WG) k_scatter(int64_t n, const uint32_t* __restrict__ code,
+                                                const uint32_t* __restrict__ moved, const int* __restrict__ blk,
+                                                VoxView<D> src, VoxView<D> dst, int64_t nchild,
+                                                int* __restrict__ err) {
+  k_scatter_body<D, SW, DW>(n, code, moved, blk, src, dst, nchild, err, (int)blockIdx.x);
 }
 
 // initial lattice, x fastest (list_of_voxels.ixx:11-28, Loops.ixx:44-74)

@@ -81,3 +81,64 @@ def test_batch256_subset0_one_worker_matches_oracle():
         o = fx["oracle"][k]
         assert len(rad) == o["n"], k
         assert set_hash(pos, rad) == o["sha256"], k
+
+
+# ---------------------------------------------------------------- the device work queue (rsa_gpu_batch_queue)
+
+def test_batch_queue_rejects_bad_arguments():
+    import rsa_mpi_amd as R
+    with pytest.raises(RuntimeError, match="dim must be"):
+        R.batch_queue(5, [1.0], [0.01], [0])
+    with pytest.raises(RuntimeError, match="active must be"):
+        R.batch_queue(3, [1.0], [0.01], [0], active=0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,sides,radii,seeds,active", [
+    (2, [0.3, 0.5, 0.21, 0.4, 0.35, 0.6, 0.25], [0.01, 0.012, 0.008, 0.015, 0.011, 0.02, 0.009], [3, 1, 4, 1, 5, 9, 2], 3),
+    (3, [0.12, 0.15, 0.1, 0.2, 0.11], [0.01, 0.012, 0.009, 0.015, 0.008], [0, 7, 11, 2, 5], 5),
+    (3, [0.12, 0.15, 0.1, 0.2, 0.11], [0.01, 0.012, 0.009, 0.015, 0.008], [0, 7, 11, 2, 5], 1),
+    (4, [0.3, 0.25], [0.04, 0.035], [1, 2], 2),
+])
+def test_batch_queue_small_bit_exact(dim, sides, radii, seeds, active):
+    """Realizations driven together (one launch per step for all of them) give the
+    oracle's uniform_generate<dim, 1> result for each, in extraction order."""
+    import rsa_mpi_amd as R
+    import _oracle as O
+    print(f"batch_queue dim={dim} n={len(sides)} active={active}", flush=True)
+    out = R.batch_queue(dim, sides, radii, seeds, active=active, extract=True)
+    assert 1 <= out["stats"]["max_active"] <= min(active, len(sides))
+    for k, (pos, rad) in enumerate(out["spheres"]):
+        o = O.run(dim, [0.0] * dim, [sides[k]] * dim, [(radii[k], 1.0, 0)], seed=seeds[k])
+        assert len(rad) == o["n"] and o["n"] > 0, k
+        assert np.array_equal(pos.view(np.uint64), o["pos"].view(np.uint64)), k
+        assert np.array_equal(rad.view(np.uint64), o["rad"].view(np.uint64)), k
+
+
+@pytest.mark.gpu
+def test_batch_queue_list_redo_path_bit_exact(monkeypatch):
+    """With the first-round list limit lowered to 1 the host-built lists (k_lists)
+    and the redone first rounds run inside the queue's rounds."""
+    import rsa_mpi_amd as R
+    import _oracle as O
+    monkeypatch.setenv("RSA_GPU_LIST_THRESHOLD", "1")
+    sides, radii, seeds = [0.15, 0.12, 0.2], [0.01, 0.012, 0.014], [4, 8, 15]
+    out = R.batch_queue(3, sides, radii, seeds, active=3, extract=True)
+    for k, (pos, rad) in enumerate(out["spheres"]):
+        o = O.run(3, [0.0] * 3, [sides[k]] * 3, [(radii[k], 1.0, 0)], seed=seeds[k])
+        assert np.array_equal(pos.view(np.uint64), o["pos"].view(np.uint64)), k
+
+
+@pytest.mark.gpu
+def test_batch_queue_256_sets_match_oracle():
+    """All 256 realizations of the batched workload, 64 in flight: every placed set's
+    SHA-256 equals the oracle's."""
+    import rsa_mpi_amd as R
+    fx = load()
+    real = fx["realizations"]
+    out = R.batch_queue(3, [c["L"] for c in real], [c["r"] for c in real], [c["seed"] for c in real], active=64,
+                        extract=True)
+    bad = [k for k, (pos, rad) in enumerate(out["spheres"])
+           if len(rad) != fx["oracle"][k]["n"] or set_hash(pos, rad) != fx["oracle"][k]["sha256"]]
+    assert not bad, bad[:10]
+    print("batch_queue 256:", out["seconds"], out["stats"], flush=True)
