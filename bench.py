@@ -245,26 +245,34 @@ def cpu_baseline(side):
                        f"({side ** 3:.1%} of the workload volume) to maximal packing: {o['n']} spheres in {dt:.2f} s")
 
 
-def batched(R, workers, active):
+def batched(R, active):
     """The batched workload: 256 independent 3D realizations (box edge L ~ U[0.2, 1],
-    r ~ U[0.004, 0.008], seeds 0..255, each to maximal packing), timed three ways:
-    as a device work queue (`active` realizations in flight on one stream, one
-    launch per step for all of them), as `workers` concurrent contexts on host
-    threads, and one realization after another.  Every placed set (extracted
-    inside the timed region) is checked against the oracle's SHA-256."""
+    r ~ U[0.004, 0.008], seeds 0..255, each to maximal packing), timed two ways: as
+    the device work queue (`active` realizations in flight on one stream, one launch
+    per step for all of them) and one realization after another through the
+    single-context API (uniform_generate + extraction per realization).  Every placed
+    set (extracted inside the timed region) is checked against the oracle's SHA-256."""
     from _common import set_hash
     fx = json.load(open(os.path.join(ROOT, "tests", "golden", "batch_3d_256.json")))
     real = fx["realizations"]
     L = [c["L"] for c in real]
     r = [c["r"] for c in real]
     s = [c["seed"] for c in real]
-    R.batch_run(3, L[:8], r[:8], s[:8], workers=workers)  # warm-up: contexts, pools, threads
-    R.batch_queue(3, L[:8], r[:8], s[:8], active=8)
+    R.batch_queue(3, L[:8], r[:8], s[:8], active=8)  # warm-up: pools, threads
+
+    def one_at_a_time():
+        out = []
+        for k in range(len(real)):
+            dom = R.Domain([0.0] * 3, [L[k]] * 3, r[k])
+            R.uniform_generate(dom, r[k], 6000, 10, s[k])
+            p_, r_, _ = dom.extract_arrays()
+            out.append((p_, r_))
+            del dom
+        return {"spheres": out, "counts": np.array([len(x[1]) for x in out], dtype=np.uint64)}
+
     res = {}
-    runs = (("queue", lambda: R.batch_queue(3, L, r, s, active=active, extract=True)),
-            ("threads", lambda: R.batch_run(3, L, r, s, workers=workers, extract=True)),
-            ("plain", lambda: R.batch_run(3, L, r, s, workers=1, extract=True)))
-    for name, fn in runs:
+    for name, fn in (("queue", lambda: R.batch_queue(3, L, r, s, active=active, extract=True)),
+                     ("plain", one_at_a_time)):
         print(f"#   batched: {name}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         try:
@@ -280,14 +288,13 @@ def batched(R, workers, active):
         if bad:
             res[name]["value"] = None  # a wrong set is not a result
         if "stats" in out:
-            res[name]["stats"] = out["stats"]
+            res[name]["stats"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in out["stats"].items()}
     q, p = res["queue"]["value"], res["plain"]["value"]
     return {"workload": "256 independent 3D monodisperse realizations, periodic box [0,L]^3, L~U[0.2,1], "
                         "r~U[0.004,0.008], seeds 0..255, each to maximal packing (extraction in the timed region)",
             "realizations": len(real), "spheres": int(sum(c["n"] for c in fx["oracle"])), "active": active,
-            "workers": workers, "unit": "spheres/s", "value": q, "plain_value": p,
-            "speedup_vs_plain": round(q / p, 3) if q and p else None,
-            "threads_value": res["threads"]["value"], **{
+            "unit": "spheres/s", "value": q, "plain_value": p,
+            "speedup_vs_plain": round(q / p, 3) if q and p else None, **{
                 k + "_" + kk: v for k, d in res.items() for kk, v in d.items() if kk != "value"}}
 
 
@@ -298,8 +305,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-side", type=float, default=0.4)
-    ap.add_argument("--batch-workers", type=int, default=6,
-                    help="concurrent contexts of the host-thread form of the batched measurement")
     ap.add_argument("--batch-active", type=int, default=64,
                     help="realizations in flight in the device work queue form of the batched measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched measurement")
@@ -331,7 +336,7 @@ def main():
         print(json.dumps(learned(R, args.steps, None)), flush=True)
         return
     if args.only == "batched":
-        print(json.dumps(batched(R, args.batch_workers, args.batch_active)), flush=True)
+        print(json.dumps(batched(R, args.batch_active)), flush=True)
         return
     if args.only:
         w = HEADLINE if args.only == "headline" else SECONDARY
@@ -362,7 +367,7 @@ def main():
     bat = None
     if not args.no_batch:
         print("# batched", file=sys.stderr, flush=True)
-        bat = batched(R, args.batch_workers, args.batch_active)
+        bat = batched(R, args.batch_active)
     cpu = None
     if not args.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)

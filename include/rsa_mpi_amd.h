@@ -238,15 +238,14 @@ int rsa_gpu_mt_words(uint64_t seed, uint64_t rank, uint64_t n, uint32_t* out);
  * uniform_generate<DIM, 1>(domain, radius[k], nbshots, 10, seed[k])
  * (operator_algorithm.ixx:275-292) on the periodic domain [0, side[k]]^dim
  * with ghost layer 1 and domain radius radius[k], run to maximal packing: what
- * the reference runs once per process.  `workers` contexts run at once, each
- * with its own streams and host threads, taking realizations from a shared
- * counter; results do not depend on workers.  count[k] receives the spheres of
- * realization k, volume[k] (volume may be NULL) their total volume.  With pos
- * non-NULL, every realization's spheres are extracted (extract order),
- * realization after realization in index order (pos is cap*dim, radius_out
- * cap); their total must fit cap.  seconds (may be NULL) receives the batch's
- * wall time.  No counterpart in the reference, which runs one realization per
- * process. */
+ * the reference runs once per process.  The realizations run as the device
+ * work queue of rsa_gpu_batch_queue with `workers` of them in flight; results
+ * do not depend on workers.  count[k] receives the spheres of realization k,
+ * volume[k] (volume may be NULL) their total volume.  With pos non-NULL, every
+ * realization's spheres are extracted (extract order), realization after
+ * realization in index order (pos is cap*dim, radius_out cap); their total
+ * must fit cap.  seconds (may be NULL) receives the batch's wall time.  No
+ * counterpart in the reference, which runs one realization per process. */
 int rsa_gpu_batch_run(int dim, int n, const double* side, const double* radius, const uint64_t* seed,
                       int nbshots, int workers, uint64_t* count, double* volume, double* pos,
                       double* radius_out, uint64_t cap, double* seconds);
@@ -276,6 +275,9 @@ typedef struct rsa_gpu_batch_stats {
    * rounds' results, refinement groups (launches and their one wait), finishing
    * realizations (end-of-run checks, extraction) */
   double t_admit, t_plan, t_wait, t_refine, t_finish;
+  /* parts of t_plan: waiting for prepared draws (helper threads), making their
+   * words readable on the device; part of t_refine: waiting for the counting passes */
+  double t_prep, t_words, t_refine_wait;
 } rsa_gpu_batch_stats;
 
 int rsa_gpu_batch_queue(int dim, int n, const double* side, const double* radius, const uint64_t* seed,

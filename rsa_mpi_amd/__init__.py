@@ -330,17 +330,25 @@ def read_binary(path):
     return rec["c"].copy(), rec["r"].copy(), rec["p"].copy()
 
 
+def _sphere_cap(dim, sides, radii):
+    """Upper bound on the spheres of the boxes: disjoint balls fill at most the
+    whole volume (plus one per box for rounding)."""
+    vball = {2: np.pi, 3: 4.0 * np.pi / 3.0, 4: np.pi ** 2 / 2.0}[int(dim)]
+    return int(sum(float(s) ** dim / (vball * float(r) ** dim) + 1 for s, r in zip(sides, radii))) + 16
+
+
 def batch_run(dim, sides, radii, seeds, nbshots=6000, workers=None, extract=False):
     """Independent realizations, realization k being uniform_generate<dim, 1> on
     the periodic box [0, sides[k]]^dim with radius radii[k] and seed seeds[k]
-    (operator_algorithm.ixx:275-292), `workers` of them at once on the GPU (None:
-    the count autotune_batch() stored for this dimension and median box-to-radius
-    ratio, else 4).  Returns a dict: counts, volumes, seconds and, with extract,
-    per realization the (pos, rad) arrays in extraction order."""
+    (operator_algorithm.ixx:275-292), run as a device work queue with `workers` of
+    them in flight on one stream (None: the count autotune_batch() stored for this
+    dimension and median box-to-radius ratio, else 64).  Returns a dict: counts,
+    volumes, seconds and, with extract, per realization the (pos, rad) arrays in
+    extraction order."""
     sides = f64(sides)
     radii = f64(radii)
     if workers is None:
-        workers = _tuning.lookup_batch(dim, sides, radii) or 4
+        workers = _tuning.lookup_batch(dim, sides, radii) or 64
     seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
     n = len(sides)
     if len(radii) != n or len(seeds) != n:
@@ -351,7 +359,7 @@ def batch_run(dim, sides, radii, seeds, nbshots=6000, workers=None, extract=Fals
     if extract:
         # upper bound on the spheres of a box: its volume over the volume of the
         # cube inscribed in one sphere
-        cap = int(sum(float(s) ** dim / (2 * float(r) / np.sqrt(dim)) ** dim for s, r in zip(sides, radii))) + 16
+        cap = _sphere_cap(dim, sides, radii)
         pos = np.zeros(cap * dim, dtype=np.float64)
         rad = np.zeros(cap, dtype=np.float64)
         pp, rp = dptr(pos), dptr(rad)
@@ -364,7 +372,7 @@ def batch_run(dim, sides, radii, seeds, nbshots=6000, workers=None, extract=Fals
     if extract:
         res, off = [], 0
         for c in counts.astype(np.int64):
-            res.append((pos[off * dim:(off + c) * dim].reshape(c, dim).copy(), rad[off:off + c].copy()))
+            res.append((pos[off * dim:(off + c) * dim].reshape(c, dim), rad[off:off + c]))  # views
             off += c
         out["spheres"] = res
     return out
@@ -385,7 +393,7 @@ def batch_queue(dim, sides, radii, seeds, nbshots=6000, active=64, extract=False
     counts = np.zeros(n, dtype=np.uint64)
     vols = np.zeros(n, dtype=np.float64)
     if extract:
-        cap = int(sum(float(s) ** dim / (2 * float(r) / np.sqrt(dim)) ** dim for s, r in zip(sides, radii))) + 16
+        cap = _sphere_cap(dim, sides, radii)
         pos = np.zeros(cap * dim, dtype=np.float64)
         rad = np.zeros(cap, dtype=np.float64)
         pp, rp = dptr(pos), dptr(rad)
@@ -399,7 +407,7 @@ def batch_queue(dim, sides, radii, seeds, nbshots=6000, active=64, extract=False
     if extract:
         res, off = [], 0
         for c in counts.astype(np.int64):
-            res.append((pos[off * dim:(off + c) * dim].reshape(c, dim).copy(), rad[off:off + c].copy()))
+            res.append((pos[off * dim:(off + c) * dim].reshape(c, dim), rad[off:off + c]))  # views
             off += c
         out["spheres"] = res
     return out

@@ -62,7 +62,8 @@ class BatchStats(ctypes.Structure):
     """rsa_gpu_batch_stats"""
     _fields_ = [("seconds", ctypes.c_double)] + [(n, ctypes.c_int64) for n in (
         "ticks", "draw_launch_groups", "refine_launch_groups", "max_active", "draws", "refines")] + [
-        (n, ctypes.c_double) for n in ("t_admit", "t_plan", "t_wait", "t_refine", "t_finish")]
+        (n, ctypes.c_double) for n in ("t_admit", "t_plan", "t_wait", "t_refine", "t_finish", "t_prep", "t_words",
+                                        "t_refine_wait")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -32,8 +32,7 @@ class BatchDriver {
     PinnedPool::get().release(arena_);
     PinnedPool::get().release(hctr_);
     dtab_.release();
-    rtab_.release();
-    rhead_.release();
+    rblob_.release();
     if (st_) (void)hipStreamDestroy(st_);
   }
 
@@ -48,8 +47,6 @@ class BatchDriver {
         PinnedPool::get().alloc(sizeof(int) * C_NUM * kMaxBatch, hipHostMallocCoherent | hipHostMallocMapped));
     HIP_OK(hipHostGetDevicePointer((void**)&hctr_dev_, hctr_, 0));
     dtab_.reserve(sizeof(DrawDesc<D>) * kMaxBatch + 16);
-    rtab_.reserve(sizeof(RefDesc<D>) * kMaxBatch + 16);
-    rhead_.reserve(sizeof(RefHead) + 16);
     if (extract_) {
       pos_.resize(n_);
       rad_.resize(n_);
@@ -65,8 +62,21 @@ class BatchDriver {
       timed(t_admit_, [&] { admit(); });
       std::vector<R*> first;
       timed(t_plan_, [&] {
-        for (auto& b : live_)
-          if (b->s == R::READY) plan(*b, first);
+        // realizations whose next draw is not prepared yet (just refined) join a later
+        // group; one is waited for when no other realization could go
+        R* waiting = nullptr;
+        bool moved = false;
+        for (auto& b : live_) {
+          if (b->s != R::READY) continue;
+          const auto s0 = b->s;
+          plan(*b, first, false);
+          if (b->s == s0) {
+            if (!waiting) waiting = b.get();
+          } else {
+            moved = true;
+          }
+        }
+        if (!moved && waiting && !any_refine()) plan(*waiting, first, true);
       });
       if (!first.empty()) rounds(first);
       std::vector<R*> ref;
@@ -94,6 +104,9 @@ class BatchDriver {
       bst_->t_wait = t_wait_;
       bst_->t_refine = t_refine_;
       bst_->t_finish = t_finish_;
+      bst_->t_prep = t_prep_;
+      bst_->t_words = t_words_;
+      bst_->t_refine_wait = t_refine_wait_;
     }
   }
 
@@ -128,7 +141,7 @@ class BatchDriver {
   size_t arena_off_ = 0;
   int* hctr_ = nullptr;
   int* hctr_dev_ = nullptr;
-  DevBuf<uint8_t> dtab_, rtab_, rhead_;
+  DevBuf<uint8_t> dtab_, rblob_;
   std::vector<std::unique_ptr<R>> live_;
   int next_ = 0;   // realizations set up so far (admit thread)
   int taken_ = 0;  // realizations made live (this thread)
@@ -141,7 +154,8 @@ class BatchDriver {
   bool qstop_ = false, fin_busy_ = false;
   std::string qerr_;
   int64_t ticks_ = 0, draw_groups_ = 0, refine_groups_ = 0, max_active_ = 0, draws_ = 0, refines_ = 0;
-  double t_admit_ = 0, t_plan_ = 0, t_wait_ = 0, t_refine_ = 0, t_finish_ = 0;
+  double t_admit_ = 0, t_plan_ = 0, t_wait_ = 0, t_refine_ = 0, t_finish_ = 0, t_prep_ = 0, t_words_ = 0,
+         t_refine_wait_ = 0;
   template <class F>
   static void timed(double& acc, F&& f) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -179,7 +193,9 @@ class BatchDriver {
     E& e = *b->e;
     const uint64_t cnt = static_cast<uint64_t>(v * 1.0 / (volume_unit_sphere(D) * puissance<D>(radius_[k])));
     e.algo_init({RadiusClass{radius_[k], cnt, 0}}, 0.0, nbshots_);
-    // proceed(seed): the word stream, the helper thread, the initial lattice
+    // proceed(seed): the word stream, the helper thread (which also copies each
+    // prepared draw's words to the device), the initial lattice
+    e.prep_words_ = true;
     e.reset_dev_ring();
     e.hs_.start(seed_[k]);
     e.hs_.wait_ns = 0;
@@ -237,16 +253,28 @@ class BatchDriver {
 
   // the top of proceed()'s loop for one realization: ends it, runs a host-driven
   // draw, or prepares the first round of a fast draw (into `first`)
-  void plan(R& b, std::vector<R*>& first) {
+  bool any_refine() const {
+    for (auto& b : live_)
+      if (b->s == R::REFINE) return true;
+    return false;
+  }
+
+  // (without `wait`, a realization whose draw is not prepared yet stays READY)
+  void plan(R& b, std::vector<R*>& first, bool wait) {
     E& e = *b.e;
-    b.st.loop_iterations++;
     if (e.nvox_ == 0) {
+      b.st.loop_iterations++;
       b.st.end_reason = 1;
       b.s = R::DONE;
       return;
     }
     e.hs_.set_low(b.w);
-    b.cur = e.prep_get(b.w, e.nvox_);
+    if (wait) {
+      timed(t_prep_, [&] { b.cur = e.prep_get(b.w, e.nvox_); });
+    } else if (!e.prep_try_get(b.w, e.nvox_, b.cur)) {
+      return;
+    }
+    b.st.loop_iterations++;
     b.st.loop_iterations += b.cur.zero_iters;
     const uint64_t nshots = (uint64_t)b.cur.nshots;
     const uint64_t maximal = e.current_number();
@@ -267,7 +295,7 @@ class BatchDriver {
     pr.id = ++e.draw_id_;
     pr.seq = ++e.seq_;
     pr.miss_base = (uint64_t)(int)e.nb_fast_ - b.old_nb;
-    e.dev_words(pr.w_shots, pr.w_shots + (uint64_t)pr.end_rel + (uint64_t)pr.nshots);
+    // the helper thread made the words readable on the device (prep_words_)
     b.seq = pr.seq;
     b.r = 0;
     b.lists_kept = false;
@@ -447,8 +475,8 @@ class BatchDriver {
     }
   }
 
-  // remove_covered + subdivide_uncovered (Engine::refine without the grouped
-  // order or the packed copy) for every realization of `group` at once
+  // remove_covered + subdivide_uncovered (Engine::refine, with its thresholds for
+  // the grouped order and the packed copy) for every realization of `group` at once
   void refine(const std::vector<R*>& group) {
     refine_groups_++;
     ticks_++;
@@ -458,53 +486,124 @@ class BatchDriver {
       if (b->e->nvox_ > 0) g.push_back(b);
     }
     const int n = (int)g.size();
-    std::vector<RefDesc<D>> tab(n);
-    RefHead head{};
-    head.n = n;
-    for (int q = 0; q < n; q++) {
-      E& e = *g[q]->e;
-      const int64_t nv = e.nvox_;
-      const unsigned nb = nblocks(nv);
-      e.code_.reserve(nv);
-      e.moved_.reserve(nv);
-      e.blk_.reserve(nb);
-      e.holes_.reserve(nv);
-      e.tsum_.reserve(nblocks(nb, SCAN_T));
-      RefDesc<D>& a = tab[q];
-      std::memset(&a, 0, sizeof(a));
-      a.g = e.geo_;
-      a.cs = e.cs();
-      a.vx = e.vview();
-      a.vp = e.vparams(e.gmin_);
-      a.nvox = nv;
-      a.code = e.code_.p;
-      a.moved = e.moved_.p;
-      a.blk = e.blk_.p;
-      a.holes = e.holes_.p;
-      a.tsum = e.tsum_.p;
-      a.ctr = e.dctr_.p;
-      head.nb[q + 1] = head.nb[q] + (int)nb;
-      head.nt[q + 1] = head.nt[q] + (int)nblocks(nb, SCAN_T);
-      head.nbw[q + 1] = head.nbw[q] + (int)nblocks(nv, 4 * WG);
-    }
     if (n > 0) {
-      auto* rh = reinterpret_cast<const RefHead*>(rhead_.p);
-      auto* rt = reinterpret_cast<const RefDesc<D>*>(rtab_.p);
-      stage(&head, sizeof(head), rhead_.p);
-      stage(tab.data(), sizeof(RefDesc<D>) * n, rtab_.p);
+      std::vector<RefDesc<D>> tab(n);
+      RefHead head{};
+      head.n = n;
+      // scans: 0 cell counts of grouped lists, 1 packed-copy offsets, 2 uncovered
+      // voxels per block (keep), 3 children per block
+      ScanHead sh[4] = {};
+      std::vector<ScanDesc> sd[4];
+      bool any_g = false, any_p = false;
+      for (int q = 0; q < n; q++) {
+        E& e = *g[q]->e;
+        const int64_t nv = e.nvox_;
+        const unsigned nb = nblocks(nv);
+        const int64_t nc = e.geo_.ncells;
+        const bool grouped = nv >= e.group_min_;
+        const bool packed = ipow_c(3, D) <= 32 && nv >= e.pack_min_;
+        e.code_.reserve(nv);
+        e.moved_.reserve(nv);
+        e.blk_.reserve(nb);
+        e.holes_.reserve(nv);
+        e.tsum_.reserve(std::max(nblocks(nb, SCAN_T), (grouped || packed) ? nblocks(nc, SCAN_T) : 1u));
+        RefDesc<D>& a = tab[q];
+        std::memset(&a, 0, sizeof(a));
+        a.g = e.geo_;
+        a.cs = e.cs();
+        a.vx = e.vview();
+        a.vp = e.vparams(e.gmin_);
+        a.nvox = nv;
+        a.code = e.code_.p;
+        a.moved = e.moved_.p;
+        a.blk = e.blk_.p;
+        a.holes = e.holes_.p;
+        a.ctr = e.dctr_.p;
+        if (grouped) {
+          e.perm_.reserve(nv);
+          e.vcell_.reserve(nv);
+          e.vrank_.reserve(nv);
+          e.ccount_.reserve(nc);
+          a.perm = e.perm_.p;
+          a.vcell = e.vcell_.p;
+          a.vrank = e.vrank_.p;
+          a.ccount = e.ccount_.p;
+          sd[0].push_back(ScanDesc{e.ccount_.p, nc, e.tsum_.p, e.dctr_.p + C_GROUPED});
+          any_g = true;
+        }
+        if (packed) {
+          const int64_t ns = (int64_t)(e.nb_fast_ + e.added_);
+          e.coff_.reserve((size_t)nc + 1);
+          e.csph_.reserve((size_t)ns + 1);
+          a.coff = e.coff_.p;
+          a.csph = e.csph_.p;
+          a.ncs = (int)std::min<int64_t>(ns, INT32_MAX);
+          a.ncap = ns;
+          sd[1].push_back(ScanDesc{e.coff_.p, nc, e.tsum_.p, e.coff_.p + nc});
+          any_p = true;
+        }
+        sd[2].push_back(ScanDesc{e.blk_.p, (int64_t)nb, e.tsum_.p, e.dctr_.p + C_KEEP});
+        sd[3].push_back(ScanDesc{e.blk_.p, (int64_t)nb, e.tsum_.p, e.dctr_.p + C_NCHILD});
+        const int nbw = (int)nblocks(nv, 4 * WG), ncb = (int)nblocks(nc);
+        head.nb[q + 1] = head.nb[q] + (int)nb;
+        head.nbw[q + 1] = head.nbw[q] + nbw;
+        head.nbg[q + 1] = head.nbg[q] + (grouped ? nbw : 0);
+        head.ncg[q + 1] = head.ncg[q] + (grouped ? ncb : 0);
+        head.ncp[q + 1] = head.ncp[q] + (packed ? ncb : 0);
+      }
+      for (int k = 0; k < 4; k++) {
+        sh[k].n = (int)sd[k].size();
+        for (int q = 0; q < sh[k].n; q++) sh[k].nt[q + 1] = sh[k].nt[q] + (int)nblocks(sd[k][q].n, SCAN_T);
+      }
+      // one staging copy for the group's tables
+      std::vector<char> blob;
+      auto put = [&](const void* p, size_t bytes) {
+        const size_t off = (blob.size() + 15) & ~size_t(15);
+        blob.resize(off + bytes);
+        std::memcpy(blob.data() + off, p, bytes);
+        return off;
+      };
+      const size_t o_head = put(&head, sizeof(head));
+      const size_t o_tab = put(tab.data(), sizeof(RefDesc<D>) * n);
+      size_t o_sh[4], o_sd[4];
+      for (int k = 0; k < 4; k++) {
+        o_sh[k] = put(&sh[k], sizeof(ScanHead));
+        o_sd[k] = put(sd[k].data(), sizeof(ScanDesc) * sd[k].size());
+      }
+      rblob_.reserve(blob.size() + 16);
+      stage(blob.data(), blob.size(), rblob_.p);
+      const uint8_t* base = rblob_.p;
+      auto* rh = reinterpret_cast<const RefHead*>(base + o_head);
+      auto* rt = reinterpret_cast<const RefDesc<D>*>(base + o_tab);
+      auto scan = [&](int k) {
+        if (sh[k].n == 0) return;
+        auto* h = reinterpret_cast<const ScanHead*>(base + o_sh[k]);
+        auto* t = reinterpret_cast<const ScanDesc*>(base + o_sd[k]);
+        hipLaunchKernelGGL(kbs_tile_sums, dim3(sh[k].nt[sh[k].n]), dim3(1024), 0, st_, h, t);
+        hipLaunchKernelGGL(kbs_scan_blocks, dim3(sh[k].n), dim3(1024), 0, st_, h, t);
+        hipLaunchKernelGGL(kbs_tile_scan, dim3(sh[k].nt[sh[k].n]), dim3(1024), 0, st_, h, t);
+      };
+      if (any_g) {
+        hipLaunchKernelGGL(kb_clear_ccount<D>, dim3(head.ncg[n]), dim3(WG), 0, st_, rh, rt);
+        hipLaunchKernelGGL(kb_bin<D>, dim3(head.nbg[n]), dim3(WG), 0, st_, rh, rt);
+        scan(0);
+        hipLaunchKernelGGL(kb_place<D>, dim3(head.nbg[n]), dim3(WG), 0, st_, rh, rt);
+      }
+      if (any_p) {
+        hipLaunchKernelGGL(kb_cell_cnt<D>, dim3(head.ncp[n]), dim3(WG), 0, st_, rh, rt);
+        scan(1);
+        hipLaunchKernelGGL(kb_compact<D>, dim3(head.ncp[n]), dim3(WG), 0, st_, rh, rt);
+      }
       hipLaunchKernelGGL(kb_cover<D>, dim3(head.nb[n]), dim3(WG), 0, st_, rh, rt);
-      hipLaunchKernelGGL(kb_tile_sums<D>, dim3(head.nt[n]), dim3(1024), 0, st_, rh, rt);
-      hipLaunchKernelGGL(kb_scan_blocks<D>, dim3(n), dim3(1024), 0, st_, rh, rt, (int)C_KEEP);
-      hipLaunchKernelGGL(kb_tile_scan<D>, dim3(head.nt[n]), dim3(1024), 0, st_, rh, rt);
+      if (any_g) hipLaunchKernelGGL(kb_block_count<D>, dim3(head.nbg[n]), dim3(WG), 0, st_, rh, rt);
+      scan(2);
       hipLaunchKernelGGL(kb_holes<D>, dim3(head.nbw[n]), dim3(WG), 0, st_, rh, rt);
       hipLaunchKernelGGL(kb_move<D>, dim3(head.nbw[n]), dim3(WG), 0, st_, rh, rt);
       hipLaunchKernelGGL(kb_child_count<D>, dim3(head.nbw[n]), dim3(WG), 0, st_, rh, rt);
-      hipLaunchKernelGGL(kb_tile_sums<D>, dim3(head.nt[n]), dim3(1024), 0, st_, rh, rt);
-      hipLaunchKernelGGL(kb_scan_blocks<D>, dim3(n), dim3(1024), 0, st_, rh, rt, (int)C_NCHILD);
-      hipLaunchKernelGGL(kb_tile_scan<D>, dim3(head.nt[n]), dim3(1024), 0, st_, rh, rt);
+      scan(3);
       hipLaunchKernelGGL(kb_gather_ctr<D>, dim3(n), dim3(64), 0, st_, rt, n, hctr_dev_);
       HIP_OK(hipGetLastError());
-      HIP_OK(hipStreamSynchronize(st_));
+      timed(t_refine_wait_, [&] { HIP_OK(hipStreamSynchronize(st_)); });
       // the children of every list that keeps voxels
       std::vector<int> nxt(n, -1);
       int m = 0;
@@ -513,7 +612,7 @@ class BatchDriver {
       for (int q = 0; q < n; q++) {
         E& e = *g[q]->e;
         std::memcpy(e.hctr_, hctr_ + (size_t)q * C_NUM, sizeof(int) * C_NUM);
-        e.check_scatter();  // the previous refinement's children
+        e.check_scatter();  // the previous refinement's children, this one's packed copy
         e.level_++;
         const int64_t keep = e.hctr_[C_KEEP];
         if (keep == 0) {  // subdivide_uncovered returns early on an empty list (list_of_voxels.ixx:60)
@@ -543,14 +642,24 @@ class BatchDriver {
       }
       h2.n = m;
       if (m > 0) {
-        stage(&h2, sizeof(h2), rhead_.p);
-        stage(tab2.data(), sizeof(RefDesc<D>) * m, rtab_.p);
-        hipLaunchKernelGGL(kb_scatter<D>, dim3(h2.nsc[m]), dim3(WG), 0, st_, rh, rt);
+        blob.clear();
+        const size_t p_head = put(&h2, sizeof(h2));
+        const size_t p_tab = put(tab2.data(), sizeof(RefDesc<D>) * m);
+        stage(blob.data(), blob.size(), rblob_.p);
+        hipLaunchKernelGGL(kb_scatter<D>, dim3(h2.nsc[m]), dim3(WG), 0, st_,
+                           reinterpret_cast<const RefHead*>(base + p_head),
+                           reinterpret_cast<const RefDesc<D>*>(base + p_tab));
         HIP_OK(hipGetLastError());
       }
       for (int q = 0; q < n; q++) {
-        if (nxt[q] < 0) continue;
         E& e = *g[q]->e;
+        // the next draw's Poisson count depends on the new list length: its
+        // preparation starts now, while the device scatters the children
+        if (nxt[q] < 0) {
+          if (e.nvox_ > 0) e.prep_request(g[q]->w, e.nvox_);
+          continue;
+        }
+        e.prep_request(g[q]->w, e.hctr_[C_NCHILD]);
         e.vcur_ = nxt[q];
         e.nvox_ = e.hctr_[C_NCHILD];
         e.max_coord_ = 2 * e.max_coord_;

@@ -3,13 +3,11 @@
 // the message in a thread-local string.
 #include <hip/hip_runtime.h>
 
-#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "../../include/rsa_mpi_amd.h"
@@ -287,80 +285,11 @@ int rsa_gpu_mt_words(uint64_t seed, uint64_t rank, uint64_t n, uint32_t* out) {
 int rsa_gpu_batch_run(int dim, int n, const double* side, const double* radius, const uint64_t* seed, int nbshots,
                       int workers, uint64_t* count, double* volume, double* pos, double* radius_out, uint64_t cap,
                       double* seconds) {
-  return guarded([&] {
-    if (n < 0 || !side || !radius || !seed || !count) throw rsa_amd::Error("batch: null or negative arguments");
-    if (dim < 2 || dim > 4) throw rsa_amd::Error("batch: dim must be 2, 3 or 4");
-    if ((pos == nullptr) != (radius_out == nullptr)) throw rsa_amd::Error("batch: pos and radius_out go together");
-    const int nw = std::max(1, std::min(workers, std::max(n, 1)));
-    struct Out {
-      uint64_t count = 0;
-      double volume = 0;
-      std::vector<double> pos, rad;
-    };
-    std::vector<Out> res(n);
-    std::atomic<int> next{0};
-    std::atomic<bool> failed{false};
-    std::mutex em;
-    std::string err;
-    const auto t0 = std::chrono::steady_clock::now();
-    // realization k: uniform_generate<DIM, 1>(rsa_domain([0, side]^dim, 1, r), r, nbshots, 10, seed)
-    // (operator_algorithm.ixx:275-292) and its extraction
-    auto work = [&] {
-      try {
-        for (;;) {
-          const int k = next.fetch_add(1);
-          if (k >= n || failed.load()) return;
-          std::vector<double> inf(dim, 0.0), sup(dim, side[k]);
-          std::unique_ptr<rsa_amd::IEngine> eng(rsa_amd::make_engine(dim, inf.data(), sup.data(), 1, radius[k]));
-          double v = 1;
-          for (int d = 0; d < dim; d++) v *= side[k];
-          rsa_gpu_ctx tmp;
-          tmp.eng = std::move(eng);
-          const double one = 1.0;
-          const int ph = 0;
-          if (rsa_gpu_algo_init_fractions(&tmp, 1, &radius[k], &one, &ph, v, 0.0, nbshots) != 0)
-            throw rsa_amd::Error(g_err);
-          rsa_gpu_stats st;
-          tmp.eng->proceed(seed[k], -1, &st);
-          // the device's end-of-run checks: a packing that is not maximal and free of
-          // overlaps is an error here, not a result (the reference's 4D voxel path
-          // need not be maximal, so 4D checks overlaps only)
-          if (st.end_reason != 1 || st.nvox_final != 0 || st.overlap_pairs != 0 || (dim <= 3 && st.cover_free != 0))
-            throw rsa_amd::Error("batch: realization " + std::to_string(k) + " failed the end-of-run checks (" +
-                                 std::to_string(st.overlap_pairs) + " overlapping pairs, " +
-                                 std::to_string(st.cover_free) + " free sample points)");
-          Out& o = res[k];
-          o.count = tmp.eng->count();
-          if (volume) o.volume = tmp.eng->volume_of_spheres();
-          if (pos) {
-            o.pos.resize(o.count * dim);
-            o.rad.resize(o.count);
-            tmp.eng->extract(o.pos.data(), o.rad.data(), nullptr, o.count);
-          }
-        }
-      } catch (const std::exception& e) {
-        std::lock_guard<std::mutex> lk(em);
-        if (!failed.exchange(true)) err = e.what();
-      }
-    };
-    std::vector<std::thread> th;
-    for (int w = 1; w < nw; w++) th.emplace_back(work);
-    work();
-    for (auto& t : th) t.join();
-    if (failed.load()) throw rsa_amd::Error("batch: " + err);
-    uint64_t off = 0;
-    for (int k = 0; k < n; k++) {
-      count[k] = res[k].count;
-      if (volume) volume[k] = res[k].volume;
-      if (pos) {
-        if (off + res[k].count > cap) throw rsa_amd::Error("batch: output capacity exceeded");
-        std::memcpy(pos + off * dim, res[k].pos.data(), res[k].pos.size() * sizeof(double));
-        std::memcpy(radius_out + off, res[k].rad.data(), res[k].rad.size() * sizeof(double));
-        off += res[k].count;
-      }
-    }
-    if (seconds) seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  });
+  rsa_gpu_batch_stats st;
+  const int rc = rsa_gpu_batch_queue(dim, n, side, radius, seed, nbshots, std::max(1, workers), count, volume, pos,
+                                     radius_out, cap, &st);
+  if (rc == 0 && seconds) seconds[0] = st.seconds;
+  return rc;
 }
 
 int rsa_gpu_batch_queue(int dim, int n, const double* side, const double* radius, const uint64_t* seed, int nbshots,

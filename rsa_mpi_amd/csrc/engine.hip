@@ -634,6 +634,7 @@ class Engine final : public IEngine {
   std::deque<Chunk> chunks_;  // issued, not yet waited on by st_, in word order
   std::deque<hipEvent_t> free_ev_;  // reused oldest first
   uint64_t dev_hi_ = 0;  // words below dev_hi_ are copied or in flight
+  std::mutex dw_m_;
 
   CellStore<D> cs() { return CellStore<D>{cells_cnt_.p, cells_sph_.p, cells_ph_.p, cap_}; }
 
@@ -929,6 +930,10 @@ class Engine final : public IEngine {
   // above low stay in the host ring until their copy has completed, because the
   // kernel that reads them waits for it and low only moves past them later.
   void dev_words(uint64_t from, uint64_t upto) {
+    // the batched form calls it from the helper thread and, for host-driven draws,
+    // from the driver's
+    std::unique_lock<std::mutex> dlk(dw_m_, std::defer_lock);
+    if (prep_words_) dlk.lock();
     if (dev_hi_ < from) {  // words skipped on the device (read on the host only)
       dev_hi_ = from;
     }
@@ -1310,6 +1315,13 @@ class Engine final : public IEngine {
   void prep_start();
   void prep_halt();
   Prep prep_get(uint64_t w, int64_t nv);
+  // asks the helper for the draws from (w, nv) without waiting for them
+  void prep_request(uint64_t w, int64_t nv);
+  // the draw from (w, nv) if the helper has it ready (else false, and it is requested)
+  bool prep_try_get(uint64_t w, int64_t nv, Prep& out);
+  // batched form: the helper also makes each prepared draw's words readable on
+  // the device (dev_words), so that the driver only launches
+  bool prep_words_ = false;
   std::thread pth_;
   std::mutex pm_;
   std::condition_variable pcv_;

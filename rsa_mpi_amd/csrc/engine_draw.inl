@@ -180,6 +180,8 @@ void Engine<D>::prep_start() {
       std::string err;
       try {
         pr = prepare(w, nv, b);
+        if (prep_words_ && pr.nshots > 0 && pr.nshots <= kFastShots)
+          dev_words(pr.w_shots, pr.w_shots + (uint64_t)pr.end_rel + (uint64_t)pr.nshots);
       } catch (const std::exception& e) {
         err = e.what();
       }
@@ -212,8 +214,8 @@ void Engine<D>::prep_halt() {
 // The prepared draw starting at stream position w for voxel count nv: the
 // queue's front, or the one the worker is preparing, or a new request.
 template <int D>
-typename Engine<D>::Prep Engine<D>::prep_get(uint64_t w, int64_t nv) {
-  std::unique_lock<std::mutex> lk(pm_);
+void Engine<D>::prep_request(uint64_t w, int64_t nv) {
+  std::lock_guard<std::mutex> lk(pm_);
   const bool front_ok = !pready_.empty() && pready_.front().w_start == w && pready_.front().nv == nv;
   const bool coming = pready_.empty() && preq_set_ && pnext_w_ == w && preq_nv_ == nv;
   if (!front_ok && !coming) {
@@ -225,6 +227,25 @@ typename Engine<D>::Prep Engine<D>::prep_get(uint64_t w, int64_t nv) {
     preq_set_ = true;
     pcv_.notify_all();
   }
+}
+
+template <int D>
+bool Engine<D>::prep_try_get(uint64_t w, int64_t nv, Prep& out) {
+  prep_request(w, nv);
+  std::lock_guard<std::mutex> lk(pm_);
+  if (!perr_.empty()) throw Error(perr_);
+  if (pready_.empty()) return false;
+  out = pready_.front();
+  pready_.pop_front();
+  pcv_.notify_all();
+  if (out.w_start != w || out.nv != nv) throw Error("internal: prepared draw out of sequence");
+  return true;
+}
+
+template <int D>
+typename Engine<D>::Prep Engine<D>::prep_get(uint64_t w, int64_t nv) {
+  prep_request(w, nv);
+  std::unique_lock<std::mutex> lk(pm_);
   pcv_.wait(lk, [&] { return !pready_.empty() || !perr_.empty(); });
   if (!perr_.empty()) throw Error(perr_);
   Prep pr = pready_.front();

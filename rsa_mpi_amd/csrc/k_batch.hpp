@@ -70,7 +70,9 @@ struct DrawDesc {
   RejList rl;
 };
 
-// Everything the kernels of one refinement need for one realization.
+// Everything the kernels of one refinement need for one realization.  Long
+// lists (grouped) are walked in cell-grouped order (perm) and read the placed
+// spheres from a packed copy in cell order (packed), as Engine::refine does.
 template <int D>
 struct RefDesc {
   Geo<D> g;
@@ -84,17 +86,42 @@ struct RefDesc {
   uint32_t* moved;
   int* blk;
   int* holes;
-  int* tsum;
   int* ctr;
+  // cell-grouped order (null when not grouped)
+  uint32_t* perm;
+  int* vcell;
+  int* vrank;
+  int* ccount;
+  // packed copy (null when not packed)
+  int* coff;
+  Slot<D>* csph;
+  int ncs;
+  int64_t ncap;  // spheres the packed copy holds
 };
 
-// Block-count prefixes of a refinement launch group (entries 0 .. n).
+// Block-count prefixes of a refinement launch group (entries 0 .. n; a step a
+// realization skips has no blocks).
 struct RefHead {
   int n;
   int nb[kMaxBatch + 1];    // k_cover: one block of WG voxels
-  int nt[kMaxBatch + 1];    // scans over the nb block counts: tiles of SCAN_T
   int nbw[kMaxBatch + 1];   // streaming passes: four waves of WG voxels per workgroup
+  int nbg[kMaxBatch + 1];   // the same for grouped lists only (k_bin, k_place, k_block_count)
+  int ncg[kMaxBatch + 1];   // cells of grouped lists (clearing the cell counts)
+  int ncp[kMaxBatch + 1];   // cells of packed lists (k_cell_cnt, k_compact)
   int nsc[kMaxBatch + 1];   // k_scatter: one block of WG kept voxels
+};
+
+// One exclusive scan per realization: a[0, n) in place, the total to *total
+// (-1 above INT_MAX), tile sums in tsum.
+struct ScanDesc {
+  int* a;
+  int64_t n;
+  int* tsum;
+  int* total;
+};
+struct ScanHead {
+  int n;
+  int nt[kMaxBatch + 1];  // tiles of SCAN_T elements
 };
 
 // Staging copy from pinned (coherent, mapped) host memory into a device table:
@@ -186,55 +213,90 @@ __global__ void __launch_bounds__(WG) kb_remove_skips(const DrawDesc<D>* __restr
 
 // ---------------------------------------------------------------- refinement steps
 
-// k_cover over every realization's list (no processing permutation, no packed
-// copy); the realization's first workgroup also clears the moved-voxel count
+// the realization and local block of workgroup blockIdx.x in prefix `pre`
+#define KB_FIND(pre)                                   \
+  const int r = batch_find(pre, h->n, blockIdx.x);     \
+  const RefDesc<D>& a = tab[r];                        \
+  const int bid = (int)blockIdx.x - pre[r];            \
+  (void)bid
+
+template <int D>
+__global__ void __launch_bounds__(WG) kb_clear_ccount(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
+  KB_FIND(h->ncg);
+  const int64_t c = (int64_t)bid * WG + threadIdx.x;
+  if (c < a.g.ncells) a.ccount[c] = 0;
+}
+
+template <int D>
+__global__ void __launch_bounds__(WG) kb_bin(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
+  KB_FIND(h->nbg);
+  if (a.vx.wide)
+    k_bin_body<D, true>(a.g, a.vx, a.nvox, a.vp, a.ccount, a.vcell, a.vrank, nullptr, bid);
+  else
+    k_bin_body<D, false>(a.g, a.vx, a.nvox, a.vp, a.ccount, a.vcell, a.vrank, nullptr, bid);
+}
+
+template <int D>
+__global__ void __launch_bounds__(WG) kb_place(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
+  KB_FIND(h->nbg);
+  k_place_body(a.nvox, a.vcell, a.vrank, a.ccount, a.perm, bid);
+}
+
+template <int D>
+__global__ void __launch_bounds__(WG) kb_cell_cnt(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
+  KB_FIND(h->ncp);
+  k_cell_cnt_body(a.g.ncells, a.cs.cnt, a.coff, bid);
+}
+
+template <int D>
+__global__ void __launch_bounds__(WG) kb_compact(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
+  KB_FIND(h->ncp);
+  k_compact_body<D>(a.g.ncells, a.cs, a.coff, a.csph, a.ncap, a.ctr + C_COMPACT_ERR, bid);
+}
+
+// k_cover over every realization's list; the realization's first workgroup
+// also clears the moved-voxel count
 template <int D>
 __global__ void __launch_bounds__(WG) kb_cover(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
-  const int r = batch_find(h->nb, h->n, blockIdx.x);
-  const int bid = blockIdx.x - h->nb[r];
-  const RefDesc<D>& a = tab[r];
+  KB_FIND(h->nb);
   if (bid == 0 && threadIdx.x == 0) a.ctr[C_MOVED] = 0;
   if (a.vx.wide)
-    k_cover_body<D, true>(a.g, a.cs, a.vx, a.nvox, a.vp, nullptr, nullptr, nullptr, 0, a.code, a.blk, bid);
+    k_cover_body<D, true>(a.g, a.cs, a.vx, a.nvox, a.vp, a.perm, a.coff, a.csph, a.ncs, a.code, a.blk, bid);
   else
-    k_cover_body<D, false>(a.g, a.cs, a.vx, a.nvox, a.vp, nullptr, nullptr, nullptr, 0, a.code, a.blk, bid);
+    k_cover_body<D, false>(a.g, a.cs, a.vx, a.nvox, a.vp, a.perm, a.coff, a.csph, a.ncs, a.code, a.blk, bid);
 }
 
-// the three scan launches over each realization's nb block counts (blk), the
-// total to ctr[which]
 template <int D>
-__global__ void __launch_bounds__(1024) kb_tile_sums(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
+__global__ void __launch_bounds__(WG) kb_block_count(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
+  KB_FIND(h->nbg);
+  k_block_count_body(a.nvox, a.code, a.blk, bid);
+}
+
+// the three launches of one exclusive scan per realization
+__global__ void __launch_bounds__(1024) kbs_tile_sums(const ScanHead* __restrict__ h, const ScanDesc* __restrict__ t) {
   const int r = batch_find(h->nt, h->n, blockIdx.x);
-  const RefDesc<D>& a = tab[r];
-  k_tile_sums_body(a.blk, h->nb[r + 1] - h->nb[r], a.tsum, blockIdx.x - h->nt[r]);
+  k_tile_sums_body(t[r].a, t[r].n, t[r].tsum, (int)blockIdx.x - h->nt[r]);
 }
 
-template <int D>
-__global__ void __launch_bounds__(1024) kb_scan_blocks(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab,
-                                                       int which) {
-  const RefDesc<D>& a = tab[blockIdx.x];
-  k_scan_blocks_body(a.tsum, h->nt[blockIdx.x + 1] - h->nt[blockIdx.x], a.ctr + which, 0);
+__global__ void __launch_bounds__(1024) kbs_scan_blocks(const ScanHead* __restrict__ h, const ScanDesc* __restrict__ t) {
+  const int r = blockIdx.x;
+  k_scan_blocks_body(t[r].tsum, h->nt[r + 1] - h->nt[r], t[r].total, 0);
 }
 
-template <int D>
-__global__ void __launch_bounds__(1024) kb_tile_scan(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
+__global__ void __launch_bounds__(1024) kbs_tile_scan(const ScanHead* __restrict__ h, const ScanDesc* __restrict__ t) {
   const int r = batch_find(h->nt, h->n, blockIdx.x);
-  const RefDesc<D>& a = tab[r];
-  k_tile_scan_body(a.blk, h->nb[r + 1] - h->nb[r], a.tsum, blockIdx.x - h->nt[r]);
+  k_tile_scan_body(t[r].a, t[r].n, t[r].tsum, (int)blockIdx.x - h->nt[r]);
 }
 
 template <int D>
 __global__ void __launch_bounds__(WG) kb_holes(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
-  const int r = batch_find(h->nbw, h->n, blockIdx.x);
-  const RefDesc<D>& a = tab[r];
-  k_holes_body(a.nvox, a.code, a.blk, a.ctr, a.holes, blockIdx.x - h->nbw[r]);
+  KB_FIND(h->nbw);
+  k_holes_body(a.nvox, a.code, a.blk, a.ctr, a.holes, bid);
 }
 
 template <int D>
 __global__ void __launch_bounds__(WG) kb_move(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
-  const int r = batch_find(h->nbw, h->n, blockIdx.x);
-  const RefDesc<D>& a = tab[r];
-  const int bid = blockIdx.x - h->nbw[r];
+  KB_FIND(h->nbw);
   if (a.vx.wide)
     k_move_body<D, true>(a.nvox, a.code, a.moved, a.blk, a.ctr, a.holes, a.vx, bid);
   else
@@ -243,16 +305,13 @@ __global__ void __launch_bounds__(WG) kb_move(const RefHead* __restrict__ h, con
 
 template <int D>
 __global__ void __launch_bounds__(WG) kb_child_count(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
-  const int r = batch_find(h->nbw, h->n, blockIdx.x);
-  const RefDesc<D>& a = tab[r];
-  k_child_count_body(a.nvox, a.ctr, a.code, a.moved, a.blk, blockIdx.x - h->nbw[r]);
+  KB_FIND(h->nbw);
+  k_child_count_body(a.nvox, a.ctr, a.code, a.moved, a.blk, bid);
 }
 
 template <int D>
 __global__ void __launch_bounds__(WG) kb_scatter(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
-  const int r = batch_find(h->nsc, h->n, blockIdx.x);
-  const RefDesc<D>& a = tab[r];
-  const int bid = blockIdx.x - h->nsc[r];
+  KB_FIND(h->nsc);
   int* err = a.ctr + C_SCATTER_ERR;
   if (a.vx.wide) {
     if (a.dst.wide)
@@ -266,6 +325,8 @@ __global__ void __launch_bounds__(WG) kb_scatter(const RefHead* __restrict__ h, 
       k_scatter_body<D, false, false>(a.keep, a.code, a.moved, a.blk, a.vx, a.dst, a.nchild, err, bid);
   }
 }
+
+#undef KB_FIND
 
 // every realization's counters to pinned host memory (C_NUM ints each): one
 // host wait serves the whole group

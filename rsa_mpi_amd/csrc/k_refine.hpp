@@ -81,16 +81,21 @@ __device__ __forceinline__ double far_term(double delta, double len) {
 // an exclusive scan (with off[ncells] = total) next, then k_compact copies
 // cell c's spheres to off[c] .. off[c + 1] - 1.  The cells along dimension 0
 // are consecutive, so three neighbours in a row are one contiguous range.
-__global__ void __launch_bounds__(WG) k_cell_cnt(int64_t ncells, const uint8_t* __restrict__ cnt,
-                                                 int* __restrict__ off) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_cell_cnt_body(int64_t ncells, const uint8_t* __restrict__ cnt,
+                                                 int* __restrict__ off, const int bid) {
+  const int64_t c = (int64_t)bid * blockDim.x + threadIdx.x;
   if (c < ncells) off[c] = cnt[c];
 }
 
+__global__ void __launch_bounds__(WG) k_cell_cnt(int64_t ncells, const uint8_t* __restrict__ cnt,
+                                                 int* __restrict__ off) {
+  k_cell_cnt_body(ncells, cnt, off, (int)blockIdx.x);
+}
+
 template <int D>
-__global__ void __launch_bounds__(WG) k_compact(int64_t ncells, CellStore<D> cs, const int* __restrict__ off,
-                                                Slot<D>* __restrict__ out, int64_t cap, int* __restrict__ err) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_compact_body(int64_t ncells, CellStore<D> cs, const int* __restrict__ off,
+                                                Slot<D>* __restrict__ out, int64_t cap, int* __restrict__ err, const int bid) {
+  const int64_t c = (int64_t)bid * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const int n = cs.cnt[c], o = off[c];
   if ((int64_t)o + n > cap) {
@@ -98,6 +103,12 @@ __global__ void __launch_bounds__(WG) k_compact(int64_t ncells, CellStore<D> cs,
     return;
   }
   for (int k = 0; k < n; k++) out[o + k] = cs.sph[(size_t)c * cs.cap + k];
+}
+
+template <int D>
+__global__ void __launch_bounds__(WG) k_compact(int64_t ncells, CellStore<D> cs, const int* __restrict__ off,
+                                                Slot<D>* __restrict__ out, int64_t cap, int* __restrict__ err) {
+  k_compact_body<D>(ncells, cs, off, out, cap, err, (int)blockIdx.x);
 }
 
 // remove_covered's coverage flag (list_of_voxels.ixx:30-40, 240-259) and
@@ -329,14 +340,14 @@ __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView
 // ones (keys ncells .. 2 ncells - 1), so that a wave's voxels tend to share their
 // outcome as well as their cells.
 template <int D, bool WIDE>
-__global__ void __launch_bounds__(WG) k_bin(Geo<D> g, VoxView<D> vx, int64_t nvox, VoxParams<D> vp,
+__device__ __forceinline__ void k_bin_body(Geo<D> g, VoxView<D> vx, int64_t nvox, VoxParams<D> vp,
                                             int* __restrict__ ccount, int* __restrict__ vcell,
-                                            int* __restrict__ vrank, const uint8_t* __restrict__ pred) {
+                                            int* __restrict__ vrank, const uint8_t* __restrict__ pred, const int bid) {
   // each wave takes 4 x 64 consecutive voxels, in four passes of 64 whose loads
   // are all issued first (launch nblocks(n, 4 * WG))
   constexpr int NP = 4;
   const int lane = threadIdx.x & 63;
-  const int64_t w0 = ((int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6)) * (64 * NP);
+  const int64_t w0 = ((int64_t)bid * (WG / 64) + (threadIdx.x >> 6)) * (64 * NP);
   int64_t cv[NP][D];
 #pragma unroll
   for (int q = 0; q < NP; q++) {
@@ -379,13 +390,20 @@ __global__ void __launch_bounds__(WG) k_bin(Geo<D> g, VoxView<D> vx, int64_t nvo
   }
 }
 
-__global__ void __launch_bounds__(WG) k_place(int64_t nvox, const int* __restrict__ vcell,
+template <int D, bool WIDE>
+__global__ void __launch_bounds__(WG) k_bin(Geo<D> g, VoxView<D> vx, int64_t nvox, VoxParams<D> vp,
+                                            int* __restrict__ ccount, int* __restrict__ vcell,
+                                            int* __restrict__ vrank, const uint8_t* __restrict__ pred) {
+  k_bin_body<D, WIDE>(g, vx, nvox, vp, ccount, vcell, vrank, pred, (int)blockIdx.x);
+}
+
+__device__ __forceinline__ void k_place_body(int64_t nvox, const int* __restrict__ vcell,
                                               const int* __restrict__ vrank, const int* __restrict__ cstart,
-                                              uint32_t* __restrict__ perm) {
+                                              uint32_t* __restrict__ perm, const int bid) {
   // four voxels per thread, 64 apart within the wave's 256 (launch nblocks(n, 4 * WG))
   constexpr int NP = 4;
   const int lane = threadIdx.x & 63;
-  const int64_t w0 = ((int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6)) * (64 * NP);
+  const int64_t w0 = ((int64_t)bid * (WG / 64) + (threadIdx.x >> 6)) * (64 * NP);
   int c[NP], r[NP], o[NP];
 #pragma unroll
   for (int q = 0; q < NP; q++) {
@@ -400,6 +418,12 @@ __global__ void __launch_bounds__(WG) k_place(int64_t nvox, const int* __restric
     const int64_t i = w0 + q * 64 + lane;
     if (i < nvox) perm[o[q] + r[q]] = (uint32_t)i;
   }
+}
+
+__global__ void __launch_bounds__(WG) k_place(int64_t nvox, const int* __restrict__ vcell,
+                                              const int* __restrict__ vrank, const int* __restrict__ cstart,
+                                              uint32_t* __restrict__ perm) {
+  k_place_body(nvox, vcell, vrank, cstart, perm, (int)blockIdx.x);
 }
 
 // The streaming passes over the list (k_block_count, k_holes, k_move,
@@ -427,9 +451,9 @@ __device__ __forceinline__ int wave_sum(int x) {
 }
 
 // uncovered voxels per block of WG (the blocks of the list order)
-__global__ void __launch_bounds__(WG) k_block_count(int64_t nvox, const uint32_t* __restrict__ code,
-                                                    int* __restrict__ blk) {
-  const int64_t b = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+__device__ __forceinline__ void k_block_count_body(int64_t nvox, const uint32_t* __restrict__ code,
+                                                    int* __restrict__ blk, const int bid) {
+  const int64_t b = (int64_t)bid * (WG / 64) + (threadIdx.x >> 6);
   const int64_t i0 = b * WG + (threadIdx.x & 63) * VPL;
   uint32_t c[VPL];
   load4(code, i0, nvox, c);
@@ -438,6 +462,11 @@ __global__ void __launch_bounds__(WG) k_block_count(int64_t nvox, const uint32_t
   for (int k = 0; k < VPL; k++) u += c[k] != 0;
   u = wave_sum(u);
   if ((threadIdx.x & 63) == 0 && b * WG < nvox) blk[b] = u;
+}
+
+__global__ void __launch_bounds__(WG) k_block_count(int64_t nvox, const uint32_t* __restrict__ code,
+                                                    int* __restrict__ blk) {
+  k_block_count_body(nvox, code, blk, (int)blockIdx.x);
 }
 
 // Device-wide exclusive scan in three launches: tile sums, a scan of the

@@ -5,7 +5,8 @@ What is tuned never changes a result, only how the device gets it:
   - the refinement thresholds of a context (rsa_gpu_set_tuning): the voxel-list
     length from which the coverage pass walks voxels in cell-grouped order
     (group_min) and reads the placed spheres from a packed copy (pack_min);
-  - the number of realizations batch_run keeps on the GPU at once (workers).
+  - the number of realizations batch_run keeps in flight in its device work
+    queue (workers).
 autotune() and autotune_batch() time the candidates on the workload they are
 given, check that every candidate places the same spheres, and store the fastest
 in a JSON cache; Domain(..., tuning="auto") and batch_run(workers=None) read it.
@@ -27,7 +28,7 @@ VERSION = 1
 # candidate thresholds: 2^18 .. 2^21, and "never" for either choice
 NEVER = (1 << 62)
 THRESHOLD_GRID = tuple((1 << k, 1 << k) for k in (18, 19, 20, 21)) + ((1 << 20, NEVER), (NEVER, NEVER))
-WORKER_GRID = (1, 2, 3, 4, 6)  # more concurrent engines are not run (see batch_run's tests)
+WORKER_GRID = (16, 32, 64, 128)  # realizations in flight in the device work queue
 
 
 def cache_path():

@@ -1,8 +1,8 @@
 """The batched form: 256 independent 3D realizations (box edge L ~ U[0.2, 1],
 radius r ~ U[0.004, 0.008], seeds 0..255; tests/golden/batch_3d_256.json) run
-several at a time on the GPU.  Each realization is uniform_generate<3, 1> on
-[0, L]^3 (operator_algorithm.ixx:275-292); its placed set must be the oracle's,
-whatever the number of concurrent workers."""
+as a device work queue, many in flight on one stream.  Each realization is
+uniform_generate<3, 1> on [0, L]^3 (operator_algorithm.ixx:275-292); its placed
+set must be the oracle's, whatever the number in flight."""
 import json
 import os
 
@@ -40,7 +40,7 @@ def test_batch_rejects_bad_arguments():
 
 @pytest.mark.gpu
 def test_batch256_counts_match_oracle():
-    """All 256 realizations at full size, 4 at a time: counts equal the oracle's."""
+    """All 256 realizations at full size, 4 in flight: counts equal the oracle's."""
     import rsa_mpi_amd as R
     fx = load()
     real = fx["realizations"]
@@ -51,11 +51,9 @@ def test_batch256_counts_match_oracle():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("workers", [1, 3, pytest.param(8, marks=pytest.mark.skip(
-    reason="8 concurrent contexts once gave one realization's packing a different set (right count); its cause is not "
-           "found, so this case is off until it is (the subset is checked with one worker instead)"))])
+@pytest.mark.parametrize("workers", [1, 3, 8])
 def test_batch256_sets_match_oracle(workers):
-    """Every 8th realization extracted: the placed sets are the oracle's bit for bit."""
+    """Every 8th realization extracted, `workers` in flight: the placed sets are the oracle's bit for bit."""
     import rsa_mpi_amd as R
     fx = load()
     idx = list(range(workers % 8, 256, 8))
@@ -70,7 +68,7 @@ def test_batch256_sets_match_oracle(workers):
 
 @pytest.mark.gpu
 def test_batch256_subset0_one_worker_matches_oracle():
-    """The realizations 0, 8, .., 248 (the 8-context case's subset) run one at a time: sets equal the oracle's."""
+    """The realizations 0, 8, .., 248 (the 8-in-flight case's subset) run one at a time: sets equal the oracle's."""
     import rsa_mpi_amd as R
     fx = load()
     idx = list(range(0, 256, 8))
