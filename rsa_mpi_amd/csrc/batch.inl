@@ -535,8 +535,10 @@ class BatchDriver {
           const int64_t ns = (int64_t)(e.nb_fast_ + e.added_);
           e.coff_.reserve((size_t)nc + 1);
           e.csph_.reserve((size_t)ns + 1);
+          e.cprad_.reserve((size_t)ns + 1);
           a.coff = e.coff_.p;
           a.csph = e.csph_.p;
+          a.crad = e.cprad_.p;
           a.ncs = (int)std::min<int64_t>(ns, INT32_MAX);
           a.ncap = ns;
           sd[1].push_back(ScanDesc{e.coff_.p, nc, e.tsum_.p, e.coff_.p + nc});

@@ -552,7 +552,8 @@ class Engine final : public IEngine {
   DevBuf<uint32_t> code_, moved_, perm_;
   DevBuf<int> blk_, holes_, ccount_, vcell_, vrank_, tsum_;
   DevBuf<int> coff_;         // packed copy of the cell store: offsets per cell (+ total)
-  DevBuf<Slot<D>> csph_;     // and the spheres in cell order
+  DevBuf<Slot<D>> csph_;     // and the spheres in cell order (k_cover's listing threshold in place of the radius)
+  DevBuf<double> cprad_;     // and their radii
 
   // word stream: generated on the host into a pinned ring (the MT19937
   // recurrence is sequential), copied asynchronously into the device ring
@@ -1157,28 +1158,31 @@ class Engine final : public IEngine {
     // for long lists only: below pack_min_ voxels the copy costs more than it saves
     const int* coff = nullptr;
     const Slot<D>* csph = nullptr;
+    const double* crad = nullptr;
     int ncs = 0;
     if (ipow_c(3, D) <= 32 && nvox_ >= pack_min_) {
       const int64_t nc = geo_.ncells;
       coff_.reserve((size_t)nc + 1);
       csph_.reserve((size_t)(nb_fast_ + added_) + 1);
+      cprad_.reserve((size_t)(nb_fast_ + added_) + 1);
       timed("k_compact", (double)nc * 9.0 + (double)(nb_fast_ + added_) * 2.0 * sizeof(Slot<D>), [&] {
         hipLaunchKernelGGL(k_cell_cnt, dim3(nblocks(nc)), dim3(WG), 0, st_, nc, cells_cnt_.p, coff_.p);
         scan_excl(coff_.p, nc, coff_.p + nc);
-        hipLaunchKernelGGL(k_compact<D>, dim3(nblocks(nc)), dim3(WG), 0, st_, nc, cs(), coff_.p, csph_.p,
+        hipLaunchKernelGGL(k_compact<D>, dim3(nblocks(nc)), dim3(WG), 0, st_, nc, cs(), coff_.p, csph_.p, cprad_.p, vp,
                            (int64_t)(nb_fast_ + added_), dctr_.p + C_COMPACT_ERR);
       }, 1);
       coff = coff_.p;
       csph = csph_.p;
+      crad = cprad_.p;
       ncs = (int)std::min<uint64_t>(nb_fast_ + added_, (uint64_t)INT32_MAX);
     }
     timed("k_cover", cover_bytes, [&] {
       if (wide_[vcur_])
         hipLaunchKernelGGL((k_cover<D, true>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, perm, coff,
-                           csph, ncs, code_.p, blk_.p);
+                           csph, crad, ncs, code_.p, blk_.p);
       else
         hipLaunchKernelGGL((k_cover<D, false>), dim3(nb), dim3(WG), 0, st_, geo_, cs(), vx, nvox_, vp, perm, coff,
-                           csph, ncs, code_.p, blk_.p);
+                           csph, crad, ncs, code_.p, blk_.p);
     }, 1);
     // the streaming passes: one wave per block of WG voxels, four blocks per workgroup
     const unsigned nbw = nblocks(nvox_, 4 * WG);

@@ -95,6 +95,7 @@ struct RefDesc {
   // packed copy (null when not packed)
   int* coff;
   Slot<D>* csph;
+  double* crad;
   int ncs;
   int64_t ncap;  // spheres the packed copy holds
 };
@@ -251,7 +252,7 @@ __global__ void __launch_bounds__(WG) kb_cell_cnt(const RefHead* __restrict__ h,
 template <int D>
 __global__ void __launch_bounds__(WG) kb_compact(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
   KB_FIND(h->ncp);
-  k_compact_body<D>(a.g.ncells, a.cs, a.coff, a.csph, a.ncap, a.ctr + C_COMPACT_ERR, bid);
+  k_compact_body<D>(a.g.ncells, a.cs, a.coff, a.csph, a.crad, a.vp, a.ncap, a.ctr + C_COMPACT_ERR, bid);
 }
 
 // k_cover over every realization's list; the realization's first workgroup
@@ -261,9 +262,9 @@ __global__ void __launch_bounds__(WG) kb_cover(const RefHead* __restrict__ h, co
   KB_FIND(h->nb);
   if (bid == 0 && threadIdx.x == 0) a.ctr[C_MOVED] = 0;
   if (a.vx.wide)
-    k_cover_body<D, true>(a.g, a.cs, a.vx, a.nvox, a.vp, a.perm, a.coff, a.csph, a.ncs, a.code, a.blk, bid);
+    k_cover_body<D, true>(a.g, a.cs, a.vx, a.nvox, a.vp, a.perm, a.coff, a.csph, a.crad, a.ncs, a.code, a.blk, bid);
   else
-    k_cover_body<D, false>(a.g, a.cs, a.vx, a.nvox, a.vp, a.perm, a.coff, a.csph, a.ncs, a.code, a.blk, bid);
+    k_cover_body<D, false>(a.g, a.cs, a.vx, a.nvox, a.vp, a.perm, a.coff, a.csph, a.crad, a.ncs, a.code, a.blk, bid);
 }
 
 template <int D>

@@ -94,7 +94,8 @@ __global__ void __launch_bounds__(WG) k_cell_cnt(int64_t ncells, const uint8_t* 
 
 template <int D>
 __device__ __forceinline__ void k_compact_body(int64_t ncells, CellStore<D> cs, const int* __restrict__ off,
-                                                Slot<D>* __restrict__ out, int64_t cap, int* __restrict__ err, const int bid) {
+                                                Slot<D>* __restrict__ out, double* __restrict__ rad, VoxParams<D> vp,
+                                                int64_t cap, int* __restrict__ err, const int bid) {
   const int64_t c = (int64_t)bid * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const int n = cs.cnt[c], o = off[c];
@@ -102,13 +103,23 @@ __device__ __forceinline__ void k_compact_body(int64_t ncells, CellStore<D> cs, 
     atomicOr(err, 1);
     return;
   }
-  for (int k = 0; k < n; k++) out[o + k] = cs.sph[(size_t)c * cs.cap + k];
+  for (int k = 0; k < n; k++) {
+    // the centre with k_cover's listing threshold U = R^2 + slack(R^2, r + r_min) in
+    // place of the radius (the radius goes to rad)
+    Slot<D> q = cs.sph[(size_t)c * cs.cap + k];
+    const double rr = q.r + vp.rmin;
+    const double R2 = rr * rr;
+    rad[o + k] = q.r;
+    q.r = R2 + vp.slack(R2, rr);
+    out[o + k] = q;
+  }
 }
 
 template <int D>
 __global__ void __launch_bounds__(WG) k_compact(int64_t ncells, CellStore<D> cs, const int* __restrict__ off,
-                                                Slot<D>* __restrict__ out, int64_t cap, int* __restrict__ err) {
-  k_compact_body<D>(ncells, cs, off, out, cap, err, (int)blockIdx.x);
+                                                Slot<D>* __restrict__ out, double* __restrict__ rad, VoxParams<D> vp,
+                                                int64_t cap, int* __restrict__ err) {
+  k_compact_body<D>(ncells, cs, off, out, rad, vp, cap, err, (int)blockIdx.x);
 }
 
 // remove_covered's coverage flag (list_of_voxels.ixx:30-40, 240-259) and
@@ -121,7 +132,8 @@ template <int D, bool WIDE>
 __device__ __forceinline__ void k_cover_body(Geo<D> g, CellStore<D> cs, VoxView<D> vx, int64_t nvox,
                                               VoxParams<D> vp, const uint32_t* __restrict__ perm,
                                               const int* __restrict__ coff, const Slot<D>* __restrict__ csph,
-                                              int ncs, uint32_t* __restrict__ out, int* __restrict__ blk, const int bid) {
+                                              const double* __restrict__ crad, int ncs, uint32_t* __restrict__ out,
+                                              int* __restrict__ blk, const int bid) {
   using TC = TabCorner<D>;
   constexpr int NCH = ipow_c(2, D);
   constexpr int NST = ipow_c(3, D);
@@ -245,8 +257,9 @@ __device__ __forceinline__ void k_cover_body(Geo<D> g, CellStore<D> cs, VoxView<
         int nl = 0;
         auto flush = [&]() {
           for (int t = 0; t < nl && !covered; t++) {
-            const Slot<D> q = csph[s_list[t][threadIdx.x]];
-            test(q.c, q.r, do_cov, do_child);
+            const uint32_t k = s_list[t][threadIdx.x];
+            const Slot<D> q = csph[k];
+            test(q.c, crad[k], do_cov, do_child);
           }
           nl = 0;
         };
@@ -262,18 +275,19 @@ __device__ __forceinline__ void k_cover_body(Geo<D> g, CellStore<D> cs, VoxView<
             an = coff[src];
             bn = coff[src + 3];
           }
+          // The copy holds U = R^2 + slack(R^2, r + r_min) in place of the radius
+          // (k_compact).  Both gates imply s2 <= U: the subdivide pre-filter s2 < R^2
+          // (slack >= 0), and the coverage pre-check s2 <= (R^2 - h2) + slack with
+          // h2 >= 0 (D <= 3), rounding being monotone.  A sphere with s2 <= U is listed
+          // and gets the exact gates in test().
           auto consider = [&](const Slot<D>& q, uint32_t idx) {
-            const double rr = q.r + vp.rmin;
-            const double R2 = rr * rr;
             double s2 = 0.;
 #pragma unroll
             for (int d = 0; d < D; d++) {
               const double t = q.c[d] - cb[d];
               s2 += t * t;
             }
-            const bool use =
-                (do_child && s2 < R2) || (do_cov && (vp.h2 < 0. || s2 <= R2 - vp.h2 + vp.slack(R2, rr)));
-            if (use) {
+            if (s2 <= q.r) {
               if (nl == KLIST) flush();
               s_list[nl++][threadIdx.x] = idx;
             }
@@ -327,8 +341,9 @@ template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) k_cover(Geo<D> g, CellStore<D> cs, VoxView<D> vx, int64_t nvox,
                                               VoxParams<D> vp, const uint32_t* __restrict__ perm,
                                               const int* __restrict__ coff, const Slot<D>* __restrict__ csph,
-                                              int ncs, uint32_t* __restrict__ out, int* __restrict__ blk) {
-  k_cover_body<D, WIDE>(g, cs, vx, nvox, vp, perm, coff, csph, ncs, out, blk, (int)blockIdx.x);
+                                              const double* __restrict__ crad, int ncs, uint32_t* __restrict__ out,
+                                              int* __restrict__ blk) {
+  k_cover_body<D, WIDE>(g, cs, vx, nvox, vp, perm, coff, csph, crad, ncs, out, blk, (int)blockIdx.x);
 }
 
 // k_cover's processing order: the voxels grouped by the cell of their centre
