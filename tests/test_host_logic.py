@@ -250,3 +250,46 @@ def test_coverage_precheck_never_skips_a_covering_sphere():
     assert covered > 10000
     assert wrong == 0
     assert wrong_fixed > 0
+
+
+def test_listing_threshold_contains_both_gates():
+    """k_cover lists a packed-copy sphere when s2 <= U, U = R^2 + slack(R^2, r + rmin) as
+    k_compact computes it; the exact gates then run on the listed spheres.  Whenever
+    either gate holds (the subdivide pre-filter s2 < R^2, the coverage pre-check
+    s2 <= (R^2 - h^2) + slack), s2 <= U holds too, so no sphere a gate would pass is
+    left out.  Spheres are placed at the gates' boundaries (both sides), far from the
+    origin and with tiny radii as in the pre-check test."""
+    rng = np.random.default_rng(1)
+    passes = 0
+    for _ in range(60000):
+        X0 = 10.0 ** rng.uniform(0, 8)
+        origin = [X0 * rng.uniform(0.5, 1.0) * rng.choice([-1, 1]) for _ in range(3)]
+        W = X0 * 10.0 ** rng.uniform(-3, 0)
+        R = W * 10.0 ** rng.uniform(-9, -2)
+        k = int(rng.integers(0, 60))
+        ln = [W / 7 * 2.0 ** -k] * 3
+        c = [int(rng.integers(0, 7 * 2 ** k)) for _ in range(3)]
+        cb = [origin[d] + (float(c[d]) + 0.5) * ln[d] for d in range(3)]
+        X = max(max(abs(origin[d]), abs(origin[d] + W)) + W for d in range(3))
+        qr = rmin = 0.5 * R
+        rr = qr + rmin
+        R2 = rr * rr
+        h2 = 0.0
+        for d in range(3):
+            h2 += (0.5 * ln[d]) * (0.5 * ln[d])
+        sl = 1e-9 * R2 + 64.0 * np.sqrt(3.0) * EPS53 * X * rr + 256.0 * 3 * (EPS53 * X) * (EPS53 * X)
+        U = R2 + sl
+        # a centre at distance close to one of the two gate radii
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        target = np.sqrt(max(R2 - h2 + sl, 0.0)) if rng.uniform() < 0.5 else rr
+        dist = target * rng.uniform(0.9999999, 1.0000001)
+        q = [cb[d] + u[d] * dist for d in range(3)]
+        s2 = 0.0
+        for d in range(3):
+            t = q[d] - cb[d]
+            s2 += t * t
+        gate = (s2 < R2) or (s2 <= R2 - h2 + sl)
+        passes += gate
+        assert not gate or s2 <= U, (s2, U, R2, h2, sl)
+    assert passes > 20000
