@@ -60,34 +60,56 @@ class BatchDriver {
     fin_th_ = std::thread([this] { finish_loop(); });
     while (taken_ < n_ || !live_.empty()) {
       timed(t_admit_, [&] { admit(); });
-      std::vector<R*> first;
-      timed(t_plan_, [&] {
-        // realizations whose next draw is not prepared yet (just refined) join a later
-        // group; one is waited for when no other realization could go
-        R* waiting = nullptr;
-        bool moved = false;
-        for (auto& b : live_) {
-          if (b->s != R::READY) continue;
-          const auto s0 = b->s;
-          plan(*b, first, false);
-          if (b->s == s0) {
-            if (!waiting) waiting = b.get();
-          } else {
-            moved = true;
+      // The group is driven as two halves (by slot parity): while the device runs
+      // one half's round, the host reads the other half's results and prepares and
+      // launches its next round.
+      bool launched = false;
+      for (int h = 0; h < 2; h++) {
+        if (!inflight_[h].empty()) {
+          std::vector<R*> again = collect(inflight_[h]);
+          inflight_[h].clear();
+          if (!again.empty()) {
+            launch(again, false);
+            inflight_[h] = again;
+            launched = true;
+            continue;
           }
         }
-        if (!moved && waiting && !any_refine()) plan(*waiting, first, true);
-      });
-      if (!first.empty()) rounds(first);
+        std::vector<R*> first;
+        timed(t_plan_, [&] {
+          for (auto& b : live_)
+            if (b->s == R::READY && b->half == h) plan(*b, first, false);
+        });
+        if (!first.empty()) {
+          launch(first, true);
+          inflight_[h] = first;
+          launched = true;
+        }
+      }
       std::vector<R*> ref;
       int ready = 0;
       for (auto& b : live_) {
         if (b->s == R::REFINE) ref.push_back(b.get());
         if (b->s == R::READY) ready++;
       }
+      const bool busy = !inflight_[0].empty() || !inflight_[1].empty();
       // refinements wait for company: run them once half the group waits, or when
       // nothing else can move (a realization's own order of steps is unchanged)
-      if (!ref.empty() && (2 * ref.size() >= live_.size() || ready == 0)) timed(t_refine_, [&] { refine(ref); });
+      if (!ref.empty() && (2 * ref.size() >= live_.size() || (ready == 0 && !busy)))
+        timed(t_refine_, [&] { refine(ref); });
+      else if (!launched && !busy && ready > 0) {
+        // nothing could go: wait for the first realization whose draw is being prepared
+        for (auto& b : live_) {
+          if (b->s != R::READY) continue;
+          std::vector<R*> first;
+          timed(t_plan_, [&] { plan(*b, first, true); });
+          if (!first.empty()) {
+            launch(first, true);
+            inflight_[b->half] = first;
+          }
+          break;
+        }
+      }
       timed(t_finish_, [&] { finish(); });
     }
     timed(t_finish_, [&] { drain(); });
@@ -121,6 +143,7 @@ class BatchDriver {
     uint64_t w = 0, old_nb = 0;
     int seq = 0, r = 0;
     bool lists_kept = false, refilter = false, first = false, cont = true, lists = false;
+    int half = 0;  // which half of the group it is driven with
     int64_t local_new = 0;
     std::chrono::steady_clock::time_point t0;
   };
@@ -142,7 +165,9 @@ class BatchDriver {
   int* hctr_ = nullptr;
   int* hctr_dev_ = nullptr;
   DevBuf<uint8_t> dtab_, rblob_;
+  std::vector<DrawDesc<D>> tab_;  // host copy of a launch's descriptors
   std::vector<std::unique_ptr<R>> live_;
+  std::vector<R*> inflight_[2];  // each half's draw round in flight
   int next_ = 0;   // realizations set up so far (admit thread)
   int taken_ = 0;  // realizations made live (this thread)
   // admit and finish threads, their queues and errors (guarded by qm_)
@@ -242,6 +267,7 @@ class BatchDriver {
     if (!qerr_.empty()) throw Error(qerr_);
     bool took = false;
     while ((int)live_.size() < active_ && !pre_.empty()) {
+      pre_.front()->half = taken_ & 1;
       live_.push_back(std::move(pre_.front()));
       pre_.pop_front();
       taken_++;
@@ -359,86 +385,86 @@ class BatchDriver {
     a.lists = b.lists ? 1 : 0;
   }
 
-  // the rounds of every draw in `group` (its first rounds prepared): one launch per
-  // kernel for all of them, then the publications, then the further rounds of those
-  // that need them (finish_draw's loop, for the group)
-  void rounds(std::vector<R*> group) {
+  // one round of every draw in `group` (a first round, or the next one of draws that
+  // need it): one launch per kernel for all of them
+  void launch(const std::vector<R*>& group, bool first_round) {
     using LG = LaneGroup<D>;
-    bool first_launch = true;
-    std::vector<DrawDesc<D>> tab;
-    while (!group.empty()) {
-      ticks_++;
-      draw_groups_++;
-      int mx = 1;
-      tab.resize(group.size());
-      bool any_refilter = false, any_lists = false;
-      for (size_t q = 0; q < group.size(); q++) {
-        fill(*group[q], tab[q]);
-        mx = std::max(mx, group[q]->cur.nshots);
-        any_refilter = any_refilter || group[q]->refilter;
-        any_lists = any_lists || group[q]->lists;
-      }
-      stage(tab.data(), sizeof(DrawDesc<D>) * tab.size(), dtab_.p);
-      const auto* dt = reinterpret_cast<const DrawDesc<D>*>(dtab_.p);
-      const int nr = (int)group.size();
-      const int bpl = (int)nblocks((int64_t)mx * LG::LPC), bps = (int)nblocks(mx);
-      if (first_launch)
-        hipLaunchKernelGGL(kb_decode_filter<D>, dim3(nr * bpl), dim3(WG), 0, st_, dt, bpl);
-      if (any_lists) hipLaunchKernelGGL(kb_lists<D>, dim3(nr * bps), dim3(WG), 0, st_, dt, bps);
-      if (any_refilter) {
-        hipLaunchKernelGGL(kb_refilter<D>, dim3(nr * bpl), dim3(WG), 0, st_, dt, bpl);
-        hipLaunchKernelGGL(kb_remove_skips<D>, dim3(nr * bps), dim3(WG), 0, st_, dt, bps);
-      }
-      hipLaunchKernelGGL(kb_partner_lists<D>, dim3(nr * bpl), dim3(WG), 0, st_, dt, bpl);
-      hipLaunchKernelGGL(kb_sweep_labels<D>, dim3(nr), dim3(1024), (size_t)mx, st_, dt);
-      hipLaunchKernelGGL(kb_update_commit<D>, dim3(nr * bps), dim3(WG), 0, st_, dt, bps);
-      HIP_OK(hipGetLastError());
-      first_launch = false;
-      std::vector<R*> again;
-      for (R* bp : group) {
-        R& b = *bp;
-        E& e = *b.e;
-        b.first = false;
-        b.refilter = false;
-        b.lists = false;
-        timed(t_wait_, [&] { e.wait_pub(b.seq); });
-        const DrawPub h = e.hpub_[b.seq & (kPubSlots - 1)];
-        if (h.label_stuck) throw Error("internal: conflict labelling made no progress");
-        if (h.cell_overflow) throw Error("internal: cell capacity exceeded");
-        b.st.ghost_cell_drops += h.dropped;
-        b.st.image_cell_mismatch += h.img_mismatch;
-        auto build_lists = [&]() {  // k_lists (forced) in the group's next launch, before its round
-          b.lists = true;
-          b.lists_kept = true;
-        };
-        if (h.need_lists) {  // a cell got more than CSLOT candidates: build the lists, redo the first round
-          build_lists();
-          b.seq = ++e.seq_;
-          again.push_back(&b);
-          continue;
-        }
-        if (h.pool_overflow) {  // long partner lists: grow the pool and redo the round
-          HIP_OK(hipStreamSynchronize(st_));
-          e.pool_.reserve((size_t)h.need_pool * 2);
-          HIP_OK(hipMemsetAsync(&e.dctl_.p->pool_cursor, 0, 3 * sizeof(int), st_));
-          if (b.r == 0 && !b.lists_kept) build_lists();
-          b.seq = ++e.seq_;
-          again.push_back(&b);
-          continue;
-        }
-        b.local_new += h.acc;
-        if (h.con == 0) {
-          after_draw(b, (uint64_t)b.local_new, b.r + 1, true, h.gate_open != 0);
-          continue;
-        }
-        if (++b.r >= 100000) throw Error("too many conflict rounds in one draw");
+    ticks_++;
+    draw_groups_++;
+    int mx = 1;
+    tab_.resize(group.size());
+    bool any_refilter = false, any_lists = false;
+    for (size_t q = 0; q < group.size(); q++) {
+      fill(*group[q], tab_[q]);
+      mx = std::max(mx, group[q]->cur.nshots);
+      any_refilter = any_refilter || group[q]->refilter;
+      any_lists = any_lists || group[q]->lists;
+    }
+    stage(tab_.data(), sizeof(DrawDesc<D>) * tab_.size(), dtab_.p);
+    const auto* dt = reinterpret_cast<const DrawDesc<D>*>(dtab_.p);
+    const int nr = (int)group.size();
+    const int bpl = (int)nblocks((int64_t)mx * LG::LPC), bps = (int)nblocks(mx);
+    if (first_round) hipLaunchKernelGGL(kb_decode_filter<D>, dim3(nr * bpl), dim3(WG), 0, st_, dt, bpl);
+    if (any_lists) hipLaunchKernelGGL(kb_lists<D>, dim3(nr * bps), dim3(WG), 0, st_, dt, bps);
+    if (any_refilter) {
+      hipLaunchKernelGGL(kb_refilter<D>, dim3(nr * bpl), dim3(WG), 0, st_, dt, bpl);
+      hipLaunchKernelGGL(kb_remove_skips<D>, dim3(nr * bps), dim3(WG), 0, st_, dt, bps);
+    }
+    hipLaunchKernelGGL(kb_partner_lists<D>, dim3(nr * bpl), dim3(WG), 0, st_, dt, bpl);
+    hipLaunchKernelGGL(kb_sweep_labels<D>, dim3(nr), dim3(1024), (size_t)mx, st_, dt);
+    hipLaunchKernelGGL(kb_update_commit<D>, dim3(nr * bps), dim3(WG), 0, st_, dt, bps);
+    HIP_OK(hipGetLastError());
+    for (R* b : group) {
+      b->first = false;
+      b->refilter = false;
+      b->lists = false;
+    }
+  }
+
+  // the results of a launched round (finish_draw's loop body, for the group): the
+  // draws that need another round are returned, flagged for it
+  std::vector<R*> collect(const std::vector<R*>& group) {
+    std::vector<R*> again;
+    for (R* bp : group) {
+      R& b = *bp;
+      E& e = *b.e;
+      timed(t_wait_, [&] { e.wait_pub(b.seq); });
+      const DrawPub h = e.hpub_[b.seq & (kPubSlots - 1)];
+      if (h.label_stuck) throw Error("internal: conflict labelling made no progress");
+      if (h.cell_overflow) throw Error("internal: cell capacity exceeded");
+      b.st.ghost_cell_drops += h.dropped;
+      b.st.image_cell_mismatch += h.img_mismatch;
+      auto build_lists = [&]() {  // k_lists (forced) in the next launch, before its round
+        b.lists = true;
         b.lists_kept = true;
-        b.refilter = true;
+      };
+      if (h.need_lists) {  // a cell got more than CSLOT candidates: build the lists, redo the first round
+        build_lists();
         b.seq = ++e.seq_;
         again.push_back(&b);
+        continue;
       }
-      group.swap(again);
+      if (h.pool_overflow) {  // long partner lists: grow the pool and redo the round
+        HIP_OK(hipStreamSynchronize(st_));
+        e.pool_.reserve((size_t)h.need_pool * 2);
+        HIP_OK(hipMemsetAsync(&e.dctl_.p->pool_cursor, 0, 3 * sizeof(int), st_));
+        if (b.r == 0 && !b.lists_kept) build_lists();
+        b.seq = ++e.seq_;
+        again.push_back(&b);
+        continue;
+      }
+      b.local_new += h.acc;
+      if (h.con == 0) {
+        after_draw(b, (uint64_t)b.local_new, b.r + 1, true, h.gate_open != 0);
+        continue;
+      }
+      if (++b.r >= 100000) throw Error("too many conflict rounds in one draw");
+      b.lists_kept = true;
+      b.refilter = true;
+      b.seq = ++e.seq_;
+      again.push_back(&b);
     }
+    return again;
   }
 
   // proceed()'s bookkeeping after a draw
