@@ -258,7 +258,8 @@ def batched(R, active):
     L = [c["L"] for c in real]
     r = [c["r"] for c in real]
     s = [c["seed"] for c in real]
-    R.batch_queue(3, L[:8], r[:8], s[:8], active=8)  # warm-up: pools, threads
+    # warm-up: the pools' device and pinned blocks for `active` realizations in flight, threads
+    R.batch_queue(3, L[:active], r[:active], s[:active], active=active)
 
     def one_at_a_time():
         out = []

@@ -95,7 +95,7 @@ class BatchDriver {
       const bool busy = !inflight_[0].empty() || !inflight_[1].empty();
       // refinements wait for company: run them once half the group waits, or when
       // nothing else can move (a realization's own order of steps is unchanged)
-      if (!ref.empty() && (2 * ref.size() >= live_.size() || (ready == 0 && !busy)))
+      if (!ref.empty() && (ref.size() >= refine_frac_ * live_.size() || (ready == 0 && !busy)))
         timed(t_refine_, [&] { refine(ref); });
       else if (!launched && !busy && ready > 0) {
         // nothing could go: wait for the first realization whose draw is being prepared
@@ -168,6 +168,12 @@ class BatchDriver {
   std::vector<DrawDesc<D>> tab_;  // host copy of a launch's descriptors
   std::vector<std::unique_ptr<R>> live_;
   std::vector<R*> inflight_[2];  // each half's draw round in flight
+  // refinements wait until this fraction of the group waits for one
+  // (RSA_GPU_BATCH_REFINE_FRAC, a tuning setting; default one half)
+  double refine_frac_ = [] {
+    const char* e = std::getenv("RSA_GPU_BATCH_REFINE_FRAC");
+    return e ? std::atof(e) : 0.5;
+  }();
   int next_ = 0;   // realizations set up so far (admit thread)
   int taken_ = 0;  // realizations made live (this thread)
   // admit and finish threads, their queues and errors (guarded by qm_)

@@ -140,3 +140,21 @@ def test_batch_queue_256_sets_match_oracle():
            if len(rad) != fx["oracle"][k]["n"] or set_hash(pos, rad) != fx["oracle"][k]["sha256"]]
     assert not bad, bad[:10]
     print("batch_queue 256:", out["seconds"], out["stats"], flush=True)
+
+
+@pytest.mark.gpu
+def test_batch_queue_host_driven_draws_match_single_contexts():
+    """Draws of more than 48 000 shots (nbshots 60 000 on a box of 10 M initial voxels)
+    take the host-driven form inside the queue (alone on its stream, its words copied
+    under the same lock as the helper threads'): the sets equal the single-context
+    runs of the same realizations, in extraction order."""
+    import rsa_mpi_amd as R
+    sides, radii, seeds = [1.0, 0.3], [0.004, 0.006], [5, 6]
+    out = R.batch_queue(3, sides, radii, seeds, nbshots=60000, active=2, extract=True)
+    for k, (pos, rad) in enumerate(out["spheres"]):
+        dom = R.Domain([0.0] * 3, [sides[k]] * 3, radii[k])
+        R.uniform_generate(dom, radii[k], 60000, 10, seeds[k])
+        p1, r1, _ = dom.extract_arrays()
+        assert len(rad) == len(r1) > 0, k
+        assert np.array_equal(pos.view(np.uint64), p1.view(np.uint64)), k
+        assert np.array_equal(rad.view(np.uint64), r1.view(np.uint64)), k
