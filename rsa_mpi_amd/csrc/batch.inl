@@ -526,7 +526,7 @@ class BatchDriver {
       // voxels per block (keep), 3 children per block
       ScanHead sh[4] = {};
       std::vector<ScanDesc> sd[4];
-      bool any_g = false, any_p = false;
+      bool any_g = false, any_p = false, any_wide = false;
       for (int q = 0; q < n; q++) {
         E& e = *g[q]->e;
         const int64_t nv = e.nvox_;
@@ -544,6 +544,7 @@ class BatchDriver {
         a.g = e.geo_;
         a.cs = e.cs();
         a.vx = e.vview();
+        any_wide = any_wide || a.vx.wide;
         a.vp = e.vparams(e.gmin_);
         a.nvox = nv;
         a.code = e.code_.p;
@@ -619,7 +620,8 @@ class BatchDriver {
       };
       if (any_g) {
         hipLaunchKernelGGL(kb_clear_ccount<D>, dim3(head.ncg[n]), dim3(WG), 0, st_, rh, rt);
-        hipLaunchKernelGGL(kb_bin<D>, dim3(head.nbg[n]), dim3(WG), 0, st_, rh, rt);
+        hipLaunchKernelGGL((kb_bin<D, false>), dim3(head.nbg[n]), dim3(WG), 0, st_, rh, rt);
+        if (any_wide) hipLaunchKernelGGL((kb_bin<D, true>), dim3(head.nbg[n]), dim3(WG), 0, st_, rh, rt);
         scan(0);
         hipLaunchKernelGGL(kb_place<D>, dim3(head.nbg[n]), dim3(WG), 0, st_, rh, rt);
       }
@@ -628,11 +630,13 @@ class BatchDriver {
         scan(1);
         hipLaunchKernelGGL(kb_compact<D>, dim3(head.ncp[n]), dim3(WG), 0, st_, rh, rt);
       }
-      hipLaunchKernelGGL(kb_cover<D>, dim3(head.nb[n]), dim3(WG), 0, st_, rh, rt);
+      hipLaunchKernelGGL((kb_cover<D, false>), dim3(head.nb[n]), dim3(WG), 0, st_, rh, rt);
+      if (any_wide) hipLaunchKernelGGL((kb_cover<D, true>), dim3(head.nb[n]), dim3(WG), 0, st_, rh, rt);
       if (any_g) hipLaunchKernelGGL(kb_block_count<D>, dim3(head.nbg[n]), dim3(WG), 0, st_, rh, rt);
       scan(2);
       hipLaunchKernelGGL(kb_holes<D>, dim3(head.nbw[n]), dim3(WG), 0, st_, rh, rt);
-      hipLaunchKernelGGL(kb_move<D>, dim3(head.nbw[n]), dim3(WG), 0, st_, rh, rt);
+      hipLaunchKernelGGL((kb_move<D, false>), dim3(head.nbw[n]), dim3(WG), 0, st_, rh, rt);
+      if (any_wide) hipLaunchKernelGGL((kb_move<D, true>), dim3(head.nbw[n]), dim3(WG), 0, st_, rh, rt);
       hipLaunchKernelGGL(kb_child_count<D>, dim3(head.nbw[n]), dim3(WG), 0, st_, rh, rt);
       scan(3);
       hipLaunchKernelGGL(kb_gather_ctr<D>, dim3(n), dim3(64), 0, st_, rt, n, hctr_dev_);

@@ -228,13 +228,14 @@ __global__ void __launch_bounds__(WG) kb_clear_ccount(const RefHead* __restrict_
   if (c < a.g.ncells) a.ccount[c] = 0;
 }
 
-template <int D>
+// The kernels that read the voxel coordinates come in one instance per list
+// width, each acting on the realizations of its width (one instance with both
+// bodies would need the registers of both, and fewer waves would fit).
+template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) kb_bin(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
   KB_FIND(h->nbg);
-  if (a.vx.wide)
-    k_bin_body<D, true>(a.g, a.vx, a.nvox, a.vp, a.ccount, a.vcell, a.vrank, nullptr, bid);
-  else
-    k_bin_body<D, false>(a.g, a.vx, a.nvox, a.vp, a.ccount, a.vcell, a.vrank, nullptr, bid);
+  if (a.vx.wide != (WIDE ? 1 : 0)) return;
+  k_bin_body<D, WIDE>(a.g, a.vx, a.nvox, a.vp, a.ccount, a.vcell, a.vrank, nullptr, bid);
 }
 
 template <int D>
@@ -257,14 +258,12 @@ __global__ void __launch_bounds__(WG) kb_compact(const RefHead* __restrict__ h, 
 
 // k_cover over every realization's list; the realization's first workgroup
 // also clears the moved-voxel count
-template <int D>
+template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) kb_cover(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
   KB_FIND(h->nb);
+  if (a.vx.wide != (WIDE ? 1 : 0)) return;
   if (bid == 0 && threadIdx.x == 0) a.ctr[C_MOVED] = 0;
-  if (a.vx.wide)
-    k_cover_body<D, true>(a.g, a.cs, a.vx, a.nvox, a.vp, a.perm, a.coff, a.csph, a.crad, a.ncs, a.code, a.blk, bid);
-  else
-    k_cover_body<D, false>(a.g, a.cs, a.vx, a.nvox, a.vp, a.perm, a.coff, a.csph, a.crad, a.ncs, a.code, a.blk, bid);
+  k_cover_body<D, WIDE>(a.g, a.cs, a.vx, a.nvox, a.vp, a.perm, a.coff, a.csph, a.crad, a.ncs, a.code, a.blk, bid);
 }
 
 template <int D>
@@ -295,13 +294,11 @@ __global__ void __launch_bounds__(WG) kb_holes(const RefHead* __restrict__ h, co
   k_holes_body(a.nvox, a.code, a.blk, a.ctr, a.holes, bid);
 }
 
-template <int D>
+template <int D, bool WIDE>
 __global__ void __launch_bounds__(WG) kb_move(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
   KB_FIND(h->nbw);
-  if (a.vx.wide)
-    k_move_body<D, true>(a.nvox, a.code, a.moved, a.blk, a.ctr, a.holes, a.vx, bid);
-  else
-    k_move_body<D, false>(a.nvox, a.code, a.moved, a.blk, a.ctr, a.holes, a.vx, bid);
+  if (a.vx.wide != (WIDE ? 1 : 0)) return;
+  k_move_body<D, WIDE>(a.nvox, a.code, a.moved, a.blk, a.ctr, a.holes, a.vx, bid);
 }
 
 template <int D>
