@@ -684,9 +684,15 @@ class BatchDriver {
         const size_t p_head = put(&h2, sizeof(h2));
         const size_t p_tab = put(tab2.data(), sizeof(RefDesc<D>) * m);
         stage(blob.data(), blob.size(), rblob_.p);
-        hipLaunchKernelGGL(kb_scatter<D>, dim3(h2.nsc[m]), dim3(WG), 0, st_,
-                           reinterpret_cast<const RefHead*>(base + p_head),
-                           reinterpret_cast<const RefDesc<D>*>(base + p_tab));
+        auto* sh2 = reinterpret_cast<const RefHead*>(base + p_head);
+        auto* st2 = reinterpret_cast<const RefDesc<D>*>(base + p_tab);
+        // one instance per (list width, children width) present
+        bool kinds[2][2] = {};
+        for (const auto& a : tab2) kinds[a.vx.wide ? 1 : 0][a.dst.wide ? 1 : 0] = true;
+        if (kinds[0][0]) hipLaunchKernelGGL((kb_scatter<D, false, false>), dim3(h2.nsc[m]), dim3(WG), 0, st_, sh2, st2);
+        if (kinds[0][1]) hipLaunchKernelGGL((kb_scatter<D, false, true>), dim3(h2.nsc[m]), dim3(WG), 0, st_, sh2, st2);
+        if (kinds[1][0]) hipLaunchKernelGGL((kb_scatter<D, true, false>), dim3(h2.nsc[m]), dim3(WG), 0, st_, sh2, st2);
+        if (kinds[1][1]) hipLaunchKernelGGL((kb_scatter<D, true, true>), dim3(h2.nsc[m]), dim3(WG), 0, st_, sh2, st2);
         HIP_OK(hipGetLastError());
       }
       for (int q = 0; q < n; q++) {

@@ -307,21 +307,11 @@ __global__ void __launch_bounds__(WG) kb_child_count(const RefHead* __restrict__
   k_child_count_body(a.nvox, a.ctr, a.code, a.moved, a.blk, bid);
 }
 
-template <int D>
+template <int D, bool SW, bool DW>
 __global__ void __launch_bounds__(WG) kb_scatter(const RefHead* __restrict__ h, const RefDesc<D>* __restrict__ tab) {
   KB_FIND(h->nsc);
-  int* err = a.ctr + C_SCATTER_ERR;
-  if (a.vx.wide) {
-    if (a.dst.wide)
-      k_scatter_body<D, true, true>(a.keep, a.code, a.moved, a.blk, a.vx, a.dst, a.nchild, err, bid);
-    else
-      k_scatter_body<D, true, false>(a.keep, a.code, a.moved, a.blk, a.vx, a.dst, a.nchild, err, bid);
-  } else {
-    if (a.dst.wide)
-      k_scatter_body<D, false, true>(a.keep, a.code, a.moved, a.blk, a.vx, a.dst, a.nchild, err, bid);
-    else
-      k_scatter_body<D, false, false>(a.keep, a.code, a.moved, a.blk, a.vx, a.dst, a.nchild, err, bid);
-  }
+  if (a.vx.wide != (SW ? 1 : 0) || a.dst.wide != (DW ? 1 : 0)) return;
+  k_scatter_body<D, SW, DW>(a.keep, a.code, a.moved, a.blk, a.vx, a.dst, a.nchild, a.ctr + C_SCATTER_ERR, bid);
 }
 
 #undef KB_FIND
