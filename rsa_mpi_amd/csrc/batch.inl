@@ -417,8 +417,9 @@ class BatchDriver {
       hipLaunchKernelGGL(kb_remove_skips<D>, dim3(nr * bps), dim3(WG), 0, st_, dt, bps);
     }
     hipLaunchKernelGGL(kb_partner_lists<D>, dim3(nr * bpl), dim3(WG), 0, st_, dt, bpl);
-    hipLaunchKernelGGL(kb_sweep_labels<D>, dim3(nr), dim3(1024), (size_t)mx, st_, dt);
-    hipLaunchKernelGGL(kb_update_commit<D>, dim3(nr * bps), dim3(WG), 0, st_, dt, bps);
+    hipLaunchKernelGGL(kb_sweep_labels<D>, dim3(nr), dim3(kSweepThreads), (size_t)mx, st_, dt);
+    // one more workgroup per realization: the one that publishes its round
+    hipLaunchKernelGGL(kb_update_commit<D>, dim3(nr * (bps + 1)), dim3(WG), 0, st_, dt, bps + 1);
     HIP_OK(hipGetLastError());
     for (R* b : group) {
       b->first = false;

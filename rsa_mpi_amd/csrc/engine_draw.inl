@@ -300,11 +300,12 @@ void Engine<D>::enqueue_round(const Prep& pr, int seq, int gate_id) {
                        (int)pool_.cap, dctl_.p, gate_id, urec_.p, cellcc_.p, cslot_.p);
   });
   timed("k_sweep_labels", (double)nshots * 4.0, [&] {
-    hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(1024), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, slab_.p,
+    hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(kSweepThreads), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, slab_.p,
                        pool_.p, dctl_.p, seq, gate_id, pr.id, aimed_, pr.miss_base, urec_.p);
   });
   timed("k_update_commit", (double)nshots * (8.0 * (D + 1) + 12.0), [&] {
-    hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
+    // one more workgroup than candidates' blocks: the one that publishes the round
+    hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs + 1), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
                        ccell_.p, owner_.p, ctag_.p, lab_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, seq,
                        hpub_dev_, gate_id >= 0 ? 1 : 0, cellcc_.p, cslot_.p, geo_.ncells);
   });

@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(WG) kb_partner_lists(const DrawDesc<D>* __rest
 
 // one workgroup per realization; dynamic LDS of the largest shot count
 template <int D>
-__global__ void __launch_bounds__(1024) kb_sweep_labels(const DrawDesc<D>* __restrict__ tab) {
+__global__ void __launch_bounds__(kSweepThreads) kb_sweep_labels(const DrawDesc<D>* __restrict__ tab) {
   const DrawDesc<D>& a = tab[blockIdx.x];
   k_sweep_labels_body(a.nshots, a.ctag, a.lab, a.slab, a.pool, a.ctl, a.seq, a.gate_id, a.draw_id, a.aimed,
                       a.miss_base, a.urec, 0);

@@ -19,6 +19,21 @@ __device__ __forceinline__ T load_now(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// A word every lane reads at the same address (a control word), issued where
+// written and waited for where first used.  Read as a uniform value, the
+// compiler copies it to a scalar register right after the load and so waits for
+// it before any later load is issued; indexed by a zero it cannot see through,
+// it stays a per-lane value.
+__device__ __forceinline__ int opaque_zero() {
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+template <class T>
+__device__ __forceinline__ T load_late(const T* p) {
+  return load_now(p + opaque_zero());
+}
+
 // The packed stencil entries of a lane's CPL cells (lane `sub` takes entries
 // sub * CPL ..), read at kernel start: they depend on the lane only.
 template <int D, int CPL, int NST>
@@ -598,6 +613,8 @@ constexpr int CSLOT = 8;  // candidates of a cell kept in its fixed slots; more 
 constexpr int KMAX = 16;  // partners kept in a candidate's slab; longer lists go to a pool
 constexpr int PKEEP = 4;  // partners a lane keeps in registers between counting and writing
 constexpr int KREJ = 96;  // rejected voxel words passed with the launch; more: offsets read from host memory
+constexpr int kSweepThreads = 1024;  // the sweep's one workgroup (a constant: reading the launch size is a load)
+constexpr int kSlotsAtOnce = 4;  // slots of a stencil cell requested together by the filter (one 128-byte line in 3D)
 
 // The Lemire rejection chain of a draw as launch arguments: shot i's voxel word
 // sits at i * (2D + 1) + #{k : idx[k] <= i} (idx = shot index of each rejected
@@ -676,15 +693,17 @@ __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict_
                                                       DrawCtl* __restrict__ ctl, int draw_id, int forced,
                                                       int list_thr, const int bid) {
   using LG = LaneGroup<D>;
-  // The gate does not change during this kernel.  It is read first but tested
-  // only before the first store, so that its load overlaps the reads below
-  // (all of which stay in bounds whatever the gate says).
-  const int gate = forced ? draw_id : ctl->gate;
   const int gtid = bid * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
   const int sub = gtid % LG::LPC;
   const bool valid = i < nshots;
+  // The gate does not change during this kernel.  It is read with the ring words
+  // but tested only before the first store, so that its load overlaps the reads
+  // below (all of which stay in bounds whatever the gate says).  Lanes past the
+  // shots store nothing, whatever the gate.
+  int gate = draw_id;
+  int pk[LG::CPL] = {};  // the lane's stencil entries, read with the ring words
   double p[D];
   int64_t cell = -1;
   bool ok = false;
@@ -707,6 +726,13 @@ __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict_
       v = vpos[i];
     }
     if (hrad) r = hrad[i];
+    // The gate and the stencil entries are requested here, after the shot's
+    // offset: on the rejection-list path no load is then in flight, so the
+    // offset's computation waits for none.  The gate is a vector load whatever
+    // `forced` says (a read under that test is a scalar load waited for at once).
+    const int gate_ld = load_late(&ctl->gate);
+    gate = forced ? draw_id : gate_ld;
+    lane_stencil<D, LG::CPL, LG::NST>(g, sub, pk);
     // every ring word of the shot is requested before any of them is used
     const uint32_t pw = ring_at(ring, mask, w0 + (uint64_t)end_rel + i);
     const uint32_t gw = ring_at(ring, mask, w0 + v);
@@ -739,11 +765,13 @@ __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict_
       ok = !coords_ghost<D>(g, cc);
     }
   }
-  // The arrival slot does not depend on the filter's outcome, so it is taken
-  // before the stencil walk and the atomic's round trip overlaps the walk's
-  // loads.  With a closed gate nothing is written (the test below returns).
+  // The arrival slot does not depend on the filter's outcome.  Its atomic is
+  // issued after the walk's first loads (a cell's count, then its first slots)
+  // and waited for at the end: loads and atomics complete in issue order for the
+  // wave's counter, so an atomic issued before them would hold back every wait
+  // of the walk.  With a closed gate nothing is written (the test below returns).
+  const bool take = valid && ok && sub == 0 && gate == draw_id;
   int a = -1;
-  if (valid && ok && sub == 0 && gate == draw_id) a = atomicAdd(&cellcc[cell], 1);
   bool hit = false;
   if (valid && ok) {
     int cc[D];
@@ -751,24 +779,32 @@ __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict_
 #pragma unroll
     for (int jj = 0; jj < LG::CPL; jj++) {
       const int s = sub * LG::CPL + jj;
-      if (s >= LG::NST || hit) continue;
       Nb<D> nb;
-      if (!neighbour<D>(g, cc, s, nb)) continue;
-      // slots are read only for occupied cells: most neighbour cells of a late
-      // draw are empty, and a slot is a cache line of its own
-      const int nv = cs.cnt[nb.src];
-      const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
-      for (int k = 0; k < nv && !hit; k++) {
-        const Slot<D> q = sl[k];
+      const bool use = s < LG::NST && !hit && neighbour_at<D>(g, cc, pk[jj], nb);
+      // slots are read only for occupied cells (most neighbour cells of a late
+      // draw are empty); the first kSlotsAtOnce of them are requested together
+      const int nv = use ? (int)cs.cnt[nb.src] : 0;
+      const Slot<D>* sl = cs.sph + (size_t)(use ? nb.src : 0) * cs.cap;
+      Slot<D> q4[kSlotsAtOnce];
+#pragma unroll
+      for (int k = 0; k < kSlotsAtOnce; k++)
+        if (k < nv) q4[k] = sl[k];
+      if (jj == 0 && take) a = atomicAdd(&cellcc[cell], 1);
+      auto collides = [&](const Slot<D>& q) {
         double qp[D];
         if (nb.ghost) {
-          if (!image_of<D>(g, nb, q.c, qp)) continue;
+          if (!image_of<D>(g, nb, q.c, qp)) return false;
         } else {
 #pragma unroll
           for (int d = 0; d < D; d++) qp[d] = q.c[d];
         }
-        if (collide<D>(r, p, q.r, qp)) hit = true;
-      }
+        return collide<D>(r, p, q.r, qp);
+      };
+#pragma unroll
+      for (int k = 0; k < kSlotsAtOnce; k++)
+        if (k < nv && !hit && collides(q4[k])) hit = true;
+      for (int k = kSlotsAtOnce; k < nv && !hit; k++)
+        if (collides(sl[k])) hit = true;
     }
   }
   const bool anyhit = (__ballot(hit) & LG::mask(lane)) != 0;
@@ -1086,26 +1122,13 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
       in_row = sg == t * CSLOT;
       o = ord + sg;
     }
-    for (int k = 0; k < ln; k++) {
-      int j;
-      if (in_row && k < CSLOT) {
-        j = v[0];
-#pragma unroll
-        for (int q = 1; q < CSLOT; q++) j = k == q ? v[q] : j;
-      } else {
-        j = o[k];
-      }
-      // a candidate's tag, priority, position and radius are loaded together
-      const uint8_t tj = load_now(ctag + j);
-      const uint32_t pj = load_now(cprio + j);
-      const double rj = load_now(crad + j);
-      double jp[D], qp[D];
-#pragma unroll
-      for (int d = 0; d < D; d++) jp[d] = load_now(cpos + (size_t)d * nshots + j);
-      if (tj != T_UNDECIDED) continue;
-      if (!(pj < pi)) continue;
+    // entry j of the list (tag, priority, radius, position), in detection order
+    auto consider = [&](int j, uint8_t tj, uint32_t pj, double rj, const double* jp) {
+      if (tj != T_UNDECIDED) return;
+      if (!(pj < pi)) return;
+      double qp[D];
       if (nb.ghost) {
-        if (!image_of<D>(g, nb, jp, qp)) continue;
+        if (!image_of<D>(g, nb, jp, qp)) return;
         if (out == nullptr && mismatch) {  // the image must fall in this ghost cell
           const int64_t ic = cell_index<D>(g, qp);
           int64_t want = 0;
@@ -1126,6 +1149,46 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
         }
         n++;
       }
+    };
+    int k0 = 0;
+    if (in_row) {
+      // the first kSlotsAtOnce entries are requested together (entries past the
+      // list read candidate `self`, in bounds, and are not considered)
+      int je[kSlotsAtOnce];
+      uint8_t te[kSlotsAtOnce];
+      uint32_t pe[kSlotsAtOnce];
+      double re[kSlotsAtOnce], xe[kSlotsAtOnce][D];
+#pragma unroll
+      for (int k = 0; k < kSlotsAtOnce; k++) {
+        je[k] = k < ln ? v[k] : self;
+        te[k] = load_now(ctag + je[k]);
+        pe[k] = load_now(cprio + je[k]);
+        re[k] = load_now(crad + je[k]);
+#pragma unroll
+        for (int d = 0; d < D; d++) xe[k][d] = load_now(cpos + (size_t)d * nshots + je[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < kSlotsAtOnce; k++)
+        if (k < ln) consider(je[k], te[k], pe[k], re[k], xe[k]);
+      k0 = kSlotsAtOnce;
+    }
+    for (int k = k0; k < ln; k++) {
+      int j;
+      if (in_row && k < CSLOT) {
+        j = v[0];
+#pragma unroll
+        for (int q = 1; q < CSLOT; q++) j = k == q ? v[q] : j;
+      } else {
+        j = o[k];
+      }
+      // a candidate's tag, priority, position and radius are loaded together
+      const uint8_t tj = load_now(ctag + j);
+      const uint32_t pj = load_now(cprio + j);
+      const double rj = load_now(crad + j);
+      double jp[D];
+#pragma unroll
+      for (int d = 0; d < D; d++) jp[d] = load_now(cpos + (size_t)d * nshots + j);
+      consider(j, tj, pj, rj, jp);
     }
   }
   return n;
@@ -1149,8 +1212,10 @@ __device__ __forceinline__ void k_partner_lists_body(Geo<D> g, int nshots, const
   // A draw's first round (gate_id >= 0) builds the cell lists from the arrival
   // slots; the later ones read the lists k_update_commit kept.
   const bool first = gate_id >= 0;
-  const int gate = first ? ctl->gate : gate_id;  // tested before the first store (loads overlap)
-  const int long_list = first ? ctl->long_list : 0;
+  // tested before the first store: read whatever `first` says, so that the loads overlap
+  const int gate_ld = load_late(&ctl->gate), long_ld = load_late(&ctl->long_list);
+  const int gate = first ? gate_ld : gate_id;
+  const int long_list = first ? long_ld : 0;
   const int gtid = bid * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int i = gtid / LG::LPC;
@@ -1269,31 +1334,32 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
                                                 double aimed, uint64_t miss_base, const URec* __restrict__ urec, const int bid) {
   extern __shared__ __align__(16) uint8_t s_lab[];
   __shared__ int s_prog, s_left, s_acc, s_con, s_stuck;
-  // everything below reads first and stores only once the gate is known open
-  const int gate = gate_id >= 0 ? ctl->gate : gate_id;
-  const bool ovf = ctl->pool_overflow != 0;
-  // a first round that met a cell of more than CSLOT candidates did nothing: the
-  // host builds the lists and redoes it (like a pool overflow)
-  const bool longl = gate_id >= 0 && ctl->long_list != 0;
-  const bool redo = ovf || longl;
-  const int nu = ctl->nunres;
-  const int tid = threadIdx.x;
+  // everything below reads first and stores only once the gate is known open;
+  // the control words are read by every thread, whatever the tests on them say
+  // (a read under a test or a || is waited for at once)
+  const int gate_ld = load_late(&ctl->gate), povf_ld = load_late(&ctl->pool_overflow);
+  const int long_ld = load_late(&ctl->long_list), nu_ld = load_late(&ctl->nunres);
   // the counters the round publishes: no kernel of this round changes them
   // after k_partner_lists, so they are read now, with the loads above
-  int pub_dropped = 0, pub_mism = 0, pub_povf = 0, pub_need = 0, pub_cell = 0, acc_before = 0;
+  const int dropped_ld = load_late(&ctl->dropped), mism_ld = load_late(&ctl->img_mismatch);
+  const int need_ld = load_late(&ctl->need_pool), cell_ld = load_late(&ctl->cell_overflow);
+  const int accd_ld = load_late(&ctl->acc_draw);
+  const int gate = gate_id >= 0 ? gate_ld : gate_id;
+  const bool ovf = povf_ld != 0;
+  // a first round that met a cell of more than CSLOT candidates did nothing: the
+  // host builds the lists and redoes it (like a pool overflow)
+  const bool longl = gate_id >= 0 && long_ld != 0;
+  const bool redo_l = ovf || longl;
+  const int tid = threadIdx.x;
+  const int pub_dropped = dropped_ld, pub_mism = mism_ld, pub_povf = povf_ld, pub_need = need_ld,
+            pub_cell = cell_ld, acc_before = accd_ld;
   if (tid == 0) {
-    pub_dropped = ctl->dropped;
-    pub_mism = ctl->img_mismatch;
-    pub_povf = ctl->pool_overflow;
-    pub_need = ctl->need_pool;
-    pub_cell = ctl->cell_overflow;
-    acc_before = ctl->acc_draw;
     s_acc = 0;
     s_con = 0;
     s_stuck = 0;
   }
   // labels into LDS, 8 candidates per thread and load (the buffers start 256-byte aligned)
-  for (int q8 = tid * 8; q8 < nshots; q8 += 8 * (int)blockDim.x) {
+  for (int q8 = tid * 8; q8 < nshots; q8 += 8 * kSweepThreads) {
     if (q8 + 8 <= nshots) {
       const uint2 t = *reinterpret_cast<const uint2*>(ctag + q8);
       const uint2 l = *reinterpret_cast<const uint2*>(lab + q8);
@@ -1321,14 +1387,14 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
   const int* cpl[NC];
 #pragma unroll
   for (int c = 0; c < NC; c++) {
-    const int u = tid + c * (int)blockDim.x;
+    const int u = tid + c * kSweepThreads;
     int4 h = make_int4(-1, 0, 0, 0), e = make_int4(0, 0, 0, 0);
     if (u < nshots) {
       const int4* rc = reinterpret_cast<const int4*>(urec + u);
       h = rc[0];
       e = rc[1];
     }
-    const bool use = !redo && u < nu;
+    const bool use = !redo_l && u < nu_ld;
     cq[c] = use ? h.x : -1;
     cm[c] = use ? h.y : 0;
     cpl[c] = use ? (h.z == -1 ? slab + (size_t)h.x * KMAX : pool + h.z) : nullptr;
@@ -1337,6 +1403,10 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
     ce[c][2] = e.z;
     ce[c][3] = e.w;
   }
+  // the same for the whole workgroup: a scalar value from here on (its wait comes
+  // after the loads above are issued)
+  const int nu = __builtin_amdgcn_readfirstlane(nu_ld);
+  const bool redo = __builtin_amdgcn_readfirstlane(redo_l ? 1 : 0) != 0;
   __syncthreads();
   if (gate != gate_id) return;  // the whole workgroup
   // one candidate's label from its partners (first non-Skip one decides), or
@@ -1386,7 +1456,7 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
           left++;
         }
       }
-      for (int u = tid + NC * (int)blockDim.x; u < nu; u += blockDim.x) {
+      for (int u = tid + NC * kSweepThreads; u < nu; u += kSweepThreads) {
         const int4* rc = reinterpret_cast<const int4*>(urec + u);
         const int4 h = rc[0], e = rc[1];
         const int q = h.x;
@@ -1416,7 +1486,7 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
     }
     // counts over 8 labels per thread, summed over the wave, one LDS atomic per wave
     int acc = 0, con = 0;
-    for (int q8 = tid * 8; q8 < nshots; q8 += 8 * (int)blockDim.x) {
+    for (int q8 = tid * 8; q8 < nshots; q8 += 8 * kSweepThreads) {
       if (q8 + 8 <= nshots) {
         const uint2 w = *reinterpret_cast<const uint2*>(s_lab + q8);
 #pragma unroll
@@ -1473,7 +1543,7 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
   for (int k = 0; k < kPubFields; k++) ctl->pubv[k] = v[k];  // published by k_update_commit
 }
 
-__global__ void __launch_bounds__(1024) k_sweep_labels(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
+__global__ void __launch_bounds__(kSweepThreads) k_sweep_labels(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
                                                 const int* __restrict__ slab, const int* __restrict__ pool,
                                                 DrawCtl* ctl, int seq, int gate_id, int draw_id,
                                                 double aimed, uint64_t miss_base, const URec* __restrict__ urec) {
@@ -1496,20 +1566,31 @@ __device__ __forceinline__ void k_update_commit_body(CellStore<D> cs, int nshots
                                                       int* __restrict__ cellcc, const int* __restrict__ cslot,
                                                       int ncells, const int bid) {
   const int i = bid * blockDim.x + threadIdx.x;
-  const int swept = ctl->swept;  // all reads are issued before the tests
-  if (i == 0 && swept == seq) {
-    // the round's results to the host, as the sweep left them (seq last)
-    int v[kPubFields];
+  // The round's results go to the host from a workgroup of their own, the one
+  // after the candidates' (the launch has one more): its system-scope fence then
+  // delays no candidate's update.
+  const int pub_bid = (nshots + (int)blockDim.x - 1) / (int)blockDim.x;
+  if (bid == pub_bid) {
+    if (threadIdx.x == 0 && load_now(&ctl->swept) == seq) {
+      // as the sweep left them (seq last)
+      int v[kPubFields];
 #pragma unroll
-    for (int k = 0; k < kPubFields; k++) v[k] = ctl->pubv[k];
-    DrawPub* hp = host + (seq & (kPubSlots - 1));
+      for (int k = 0; k < kPubFields; k++) v[k] = load_now(&ctl->pubv[k]);
+      DrawPub* hp = host + (seq & (kPubSlots - 1));
 #pragma unroll
-    for (int k = 0; k < kPubFields; k++) __atomic_store_n(&reinterpret_cast<int*>(hp)[k], v[k], __ATOMIC_RELAXED);
-    __threadfence_system();
-    __atomic_store_n(&hp->seq, seq, __ATOMIC_RELEASE);
+      for (int k = 0; k < kPubFields; k++) __atomic_store_n(&reinterpret_cast<int*>(hp)[k], v[k], __ATOMIC_RELAXED);
+      __threadfence_system();
+      __atomic_store_n(&hp->seq, seq, __ATOMIC_RELEASE);
+    }
+    return;
   }
-  const bool skip_round = ctl->pool_overflow || ctl->label_stuck || (first && ctl->long_list);
-  const bool commit = ctl->con == 0;
+  // Every control word is read here, with the candidate loads below, and tested
+  // only once those are issued (a read under a test or a || is waited for at once).
+  const int swept = load_late(&ctl->swept);
+  const int povf = load_late(&ctl->pool_overflow), stuck = load_late(&ctl->label_stuck);
+  const int longl = load_late(&ctl->long_list), con = load_late(&ctl->con);
+  const bool skip_round = (povf | stuck | (first ? longl : 0)) != 0;
+  const bool commit = con == 0;
   const bool in = i < nshots;
   // Every thread also reads, without a branch (index clamped to candidate 0), its
   // owner flag and cell, and its list: in a draw's first round the cell's count
@@ -1529,9 +1610,8 @@ __device__ __forceinline__ void k_update_commit_body(CellStore<D> cs, int nshots
   const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
   const int4 r0 = make_int4((int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32));
   const int4 r1 = make_int4((int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32));
+  const int cnt0 = load_now(cs.cnt + cl);  // an owner's cell is in the grid (cl == c)
   if (swept != seq || !on || skip_round) return;  // its sweep did not run (closed gate), or no owner
-  // an owner's cell is in the grid
-  const int cnt0 = load_now(cs.cnt + c);
   if (first || (sg == i * CSLOT && n0 <= CSLOT)) {
     // the list in registers: its tags are loaded together, the removal (same
     // steps as the loop below) works on the registers
