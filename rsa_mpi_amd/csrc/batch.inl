@@ -384,7 +384,6 @@ class BatchDriver {
     a.aimed = e.aimed_;
     a.miss_base = pr.miss_base;
     a.host = e.hpub_dev_;
-    a.first = b.first ? 1 : 0;
     a.ncells = e.geo_.ncells;
     a.rl = pr.rej;
     a.refilter = b.refilter ? 1 : 0;
@@ -417,9 +416,9 @@ class BatchDriver {
       hipLaunchKernelGGL(kb_remove_skips<D>, dim3(nr * bps), dim3(WG), 0, st_, dt, bps);
     }
     hipLaunchKernelGGL(kb_partner_lists<D>, dim3(nr * bpl), dim3(WG), 0, st_, dt, bpl);
-    hipLaunchKernelGGL(kb_sweep_labels<D>, dim3(nr), dim3(kSweepThreads), (size_t)mx, st_, dt);
-    // one more workgroup per realization: the one that publishes its round
-    hipLaunchKernelGGL(kb_update_commit<D>, dim3(nr * (bps + 1)), dim3(WG), 0, st_, dt, bps + 1);
+    // the owners' workgroups and one more that publishes the round, per realization
+    const int bsc = (mx + kOwnersPerBlock - 1) / kOwnersPerBlock + 1;
+    hipLaunchKernelGGL(kb_sweep_commit<D>, dim3(nr * bsc), dim3(kSweepThreads), (size_t)mx, st_, dt, bsc);
     HIP_OK(hipGetLastError());
     for (R* b : group) {
       b->first = false;

@@ -1337,7 +1337,7 @@ class Engine final : public IEngine {
   std::string perr_;
   void wait_pub(int seq);
   DevBuf<DrawCtl> dctl_;
-  DrawPub* hpub_ = nullptr;      // pinned, coherent: kPubSlots slots written by k_sweep_labels
+  DrawPub* hpub_ = nullptr;      // pinned, coherent: kPubSlots slots written by k_sweep_commit
   DrawPub* hpub_dev_ = nullptr;  // its device address
   int seq_ = 0;
   // pinned voxel-word offsets of prepared draws (kFastShots + 1 entries each),

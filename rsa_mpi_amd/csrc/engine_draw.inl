@@ -270,10 +270,10 @@ void Engine<D>::wait_pub(int seq) {
 }
 
 // generate_spheres + add_to_sample loop (operator_algorithm.ixx:534-582, 383-443)
-// for draws that cannot overshoot the class target: four kernels per round and
+// for draws that cannot overshoot the class target: three kernels per round and
 // one host round trip (the round's counts reach pinned host memory from the
-// first thread of k_update_commit; the tag update and commit run while the
-// host moves on).
+// publishing workgroup of k_sweep_commit; the tag update and commit run while
+// the host moves on).
 // enqueue_first queues a draw's first round (gated unless forced);
 // finish_draw waits for its rounds and queues the further ones.
 template <int D>
@@ -293,21 +293,18 @@ void Engine<D>::enqueue_round(const Prep& pr, int seq, int gate_id) {
   const int nshots = pr.nshots;
   using LG = LaneGroup<D>;
   const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
-  const unsigned nbs = nblocks(nshots);
   timed("k_partner_lists", (double)nshots * (8.0 * (D + 1) + 10.0), [&] {
     hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, cpos_.p, crad_.p, cprio_.p,
                        ccell_.p, ctag_.p, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, slab_.p, pool_.p,
-                       (int)pool_.cap, dctl_.p, gate_id, urec_.p, cellcc_.p, cslot_.p);
+                       (int)pool_.cap, dctl_.p, seq, gate_id, urec_.p, cellcc_.p, cslot_.p);
   });
-  timed("k_sweep_labels", (double)nshots * 4.0, [&] {
-    hipLaunchKernelGGL(k_sweep_labels, dim3(1), dim3(kSweepThreads), (size_t)nshots, st_, nshots, ctag_.p, lab_.p, slab_.p,
-                       pool_.p, dctl_.p, seq, gate_id, pr.id, aimed_, pr.miss_base, urec_.p);
-  });
-  timed("k_update_commit", (double)nshots * (8.0 * (D + 1) + 12.0), [&] {
-    // one more workgroup than candidates' blocks: the one that publishes the round
-    hipLaunchKernelGGL(k_update_commit<D>, dim3(nbs + 1), dim3(WG), 0, st_, cs(), nshots, cpos_.p, crad_.p, cphase_.p,
-                       ccell_.p, owner_.p, ctag_.p, lab_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, seq,
-                       hpub_dev_, gate_id >= 0 ? 1 : 0, cellcc_.p, cslot_.p, geo_.ncells);
+  // the owners' workgroups and one more that publishes the round
+  const unsigned nsc = (unsigned)((nshots + kOwnersPerBlock - 1) / kOwnersPerBlock + 1);
+  timed("k_sweep_commit", (double)nshots * (8.0 * (D + 1) + 16.0), [&] {
+    hipLaunchKernelGGL(k_sweep_commit<D>, dim3(nsc), dim3(kSweepThreads), (size_t)nshots, st_, cs(), nshots,
+                       cpos_.p, crad_.p, cphase_.p, ccell_.p, owner_.p, ctag_.p, lab_.p, slab_.p, pool_.p, urec_.p,
+                       seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, seq, gate_id, pr.id, aimed_, pr.miss_base,
+                       hpub_dev_, cellcc_.p, cslot_.p, geo_.ncells);
   });
   HIP_OK(hipGetLastError());
 }

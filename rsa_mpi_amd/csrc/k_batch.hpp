@@ -57,13 +57,11 @@ struct DrawDesc {
   int pool_cap;
   int gate_id;
   URec* urec;
-  // k_sweep_labels
+  // k_sweep_commit (also gate_id, ctl, lab ... above)
   int seq;
   double aimed;
   uint64_t miss_base;
-  // k_update_commit
   DrawPub* host;
-  int first;
   int ncells;
   int refilter;  // a conflict round: k_refilter and k_remove_skips run first
   int lists;     // a redone round: k_lists builds the cell lists first (and clears the long-list flag)
@@ -166,23 +164,20 @@ __global__ void __launch_bounds__(WG) kb_partner_lists(const DrawDesc<D>* __rest
   const int r = blockIdx.x / bpr, bid = blockIdx.x % bpr;
   const DrawDesc<D>& a = tab[r];
   k_partner_lists_body<D>(a.g, a.nshots, a.cpos, a.crad, a.cprio, a.ccell, a.ctag, a.lab, a.cell2t, a.seg, a.len,
-                          a.ord, a.slab, a.pool, a.pool_cap, a.ctl, a.gate_id, a.urec, a.cellcc, a.cslot, bid);
+                          a.ord, a.slab, a.pool, a.pool_cap, a.ctl, a.seq, a.gate_id, a.urec, a.cellcc, a.cslot, bid);
 }
 
-// one workgroup per realization; dynamic LDS of the largest shot count
+// the sweep and the commit: blocks per realization as launched (those past a
+// realization's owners and its publisher return); dynamic LDS of the largest
+// shot count
 template <int D>
-__global__ void __launch_bounds__(kSweepThreads) kb_sweep_labels(const DrawDesc<D>* __restrict__ tab) {
-  const DrawDesc<D>& a = tab[blockIdx.x];
-  k_sweep_labels_body(a.nshots, a.ctag, a.lab, a.slab, a.pool, a.ctl, a.seq, a.gate_id, a.draw_id, a.aimed,
-                      a.miss_base, a.urec, 0);
-}
-
-template <int D>
-__global__ void __launch_bounds__(WG) kb_update_commit(const DrawDesc<D>* __restrict__ tab, int bpr) {
+__global__ void __launch_bounds__(kSweepThreads) kb_sweep_commit(const DrawDesc<D>* __restrict__ tab, int bpr) {
   const int r = blockIdx.x / bpr, bid = blockIdx.x % bpr;
   const DrawDesc<D>& a = tab[r];
-  k_update_commit_body<D>(a.cs, a.nshots, a.cpos, a.crad, a.cphase, a.ccell, a.owner, a.ctag, a.lab, a.seg, a.len,
-                          a.ord, a.cell2t, a.ctl, a.seq, a.host, a.first, a.cellcc, a.cslot, a.ncells, bid);
+  if (bid > (a.nshots + kOwnersPerBlock - 1) / kOwnersPerBlock) return;
+  k_sweep_commit_body<D>(a.cs, a.nshots, a.cpos, a.crad, a.cphase, a.ccell, a.owner, a.ctag, a.lab, a.slab, a.pool,
+                         a.urec, a.seg, a.len, a.ord, a.cell2t, a.ctl, a.seq, a.gate_id, a.draw_id, a.aimed,
+                         a.miss_base, a.host, a.cellcc, a.cslot, a.ncells, bid);
 }
 
 // k_lists, forced, for the realizations that redo a round with host-built lists

@@ -579,12 +579,12 @@ __global__ void k_commit(CellStore<D> cs, int nshots, const double* __restrict__
 
 // ============================================================== kernels: one draw, one host round trip
 //
-// A draw is four kernels: decode + validated filter + cell binning
+// A draw is three kernels: decode + validated filter + cell binning
 // (k_decode_filter), partner lists (k_partner_lists, which builds each
 // neighbour cell's ordered list from its arrival slots in the first round),
-// sweep labels (k_sweep_labels, one workgroup), tag update + commit + the
-// round's publication to the host (k_update_commit, which keeps the cell lists
-// for later rounds).  k_lists builds the lists instead when a cell got more
+// then sweep labels + tag update + commit + the round's publication to the
+// host (k_sweep_commit, whose workgroups each run the sweep and which keeps the
+// cell lists for later rounds).  k_lists builds the lists instead when a cell got more
 // than CSLOT candidates (the host redoes the round).  Gathers over the 3^D
 // stencil are spread over a lane group per candidate (one lane per stencil
 // cell) so the loads of all neighbour cells are in flight together.  No kernel
@@ -613,7 +613,8 @@ constexpr int CSLOT = 8;  // candidates of a cell kept in its fixed slots; more 
 constexpr int KMAX = 16;  // partners kept in a candidate's slab; longer lists go to a pool
 constexpr int PKEEP = 4;  // partners a lane keeps in registers between counting and writing
 constexpr int KREJ = 96;  // rejected voxel words passed with the launch; more: offsets read from host memory
-constexpr int kSweepThreads = 1024;  // the sweep's one workgroup (a constant: reading the launch size is a load)
+constexpr int kSweepThreads = 1024;  // workgroup of the sweep (a constant: reading the launch size is a load)
+constexpr int kOwnersPerBlock = 256;  // cell owners per workgroup of k_sweep_commit
 constexpr int kSlotsAtOnce = 4;  // slots of a stencil cell requested together by the filter (one 128-byte line in 3D)
 
 // The Lemire rejection chain of a draw as launch arguments: shot i's voxel word
@@ -633,23 +634,20 @@ struct DrawCtl {
   int pool_cursor;
   int pool_overflow;
   int need_pool;
-  int acc, con;
-  int label_stuck;
   int cell_overflow;
-  // Draw gate.  The first-round kernels of draw `id` run only while gate == id
-  // (a forced launch of k_decode_filter sets it).  A draw's last sweep opens
-  // the gate for draw id + 1 when the draw is over and its miss rate asks for
-  // no refinement; otherwise it closes it (-1).  The host may thus queue the
-  // next draw before it has seen this one's results: when the gate stays
-  // closed, every kernel of that draw returns at once.
-  int gate;
-  int swept;     // seq of the last sweep that ran (k_update_commit of that round runs)
-  int nunres;    // candidates k_partner_lists left unresolved (one URec each)
-  int acc_draw;  // spheres accepted so far in the current draw
-  int long_list; // a cell got more than CSLOT candidates this draw (set by k_decode_filter)
-  int pubv[10];  // the round's publication (DrawPub fields), left by k_sweep_labels for k_update_commit
+  // Draw gate, per parity of the draw id.  The first-round kernels of draw `id`
+  // run only while gate[id & 1] == id (a forced launch of k_decode_filter sets
+  // it).  A draw's last sweep opens the gate for draw id + 1 when the draw is
+  // over and its miss rate asks for no refinement; otherwise it closes it (-1).
+  // The host may thus queue the next draw before it has seen this one's
+  // results: when the gate stays closed, every kernel of that draw returns at
+  // once.
+  int gate[2];
+  int nunres[2];  // candidates k_partner_lists left unresolved (one URec each), per parity of seq
+  int acc_draw;   // spheres accepted so far in the current draw
+  int long_list;  // a cell got more than CSLOT candidates this draw (set by k_decode_filter)
 };
-// One unresolved candidate as k_sweep_labels reads it, written by
+// One unresolved candidate as k_sweep_commit reads it, written by
 // k_partner_lists at the candidate's place in the unresolved list: index,
 // partner count, list offset (-1: the candidate's slab of KMAX entries, -2: the
 // pool overflowed) and its first PKEEP partners (the list itself holds the rest).
@@ -659,16 +657,15 @@ struct alignas(32) URec {
 };
 static_assert(sizeof(URec) == 32, "two 16-byte loads per record");
 
-// A round's results in pinned host memory (seq last): computed by
-// k_sweep_labels, written to the host by the first thread of k_update_commit,
-// so that the PCIe writes overlap the tag update instead of ending the sweep.
+// A round's results in pinned host memory (seq last), written by the
+// publishing workgroup of k_sweep_commit while the owners update their cells.
 struct DrawPub {
   int acc, con, label_stuck, dropped, img_mismatch, pool_overflow, need_pool, cell_overflow, gate_open;
   int need_lists;  // the first round found a cell of more than CSLOT candidates: build the lists, redo it
   int seq;
 };
 constexpr int kPubFields = 10;
-static_assert(sizeof(((DrawCtl*)nullptr)->pubv) == kPubFields * sizeof(int), "DrawCtl::pubv holds a DrawPub");
+static_assert(offsetof(DrawPub, seq) == kPubFields * sizeof(int), "the fields before seq");
 // Publications go to slot seq % kPubSlots: a queued next draw may publish
 // before the host has read the current one.
 constexpr int kPubSlots = 8;
@@ -730,7 +727,7 @@ __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict_
     // offset: on the rejection-list path no load is then in flight, so the
     // offset's computation waits for none.  The gate is a vector load whatever
     // `forced` says (a read under that test is a scalar load waited for at once).
-    const int gate_ld = load_late(&ctl->gate);
+    const int gate_ld = load_late(&ctl->gate[draw_id & 1]);
     gate = forced ? draw_id : gate_ld;
     lane_stencil<D, LG::CPL, LG::NST>(g, sub, pk);
     // every ring word of the shot is requested before any of them is used
@@ -811,7 +808,7 @@ __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict_
   if (gate != draw_id) return;  // whole grid
   if (gtid == 0) {
     ctl->acc_draw = 0;
-    if (forced) ctl->gate = draw_id;
+    if (forced) ctl->gate[draw_id & 1] = draw_id;
   }
   if (valid && sub == 0) {
 #pragma unroll
@@ -906,7 +903,7 @@ __device__ __forceinline__ void k_lists_body(int nshots, const int* __restrict__
                                               int* __restrict__ seg, int* __restrict__ len, int* __restrict__ ord,
                                               int* __restrict__ cell2t, DrawCtl* __restrict__ ctl, int draw_id,
                                               int ncells, const int bid) {
-  const int gate = ctl->gate;  // tested before the first store (loads overlap)
+  const int gate = load_now(&ctl->gate[draw_id & 1]);  // tested before the first store (loads overlap)
   const int i = bid * blockDim.x + threadIdx.x;
   const bool in = i < nshots;
   // Every thread reads, without a branch (index clamped to candidate 0), its
@@ -1194,10 +1191,11 @@ __device__ __forceinline__ int lane_partners(const Geo<D>& g, const int* cc, int
   return n;
 }
 
-// Sweep labels, first step (see k_sweep_labels for the semantics): candidates without
+// Sweep labels, first step (see k_sweep_commit for the semantics): candidates without
 // partners stay Undecided, those whose first partner is an image become
 // Conflict; the others get their ordered partner list (slab of KMAX entries,
-// longer lists from a pool) and are left unresolved for k_sweep_labels.
+// longer lists from a pool) and are left unresolved for k_sweep_commit.  Every
+// other candidate's label is Dropped, so that the sweep reads labels only.
 template <int D>
 __device__ __forceinline__ void k_partner_lists_body(Geo<D> g, int nshots, const double* __restrict__ cpos,
                                                  const double* __restrict__ crad, const uint32_t* __restrict__ cprio,
@@ -1205,15 +1203,16 @@ __device__ __forceinline__ void k_partner_lists_body(Geo<D> g, int nshots, const
                                                  uint8_t* __restrict__ lab, const int* __restrict__ cell2t,
                                                  const int* __restrict__ seg, const int* __restrict__ len,
                                                  const int* __restrict__ ord, int* __restrict__ slab,
-                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int gate_id,
-                                                 URec* __restrict__ urec, const int* __restrict__ cellcc,
-                                                 const int* __restrict__ cslot, const int bid) {
+                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int seq,
+                                                 int gate_id, URec* __restrict__ urec,
+                                                 const int* __restrict__ cellcc, const int* __restrict__ cslot,
+                                                 const int bid) {
   using LG = LaneGroup<D>;
   // A draw's first round (gate_id >= 0) builds the cell lists from the arrival
-  // slots; the later ones read the lists k_update_commit kept.
+  // slots; the later ones read the lists k_sweep_commit kept.
   const bool first = gate_id >= 0;
   // tested before the first store: read whatever `first` says, so that the loads overlap
-  const int gate_ld = load_late(&ctl->gate), long_ld = load_late(&ctl->long_list);
+  const int gate_ld = load_late(&ctl->gate[gate_id & 1]), long_ld = load_late(&ctl->long_list);
   const int gate = first ? gate_ld : gate_id;
   const int long_list = first ? long_ld : 0;
   const int gtid = bid * blockDim.x + threadIdx.x;
@@ -1244,6 +1243,7 @@ __device__ __forceinline__ void k_partner_lists_body(Geo<D> g, int nshots, const
   }
   if (gate != gate_id) return;  // whole grid
   if (long_list) return;  // a cell of more than CSLOT candidates: the host builds the lists and redoes the round
+  if (!active && sub == 0 && i < nshots) lab[i] = T_DROPPED;
   if (mism) atomicAdd(&ctl->img_mismatch, mism);
   int incl = n;  // inclusive scan over the lane group (stencil order)
 #pragma unroll
@@ -1265,7 +1265,7 @@ __device__ __forceinline__ void k_partner_lists_body(Geo<D> g, int nshots, const
       lab[i] = T_CONFLICT;
     } else {
       lab[i] = T_UNRESOLVED;
-      u = atomicAdd(&ctl->nunres, 1);
+      u = atomicAdd(&ctl->nunres[seq & 1], 1);
       if (total <= KMAX) {
         off = -1;
       } else {
@@ -1313,68 +1313,113 @@ __global__ void __launch_bounds__(WG) k_partner_lists(Geo<D> g, int nshots, cons
                                                  uint8_t* __restrict__ lab, const int* __restrict__ cell2t,
                                                  const int* __restrict__ seg, const int* __restrict__ len,
                                                  const int* __restrict__ ord, int* __restrict__ slab,
-                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int gate_id,
-                                                 URec* __restrict__ urec, const int* __restrict__ cellcc,
-                                                 const int* __restrict__ cslot) {
-  k_partner_lists_body<D>(g, nshots, cpos, crad, cprio, ccell, ctag, lab, cell2t, seg, len, ord, slab, pool, pool_cap, ctl, gate_id, urec, cellcc, cslot, (int)blockIdx.x);
+                                                 int* __restrict__ pool, int pool_cap, DrawCtl* ctl, int seq,
+                                                 int gate_id, URec* __restrict__ urec,
+                                                 const int* __restrict__ cellcc, const int* __restrict__ cslot) {
+  k_partner_lists_body<D>(g, nshots, cpos, crad, cprio, ccell, ctag, lab, cell2t, seg, len, ord, slab, pool, pool_cap,
+                          ctl, seq, gate_id, urec, cellcc, cslot, (int)blockIdx.x);
 }
 
 // The sequential priority sweep (find_solve_conflicts,
-// operator_cells_and_data_storage.ixx:127-142) as a dependency evaluation: a
-// candidate's label is fixed by its first partner (detection order) that is not
-// Skip -- an image or a Conflict partner gives Conflict, an Undecided real
-// partner gives Skip, none leaves it Undecided.  Partners have strictly smaller
-// priorities, so in the reference's sorted sweep they are labelled first; the
-// passes reach the same labels.  One workgroup, labels in LDS (nshots bytes of
-// dynamic shared memory).  The round's counts are left in DrawCtl::pubv, and
-// k_update_commit writes them to pinned host memory.
-__device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
-                                                const int* __restrict__ slab, const int* __restrict__ pool,
-                                                DrawCtl* ctl, int seq, int gate_id, int draw_id,
-                                                double aimed, uint64_t miss_base, const URec* __restrict__ urec, const int bid) {
+// operator_cells_and_data_storage.ixx:127-142) as a dependency evaluation, then
+// update_sphere_tags and the commit (operator_algorithm.ixx:446-473, 383-443),
+// in one kernel of kSweepThreads-thread workgroups.
+//
+// Sweep: a candidate's label is fixed by its first partner (detection order)
+// that is not Skip -- an image or a Conflict partner gives Conflict, an
+// Undecided real partner gives Skip, none leaves it Undecided.  Partners have
+// strictly smaller priorities, so in the reference's sorted sweep they are
+// labelled first; the passes reach the same labels.  Every workgroup runs the
+// whole sweep over the round's labels in its LDS (nshots bytes of dynamic
+// shared memory): the labels are a unique fixed point, so all reach the same
+// ones, and none waits for another.
+//
+// Update and commit: one thread per cell owner, the first kOwnersPerBlock
+// threads of each workgroup (candidate bid * kOwnersPerBlock + thread; the
+// owners' gathers are spread over as many compute units as kernels of WG
+// threads would use): Skip removed from the back, Undecided -> Validated, Conflict ->
+// Undecided; when the round left no Conflict the draw is over and the
+// survivors are appended to their cells in tail order.  Owners touch disjoint
+// cells and candidates.
+//
+// The workgroup after the candidates' (the launch has one more) publishes the
+// round: the control words for the next kernels and the counts to pinned host
+// memory (seq last), its system-scope fence delaying no owner.  Control words
+// that a workgroup reads are never written in the same round: the gate and the
+// unresolved count are kept per parity (of the draw id and of seq), the
+// publisher writing the next draw's gate and clearing the next round's count
+// (also when the gate kept this round from running: that count was the
+// previous round's).
+template <int D>
+__device__ __forceinline__ void k_sweep_commit_body(
+    CellStore<D> cs, int nshots, const double* __restrict__ cpos, const double* __restrict__ crad,
+    const uint32_t* __restrict__ cphase, const int* __restrict__ ccell, const uint8_t* __restrict__ owner,
+    uint8_t* __restrict__ ctag, const uint8_t* __restrict__ lab, const int* __restrict__ slab,
+    const int* __restrict__ pool, const URec* __restrict__ urec, int* __restrict__ seg, int* __restrict__ len,
+    int* __restrict__ ord, int* __restrict__ cell2t, DrawCtl* __restrict__ ctl, int seq, int gate_id, int draw_id,
+    double aimed, uint64_t miss_base, DrawPub* host, int* __restrict__ cellcc, const int* __restrict__ cslot,
+    int ncells, const int bid) {
   extern __shared__ __align__(16) uint8_t s_lab[];
   __shared__ int s_prog, s_left, s_acc, s_con, s_stuck;
-  // everything below reads first and stores only once the gate is known open;
-  // the control words are read by every thread, whatever the tests on them say
-  // (a read under a test or a || is waited for at once)
-  const int gate_ld = load_late(&ctl->gate), povf_ld = load_late(&ctl->pool_overflow);
-  const int long_ld = load_late(&ctl->long_list), nu_ld = load_late(&ctl->nunres);
-  // the counters the round publishes: no kernel of this round changes them
-  // after k_partner_lists, so they are read now, with the loads above
-  const int dropped_ld = load_late(&ctl->dropped), mism_ld = load_late(&ctl->img_mismatch);
-  const int need_ld = load_late(&ctl->need_pool), cell_ld = load_late(&ctl->cell_overflow);
-  const int accd_ld = load_late(&ctl->acc_draw);
-  const int gate = gate_id >= 0 ? gate_ld : gate_id;
+  const int tid = threadIdx.x;
+  // a draw's first round (gate_id >= 0) built the cell lists from the arrival slots
+  const bool first = gate_id >= 0;
+  const int pub_bid = (nshots + kOwnersPerBlock - 1) / kOwnersPerBlock;
+  const bool pub = bid == pub_bid;
+  // Control words: every thread reads them, whatever the tests on them say (a
+  // read under a test or a || is waited for at once); stores come only once the
+  // gate is known open.
+  const int gate_ld = load_late(&ctl->gate[gate_id & 1]), povf_ld = load_late(&ctl->pool_overflow);
+  const int long_ld = load_late(&ctl->long_list), nu_ld = load_late(&ctl->nunres[seq & 1]);
+  const int gate = first ? gate_ld : gate_id;
   const bool ovf = povf_ld != 0;
   // a first round that met a cell of more than CSLOT candidates did nothing: the
   // host builds the lists and redoes it (like a pool overflow)
-  const bool longl = gate_id >= 0 && long_ld != 0;
+  const bool longl = first && long_ld != 0;
   const bool redo_l = ovf || longl;
-  const int tid = threadIdx.x;
-  const int pub_dropped = dropped_ld, pub_mism = mism_ld, pub_povf = povf_ld, pub_need = need_ld,
-            pub_cell = cell_ld, acc_before = accd_ld;
+  // the publisher's counters: no kernel of this round changes them after
+  // k_partner_lists, so they are read now, with the loads above
+  int dropped_ld = 0, mism_ld = 0, need_ld = 0, cell_ld = 0, accd_ld = 0;
+  if (pub) {
+    dropped_ld = load_late(&ctl->dropped);
+    mism_ld = load_late(&ctl->img_mismatch);
+    need_ld = load_late(&ctl->need_pool);
+    cell_ld = load_late(&ctl->cell_overflow);
+    accd_ld = load_late(&ctl->acc_draw);
+  }
+  // The owner's inputs, read without a branch (index clamped to candidate 0):
+  // its owner flag and cell, and its list -- in a draw's first round the cell's
+  // count and arrival slots (cell clamped into the grid), later the list kept by
+  // the previous round (offset, length, and the row at i * CSLOT where a list of
+  // at most CSLOT entries starts) -- and the cell's sphere count, all in bounds
+  // whatever they hold.
+  const int i = bid * kOwnersPerBlock + tid;
+  const bool in = !pub && tid < kOwnersPerBlock && i < nshots;
+  const int ic = in ? i : 0;
+  const uint8_t own = load_now(owner + ic);
+  const bool on = in && own;
+  const int c = load_now(ccell + ic);
+  const int cl = (c >= 0 && c < ncells) ? c : 0;
+  const int sg = first ? ic * CSLOT : load_now(seg + ic);
+  const int n0 = first ? load_now(cellcc + cl) : load_now(len + ic);
+  const int64_t* row =
+      reinterpret_cast<const int64_t*>(first ? cslot + (size_t)cl * CSLOT : ord + (size_t)ic * CSLOT);
+  const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
+  const int4 r0 = make_int4((int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32));
+  const int4 r1 = make_int4((int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32));
+  const int cnt0 = load_now(cs.cnt + cl);  // an owner's cell is in the grid (cl == c)
   if (tid == 0) {
     s_acc = 0;
     s_con = 0;
     s_stuck = 0;
   }
-  // labels into LDS, 8 candidates per thread and load (the buffers start 256-byte aligned)
+  // the round's labels into LDS, 8 candidates per thread and load (k_partner_lists
+  // wrote one for every candidate; the buffer starts 256-byte aligned)
   for (int q8 = tid * 8; q8 < nshots; q8 += 8 * kSweepThreads) {
     if (q8 + 8 <= nshots) {
-      const uint2 t = *reinterpret_cast<const uint2*>(ctag + q8);
-      const uint2 l = *reinterpret_cast<const uint2*>(lab + q8);
-      auto merge = [](uint32_t tw, uint32_t lw) {
-        uint32_t o = 0;
-#pragma unroll
-        for (int b = 0; b < 32; b += 8) {
-          const uint32_t tb = (tw >> b) & 0xffu;
-          o |= (tb == T_UNDECIDED ? (lw >> b) & 0xffu : (uint32_t)T_DROPPED) << b;
-        }
-        return o;
-      };
-      *reinterpret_cast<uint2*>(s_lab + q8) = make_uint2(merge(t.x, l.x), merge(t.y, l.y));
+      *reinterpret_cast<uint2*>(s_lab + q8) = *reinterpret_cast<const uint2*>(lab + q8);
     } else {
-      for (int q = q8; q < nshots; q++) s_lab[q] = ctag[q] == T_UNDECIDED ? lab[q] : T_DROPPED;
+      for (int q = q8; q < nshots; q++) s_lab[q] = lab[q];
     }
   }
   // each thread keeps up to two unresolved candidates and their first PKEEP
@@ -1403,12 +1448,16 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
     ce[c][2] = e.z;
     ce[c][3] = e.w;
   }
-  // the same for the whole workgroup: a scalar value from here on (its wait comes
-  // after the loads above are issued)
+  // the same for the whole workgroup: scalar values from here on (their waits
+  // come after the loads above are issued)
   const int nu = __builtin_amdgcn_readfirstlane(nu_ld);
   const bool redo = __builtin_amdgcn_readfirstlane(redo_l ? 1 : 0) != 0;
+  const bool ran = __builtin_amdgcn_readfirstlane(gate == gate_id ? 1 : 0) != 0;
   __syncthreads();
-  if (gate != gate_id) return;  // the whole workgroup
+  if (!ran) {  // the whole grid
+    if (pub && tid == 0) ctl->nunres[(seq + 1) & 1] = 0;
+    return;
+  }
   // one candidate's label from its partners (first non-Skip one decides), or
   // "wait" while that partner is unresolved
   auto resolve = [&](int m, const int* pl, const int* cache, uint8_t& res) -> bool {
@@ -1450,7 +1499,6 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
         uint8_t res;
         if (resolve(cm[c], cpl[c], ce[c], res)) {
           s_lab[q] = res;
-          lab[q] = res;
           prog++;
         } else {
           left++;
@@ -1467,7 +1515,6 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
         uint8_t res;
         if (resolve(m, pl, cache, res)) {
           s_lab[q] = res;
-          lab[q] = res;
           prog++;
         } else {
           left++;
@@ -1513,105 +1560,42 @@ __device__ __forceinline__ void k_sweep_labels_body(int nshots, const uint8_t* _
     }
   }
   __syncthreads();
-  if (tid != 0) return;
-  ctl->acc = s_acc;
-  ctl->con = s_con;
-  ctl->label_stuck = s_stuck;
-  // counters whose values go into the publication below
-  ctl->dropped = 0;
-  ctl->img_mismatch = 0;
-  ctl->seg_cursor = 0;
-  ctl->nunres = 0;
-  if (!ovf) ctl->pool_cursor = 0;
-  // the draw's refinement decision, as the host makes it after the draw
-  // (operator_algorithm.ixx:170-180): miss rate = misses / (1e-6 + shots),
-  // misses = shots - (spheres now - spheres after the previous draw of this
-  // proceed) in unsigned 64-bit arithmetic; miss_base carries the part of the
-  // difference placed before the draw (non-zero on a proceed's first draw).
-  int open = 0;
-  const int acc_draw = redo ? acc_before : acc_before + s_acc;
-  ctl->acc_draw = acc_draw;
-  if (!redo && !s_stuck && s_con == 0) {
-    const uint64_t miss = (uint64_t)nshots - (uint64_t)acc_draw - miss_base;
-    const double rate = (double)miss / (0.000001 + (double)(uint64_t)nshots);
-    open = !(rate > aimed);
-  }
-  ctl->gate = open ? draw_id + 1 : -1;
-  ctl->swept = seq;
-  const int v[kPubFields] = {s_acc,    s_con,    s_stuck, pub_dropped, pub_mism,
-                             pub_povf, pub_need, pub_cell, open,        longl ? 1 : 0};
-  for (int k = 0; k < kPubFields; k++) ctl->pubv[k] = v[k];  // published by k_update_commit
-}
-
-__global__ void __launch_bounds__(kSweepThreads) k_sweep_labels(int nshots, const uint8_t* __restrict__ ctag, uint8_t* lab,
-                                                const int* __restrict__ slab, const int* __restrict__ pool,
-                                                DrawCtl* ctl, int seq, int gate_id, int draw_id,
-                                                double aimed, uint64_t miss_base, const URec* __restrict__ urec) {
-  k_sweep_labels_body(nshots, ctag, lab, slab, pool, ctl, seq, gate_id, draw_id, aimed, miss_base, urec, (int)blockIdx.x);
-}
-
-// update_sphere_tags on the touched real cells (operator_algorithm.ixx:446-473),
-// one thread per cell owner: Skip removed from the back, Undecided ->
-// Validated, Conflict -> Undecided.  When the round left no Conflict the draw
-// is over and the survivors are appended to their cells in tail order.
-template <int D>
-__device__ __forceinline__ void k_update_commit_body(CellStore<D> cs, int nshots, const double* __restrict__ cpos,
-                                                      const double* __restrict__ crad,
-                                                      const uint32_t* __restrict__ cphase,
-                                                      const int* __restrict__ ccell, const uint8_t* __restrict__ owner,
-                                                      uint8_t* __restrict__ ctag, const uint8_t* __restrict__ lab,
-                                                      int* __restrict__ seg, int* __restrict__ len,
-                                                      int* __restrict__ ord, int* __restrict__ cell2t,
-                                                      DrawCtl* __restrict__ ctl, int seq, DrawPub* host, int first,
-                                                      int* __restrict__ cellcc, const int* __restrict__ cslot,
-                                                      int ncells, const int bid) {
-  const int i = bid * blockDim.x + threadIdx.x;
-  // The round's results go to the host from a workgroup of their own, the one
-  // after the candidates' (the launch has one more): its system-scope fence then
-  // delays no candidate's update.
-  const int pub_bid = (nshots + (int)blockDim.x - 1) / (int)blockDim.x;
-  if (bid == pub_bid) {
-    if (threadIdx.x == 0 && load_now(&ctl->swept) == seq) {
-      // as the sweep left them (seq last)
-      int v[kPubFields];
-#pragma unroll
-      for (int k = 0; k < kPubFields; k++) v[k] = load_now(&ctl->pubv[k]);
-      DrawPub* hp = host + (seq & (kPubSlots - 1));
-#pragma unroll
-      for (int k = 0; k < kPubFields; k++) __atomic_store_n(&reinterpret_cast<int*>(hp)[k], v[k], __ATOMIC_RELAXED);
-      __threadfence_system();
-      __atomic_store_n(&hp->seq, seq, __ATOMIC_RELEASE);
+  const int acc = s_acc, con = s_con, stuck = s_stuck;
+  if (pub) {
+    if (tid != 0) return;
+    // counters whose values go into the publication below, cleared for the next round
+    ctl->dropped = 0;
+    ctl->img_mismatch = 0;
+    ctl->seg_cursor = 0;
+    ctl->nunres[(seq + 1) & 1] = 0;
+    if (!ovf) ctl->pool_cursor = 0;
+    // the draw's refinement decision, as the host makes it after the draw
+    // (operator_algorithm.ixx:170-180): miss rate = misses / (1e-6 + shots),
+    // misses = shots - (spheres now - spheres after the previous draw of this
+    // proceed) in unsigned 64-bit arithmetic; miss_base carries the part of the
+    // difference placed before the draw (non-zero on a proceed's first draw).
+    int open = 0;
+    const int acc_draw = redo ? accd_ld : accd_ld + acc;
+    ctl->acc_draw = acc_draw;
+    if (!redo && !stuck && con == 0) {
+      const uint64_t miss = (uint64_t)nshots - (uint64_t)acc_draw - miss_base;
+      const double rate = (double)miss / (0.000001 + (double)(uint64_t)nshots);
+      open = !(rate > aimed);
     }
+    ctl->gate[(draw_id + 1) & 1] = open ? draw_id + 1 : -1;
+    // the round's results to the host (seq last)
+    const int v[kPubFields] = {acc,     con,    stuck,   dropped_ld, mism_ld,
+                               povf_ld, need_ld, cell_ld, open,       longl ? 1 : 0};
+    DrawPub* hp = host + (seq & (kPubSlots - 1));
+#pragma unroll
+    for (int k = 0; k < kPubFields; k++) __atomic_store_n(&reinterpret_cast<int*>(hp)[k], v[k], __ATOMIC_RELAXED);
+    __threadfence_system();
+    __atomic_store_n(&hp->seq, seq, __ATOMIC_RELEASE);
     return;
   }
-  // Every control word is read here, with the candidate loads below, and tested
-  // only once those are issued (a read under a test or a || is waited for at once).
-  const int swept = load_late(&ctl->swept);
-  const int povf = load_late(&ctl->pool_overflow), stuck = load_late(&ctl->label_stuck);
-  const int longl = load_late(&ctl->long_list), con = load_late(&ctl->con);
-  const bool skip_round = (povf | stuck | (first ? longl : 0)) != 0;
+  const bool skip_round = ovf || stuck || longl;
   const bool commit = con == 0;
-  const bool in = i < nshots;
-  // Every thread also reads, without a branch (index clamped to candidate 0), its
-  // owner flag and cell, and its list: in a draw's first round the cell's count
-  // and arrival slots (cell clamped into the grid), later the list kept by the
-  // previous round (offset, length, and the row at i * CSLOT where a list of at
-  // most CSLOT entries starts).  The loads are issued with the ones above, all in
-  // bounds whatever they hold.
-  const int ic = in ? i : 0;
-  const uint8_t own = load_now(owner + ic);
-  const bool on = in && own;
-  const int c = load_now(ccell + ic);
-  const int cl = (c >= 0 && c < ncells) ? c : 0;
-  const int sg = first ? ic * CSLOT : load_now(seg + ic);
-  const int n0 = first ? load_now(cellcc + cl) : load_now(len + ic);
-  const int64_t* row =
-      reinterpret_cast<const int64_t*>(first ? cslot + (size_t)cl * CSLOT : ord + (size_t)ic * CSLOT);
-  const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
-  const int4 r0 = make_int4((int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32));
-  const int4 r1 = make_int4((int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32));
-  const int cnt0 = load_now(cs.cnt + cl);  // an owner's cell is in the grid (cl == c)
-  if (swept != seq || !on || skip_round) return;  // its sweep did not run (closed gate), or no owner
+  if (!on || skip_round) return;  // no owner, or a round the host redoes
   if (first || (sg == i * CSLOT && n0 <= CSLOT)) {
     // the list in registers: its tags are loaded together, the removal (same
     // steps as the loop below) works on the registers
@@ -1630,7 +1614,7 @@ __device__ __forceinline__ void k_update_commit_body(CellStore<D> cs, int nshots
       for (int k = 0; k < CSLOT; k++) {
         const int j = k < na ? v[k] : i;
         tg[k] = ctag[j];
-        lb[k] = lab[j];
+        lb[k] = s_lab[j];
       }
       nn = na;
 #pragma unroll
@@ -1658,7 +1642,7 @@ __device__ __forceinline__ void k_update_commit_body(CellStore<D> cs, int nshots
       for (int k = 0; k < CSLOT; k++) {
         const int j = k < nn ? v[k] : i;
         tg[k] = ctag[j];
-        lb[k] = lab[j];
+        lb[k] = s_lab[j];
       }
     }
     uint8_t e[CSLOT];
@@ -1731,7 +1715,7 @@ __device__ __forceinline__ void k_update_commit_body(CellStore<D> cs, int nshots
   int n = n0;
   for (int q = n - 1; q >= 0; q--) {
     const int j = a[q];
-    const uint8_t e = ctag[j] == T_UNDECIDED ? lab[j] : ctag[j];
+    const uint8_t e = ctag[j] == T_UNDECIDED ? s_lab[j] : ctag[j];
     if (e == T_SKIP) {
       ctag[j] = T_SKIP;
       a[q] = a[n - 1];
@@ -1741,7 +1725,7 @@ __device__ __forceinline__ void k_update_commit_body(CellStore<D> cs, int nshots
   len[i] = n;
   for (int q = 0; q < n; q++) {
     const int j = a[q];
-    if (ctag[j] == T_UNDECIDED && lab[j] == T_UNDECIDED) ctag[j] = T_VALIDATED;
+    if (ctag[j] == T_UNDECIDED && s_lab[j] == T_UNDECIDED) ctag[j] = T_VALIDATED;
     // Conflict candidates keep the Undecided tag for the next round
   }
   if (!commit) return;
@@ -1765,17 +1749,17 @@ __device__ __forceinline__ void k_update_commit_body(CellStore<D> cs, int nshots
 }
 
 template <int D>
-__global__ void __launch_bounds__(WG) k_update_commit(CellStore<D> cs, int nshots, const double* __restrict__ cpos,
-                                                      const double* __restrict__ crad,
-                                                      const uint32_t* __restrict__ cphase,
-                                                      const int* __restrict__ ccell, const uint8_t* __restrict__ owner,
-                                                      uint8_t* __restrict__ ctag, const uint8_t* __restrict__ lab,
-                                                      int* __restrict__ seg, int* __restrict__ len,
-                                                      int* __restrict__ ord, int* __restrict__ cell2t,
-                                                      DrawCtl* __restrict__ ctl, int seq, DrawPub* host, int first,
-                                                      int* __restrict__ cellcc, const int* __restrict__ cslot,
-                                                      int ncells) {
-  k_update_commit_body<D>(cs, nshots, cpos, crad, cphase, ccell, owner, ctag, lab, seg, len, ord, cell2t, ctl, seq, host, first, cellcc, cslot, ncells, (int)blockIdx.x);
+__global__ void __launch_bounds__(kSweepThreads) k_sweep_commit(
+    CellStore<D> cs, int nshots, const double* __restrict__ cpos, const double* __restrict__ crad,
+    const uint32_t* __restrict__ cphase, const int* __restrict__ ccell, const uint8_t* __restrict__ owner,
+    uint8_t* __restrict__ ctag, const uint8_t* __restrict__ lab, const int* __restrict__ slab,
+    const int* __restrict__ pool, const URec* __restrict__ urec, int* __restrict__ seg, int* __restrict__ len,
+    int* __restrict__ ord, int* __restrict__ cell2t, DrawCtl* __restrict__ ctl, int seq, int gate_id, int draw_id,
+    double aimed, uint64_t miss_base, DrawPub* host, int* __restrict__ cellcc, const int* __restrict__ cslot,
+    int ncells) {
+  k_sweep_commit_body<D>(cs, nshots, cpos, crad, cphase, ccell, owner, ctag, lab, slab, pool, urec, seg, len, ord,
+                         cell2t, ctl, seq, gate_id, draw_id, aimed, miss_base, host, cellcc, cslot, ncells,
+                         (int)blockIdx.x);
 }
 
 }  // namespace rsa_amd
