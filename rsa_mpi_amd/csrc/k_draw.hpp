@@ -34,6 +34,14 @@ __device__ __forceinline__ T load_late(const T* p) {
   return load_now(p + opaque_zero());
 }
 
+// A workgroup barrier for LDS data only: global loads in flight stay in flight
+// (__syncthreads also waits for every outstanding global access).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // The packed stencil entries of a lane's CPL cells (lane `sub` takes entries
 // sub * CPL ..), read at kernel start: they depend on the lane only.
 template <int D, int CPL, int NST>
@@ -859,17 +867,35 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
 // candidate index (the tag loads of unused entries read it, without a branch).
 // The first n arrival slots in shot order (entries n.. become INT_MAX):
 // 19-comparator network for 8 keys (Batcher odd-even merge).
+constexpr int kSortNet[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                 {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
 __device__ __forceinline__ void sort_slots(int n, int (&v)[CSLOT]) {
 #pragma unroll
   for (int k = 0; k < CSLOT; k++)
     if (k >= n) v[k] = INT_MAX;
-  constexpr int NET[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
-                              {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
 #pragma unroll
   for (int e = 0; e < 19; e++) {
-    const int x = v[NET[e][0]], y = v[NET[e][1]];
-    v[NET[e][0]] = min(x, y);
-    v[NET[e][1]] = max(x, y);
+    const int x = v[kSortNet[e][0]], y = v[kSortNet[e][1]];
+    v[kSortNet[e][0]] = min(x, y);
+    v[kSortNet[e][1]] = max(x, y);
+  }
+}
+// the same network, each entry's tag moved with it (the first n entries are
+// distinct candidates; the padding entries' tags are never read)
+__device__ __forceinline__ void sort_slots(int n, int (&v)[CSLOT], uint8_t (&t)[CSLOT]) {
+#pragma unroll
+  for (int k = 0; k < CSLOT; k++)
+    if (k >= n) v[k] = INT_MAX;
+#pragma unroll
+  for (int e = 0; e < 19; e++) {
+    const int a = kSortNet[e][0], b = kSortNet[e][1];
+    const int x = v[a], y = v[b];
+    const uint8_t tx = t[a], ty = t[b];
+    const bool sw = y < x;
+    v[a] = sw ? y : x;
+    v[b] = sw ? x : y;
+    t[a] = sw ? ty : tx;
+    t[b] = sw ? tx : ty;
   }
 }
 
@@ -1393,25 +1419,35 @@ __device__ __forceinline__ void k_sweep_commit_body(
   // the previous round (offset, length, and the row at i * CSLOT where a list of
   // at most CSLOT entries starts) -- and the cell's sphere count, all in bounds
   // whatever they hold.
+  // Those that depend on the owner's index only are issued here, the ones that
+  // depend on its cell after the sweep's loads: the sweep's barriers wait for
+  // LDS only, so the owner's loads stay in flight across them.
+  // (Both rounds' inputs are read, without a branch, and chosen only where the
+  // update begins: a value loaded on one side of a branch is waited for where
+  // the branches meet.)
   const int i = bid * kOwnersPerBlock + tid;
   const bool in = !pub && tid < kOwnersPerBlock && i < nshots;
   const int ic = in ? i : 0;
   const uint8_t own = load_now(owner + ic);
   const bool on = in && own;
   const int c = load_now(ccell + ic);
-  const int cl = (c >= 0 && c < ncells) ? c : 0;
-  const int sg = first ? ic * CSLOT : load_now(seg + ic);
-  const int n0 = first ? load_now(cellcc + cl) : load_now(len + ic);
-  const int64_t* row =
-      reinterpret_cast<const int64_t*>(first ? cslot + (size_t)cl * CSLOT : ord + (size_t)ic * CSLOT);
-  const int64_t q0 = load_now(row), q1 = load_now(row + 1), q2 = load_now(row + 2), q3 = load_now(row + 3);
-  const int4 r0 = make_int4((int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32));
-  const int4 r1 = make_int4((int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32));
-  const int cnt0 = load_now(cs.cnt + cl);  // an owner's cell is in the grid (cl == c)
+  const int sg_l = load_now(seg + ic), n0_l = load_now(len + ic);
+  const int64_t* row_l = reinterpret_cast<const int64_t*>(ord + (size_t)ic * CSLOT);
+  const int64_t l0 = load_now(row_l), l1 = load_now(row_l + 1), l2 = load_now(row_l + 2), l3 = load_now(row_l + 3);
   if (tid == 0) {
     s_acc = 0;
     s_con = 0;
     s_stuck = 0;
+  }
+  // the unresolved records a thread keeps (see below), requested with the labels
+  constexpr int NCR = 2;
+  int4 uh[NCR], ue[NCR];
+#pragma unroll
+  for (int c = 0; c < NCR; c++) {
+    const int u = tid + c * kSweepThreads;
+    const int4* rc = reinterpret_cast<const int4*>(urec + (u < nshots ? u : 0));
+    uh[c] = rc[0];
+    ue[c] = rc[1];
   }
   // the round's labels into LDS, 8 candidates per thread and load (k_partner_lists
   // wrote one for every candidate; the buffer starts 256-byte aligned)
@@ -1424,8 +1460,8 @@ __device__ __forceinline__ void k_sweep_commit_body(
   }
   // each thread keeps up to two unresolved candidates and their first PKEEP
   // partners in registers for all passes; further ones are re-read per pass.
-  // Records below nshots are read whatever the count says (they are in bounds)
-  // and used below the count only, so their loads overlap the count's.
+  // The records are read (below) with the labels, whatever the count says
+  // (index clamped below nshots), and used below the count only.
   constexpr int NC = 2;
   int cq[NC], cm[NC];
   int ce[NC][PKEEP];
@@ -1433,12 +1469,7 @@ __device__ __forceinline__ void k_sweep_commit_body(
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const int u = tid + c * kSweepThreads;
-    int4 h = make_int4(-1, 0, 0, 0), e = make_int4(0, 0, 0, 0);
-    if (u < nshots) {
-      const int4* rc = reinterpret_cast<const int4*>(urec + u);
-      h = rc[0];
-      e = rc[1];
-    }
+    const int4 h = uh[c], e = ue[c];
     const bool use = !redo_l && u < nu_ld;
     cq[c] = use ? h.x : -1;
     cm[c] = use ? h.y : 0;
@@ -1448,12 +1479,36 @@ __device__ __forceinline__ void k_sweep_commit_body(
     ce[c][2] = e.z;
     ce[c][3] = e.w;
   }
+  // the owner's cell: its count and arrival slots (a first round's list), and its sphere count
+  const int cl = (c >= 0 && c < ncells) ? c : 0;
+  const int n0_f = load_now(cellcc + cl);
+  const int64_t* row_f = reinterpret_cast<const int64_t*>(cslot + (size_t)cl * CSLOT);
+  const int64_t f0 = load_now(row_f), f1 = load_now(row_f + 1), f2 = load_now(row_f + 2), f3 = load_now(row_f + 3);
+  const int cnt0 = load_now(cs.cnt + cl);  // an owner's cell is in the grid (cl == c)
+  // The owner's list and its entries' tags, requested before the sweep's
+  // barrier (the labels come from the sweep): a first round's arrival slots in
+  // arrival order, or the list the previous round kept at i * CSLOT.  Only an
+  // owner of a running round with its list in registers reads its entries; the
+  // other reads (entries past the list too) take candidate ic's tag, and every
+  // index is kept below nshots.
+  const int sg = first ? ic * CSLOT : sg_l;
+  const int n0 = first ? n0_f : n0_l;
+  const int64_t q0 = first ? f0 : l0, q1 = first ? f1 : l1, q2 = first ? f2 : l2, q3 = first ? f3 : l3;
+  int ve[CSLOT] = {(int)q0, (int)(q0 >> 32), (int)q1, (int)(q1 >> 32),
+                   (int)q2, (int)(q2 >> 32), (int)q3, (int)(q3 >> 32)};
+  const bool reg = on && gate == gate_id && (first || (sg == i * CSLOT && n0 <= CSLOT));
+  uint8_t te[CSLOT];
+#pragma unroll
+  for (int k = 0; k < CSLOT; k++) {
+    const int j = (reg && k < n0 && (unsigned)ve[k] < (unsigned)nshots) ? ve[k] : ic;
+    te[k] = load_now(ctag + j);
+  }
   // the same for the whole workgroup: scalar values from here on (their waits
   // come after the loads above are issued)
   const int nu = __builtin_amdgcn_readfirstlane(nu_ld);
   const bool redo = __builtin_amdgcn_readfirstlane(redo_l ? 1 : 0) != 0;
   const bool ran = __builtin_amdgcn_readfirstlane(gate == gate_id ? 1 : 0) != 0;
-  __syncthreads();
+  lds_barrier();
   if (!ran) {  // the whole grid
     if (pub && tid == 0) ctl->nunres[(seq + 1) & 1] = 0;
     return;
@@ -1490,7 +1545,7 @@ __device__ __forceinline__ void k_sweep_commit_body(
         s_prog = 0;
         s_left = 0;
       }
-      __syncthreads();
+      lds_barrier();
       int prog = 0, left = 0;
 #pragma unroll
       for (int c = 0; c < NC; c++) {
@@ -1522,16 +1577,16 @@ __device__ __forceinline__ void k_sweep_commit_body(
       }
       if (prog) atomicAdd(&s_prog, prog);
       if (left) atomicAdd(&s_left, left);
-      __syncthreads();
+      lds_barrier();
       const int sp = s_prog, sl = s_left;
-      __syncthreads();
+      lds_barrier();
       if (sl == 0) break;
       if (sp == 0) {
         if (tid == 0) s_stuck = 1;
         break;
       }
     }
-    // counts over 8 labels per thread, summed over the wave, one LDS atomic per wave
+    // counts over 8 labels per thread
     int acc = 0, con = 0;
     for (int q8 = tid * 8; q8 < nshots; q8 += 8 * kSweepThreads) {
       if (q8 + 8 <= nshots) {
@@ -1549,17 +1604,21 @@ __device__ __forceinline__ void k_sweep_commit_body(
         }
       }
     }
+    if (pub) {
+      // both counts in one word (at most 8 per thread and 64 threads: no carry
+      // into the high half), summed over the wave
+      int both = acc + (con << 16);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      acc += __shfl_down(acc, off, 64);
-      con += __shfl_down(con, off, 64);
-    }
-    if ((tid & 63) == 0) {
-      if (acc) atomicAdd(&s_acc, acc);
-      if (con) atomicAdd(&s_con, con);
+      for (int off = 32; off > 0; off >>= 1) both += __shfl_down(both, off, 64);
+      if ((tid & 63) == 0 && both) {
+        atomicAdd(&s_acc, both & 0xffff);
+        atomicAdd(&s_con, both >> 16);
+      }
+    } else if (__ballot(con != 0) != 0 && (tid & 63) == 0) {
+      s_con = 1;  // the owners need only whether some candidate is in Conflict
     }
   }
-  __syncthreads();
+  lds_barrier();
   const int acc = s_acc, con = s_con, stuck = s_stuck;
   if (pub) {
     if (tid != 0) return;
@@ -1599,23 +1658,24 @@ __device__ __forceinline__ void k_sweep_commit_body(
   if (first || (sg == i * CSLOT && n0 <= CSLOT)) {
     // the list in registers: its tags are loaded together, the removal (same
     // steps as the loop below) works on the registers
-    int v[CSLOT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
-    int nn = n0;
-    // tags and labels without a branch (entries past the list read the owner's own)
+    int v[CSLOT], nn = n0;
+    // tags (requested above) and labels, without a branch (entries past the list
+    // read the owner's own)
     uint8_t tg[CSLOT], lb[CSLOT];
+#pragma unroll
+    for (int k = 0; k < CSLOT; k++) {
+      v[k] = ve[k];
+      tg[k] = te[k];
+    }
     if (first) {
       // the list k_lists would build (shot order, round-0 Skips removed from the
       // back with swap-with-last), its tags and labels loaded once and moved with
       // their entries; kept at i * CSLOT for the later rounds; the cell's
       // arrivals restart at 0
       const int na = n0 < CSLOT ? n0 : CSLOT;
-      sort_slots(na, v);
+      sort_slots(na, v, tg);
 #pragma unroll
-      for (int k = 0; k < CSLOT; k++) {
-        const int j = k < na ? v[k] : i;
-        tg[k] = ctag[j];
-        lb[k] = s_lab[j];
-      }
+      for (int k = 0; k < CSLOT; k++) lb[k] = s_lab[k < na ? v[k] : i];
       nn = na;
 #pragma unroll
       for (int q = CSLOT - 1; q >= 0; q--) {
@@ -1639,11 +1699,7 @@ __device__ __forceinline__ void k_sweep_commit_body(
       seg[i] = i * CSLOT;
     } else {
 #pragma unroll
-      for (int k = 0; k < CSLOT; k++) {
-        const int j = k < nn ? v[k] : i;
-        tg[k] = ctag[j];
-        lb[k] = s_lab[j];
-      }
+      for (int k = 0; k < CSLOT; k++) lb[k] = s_lab[k < nn ? v[k] : i];
     }
     uint8_t e[CSLOT];
     bool und[CSLOT];
