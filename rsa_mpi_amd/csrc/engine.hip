@@ -873,6 +873,11 @@ class Engine final : public IEngine {
     ovhead_.reserve(geo_.ncells);
     HIP_OK(hipMemsetAsync(ovhead_.p, 0xff, (size_t)geo_.ncells * sizeof(int), st_));
     cslot_.reserve((size_t)geo_.ncells * CSLOT);
+    cellcc1_.reserve(geo_.ncells);
+    ovhead1_.reserve(geo_.ncells);
+    HIP_OK(hipMemsetAsync(ovhead1_.p, 0xff, (size_t)geo_.ncells * sizeof(int), st_));
+    cslot1_.reserve((size_t)geo_.ncells * CSLOT);
+    HIP_OK(hipMemsetAsync(cellcc1_.p, 0, (size_t)geo_.ncells * sizeof(int), st_));
     HIP_OK(hipMemsetAsync(cells_cnt_.p, 0, geo_.ncells, st_));
     HIP_OK(hipMemsetAsync(cell2t_.p, 0xff, (size_t)geo_.ncells * sizeof(int), st_));
     HIP_OK(hipMemsetAsync(cellcc_.p, 0, (size_t)geo_.ncells * sizeof(int), st_));
@@ -994,10 +999,21 @@ class Engine final : public IEngine {
     owner_.reserve(c);
     slab_.reserve(c * KMAX);
     urec_.reserve(c);
+    cpos1_.reserve(c * D);
+    crad1_.reserve(c);
+    cprio1_.reserve(c);
+    cphase1_.reserve(c);
+    ccell1_.reserve(c);
+    ctag1_.reserve(c);
+    next1_.reserve(c);
+    owner1_.reserve(c);
     // kernels of a draw whose gate stays closed read these before returning
     HIP_OK(hipMemsetAsync(owner_.p, 0, c * sizeof(uint8_t), st_));
     HIP_OK(hipMemsetAsync(ccell_.p, 0xff, c * sizeof(int), st_));
     HIP_OK(hipMemsetAsync(ctag_.p, T_DROPPED, c * sizeof(uint8_t), st_));
+    HIP_OK(hipMemsetAsync(owner1_.p, 0, c * sizeof(uint8_t), st_));
+    HIP_OK(hipMemsetAsync(ccell1_.p, 0xff, c * sizeof(int), st_));
+    HIP_OK(hipMemsetAsync(ctag1_.p, T_DROPPED, c * sizeof(uint8_t), st_));
     if (plist_.cap < 8 * c) plist_.reserve(8 * c);
   }
 
@@ -1300,7 +1316,10 @@ class Engine final : public IEngine {
   void law_radii(const Prep& pr, double* out);
   void init_pub();
   void enqueue_round(const Prep& pr, int seq, int gate_id);
-  void enqueue_first(Prep& pr, bool forced, uint64_t miss_base);
+  void enqueue_decode(Prep& pr);
+  void enqueue_front(Prep& pr, uint64_t miss_base);
+  void enqueue_settle(const Prep& pr, int seq, int gate_id, Prep* next);
+  DecodeArgs<D> decode_args(const Prep& pr, int forced);
   int64_t finish_draw(const Prep& pr, int64_t& rounds, bool* next_live, bool& gate_open, rsa_gpu_stats* st);
   int draw_id_ = 0;
   int64_t single_draw_slow(int nshots, const double* hrad, uint64_t maximal, bool may_outreach, uint64_t w0,
@@ -1357,6 +1376,33 @@ class Engine final : public IEngine {
   DevBuf<int> next_, cslot_, ovhead_, slab_, pool_;
   DevBuf<URec> urec_;
   DevBuf<uint8_t> owner_;
+  // What a draw's decode writes, in two sets: a draw uses the set of the parity
+  // of its id, so that the next draw's decode runs while this one commits
+  // (k_sweep_commit_decode).  Set 0 is the buffers above (also those of the
+  // host-driven path and of the batched form).
+  DevBuf<double> cpos1_, crad1_;
+  DevBuf<uint32_t> cprio1_, cphase1_;
+  DevBuf<int> ccell1_, next1_, cellcc1_, cslot1_, ovhead1_;
+  DevBuf<uint8_t> ctag1_, owner1_;
+  struct CandSet {
+    double* cpos;
+    double* crad;
+    uint32_t* cprio;
+    uint32_t* cphase;
+    int* ccell;
+    uint8_t* ctag;
+    uint8_t* owner;
+    int* next;
+    int* cellcc;
+    int* cslot;
+    int* ovhead;
+  };
+  CandSet cset(int id) const {
+    if (id & 1) return {cpos1_.p, crad1_.p, cprio1_.p, cphase1_.p, ccell1_.p, ctag1_.p, owner1_.p, next1_.p, cellcc1_.p,
+                        cslot1_.p, ovhead1_.p};
+    return {cpos_.p, crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p, owner_.p, next_.p, cellcc_.p, cslot_.p,
+            ovhead_.p};
+  }
   DevBuf<int> stab_;  // packed stencil offsets (Geo::stab), then linear ones (Geo::slin)
 
   // learned predictor
@@ -1496,32 +1542,38 @@ void Engine<D>::proceed(uint64_t seed, int64_t max_draws, rsa_gpu_stats* st) {
     } else {
       fast = true;
       init_pub();
+      const auto tq = std::chrono::steady_clock::now();
       if (!cur_live) {
         ensure_cand(cur.nshots);
-        enqueue_first(cur, true, (uint64_t)(int)nb_fast_ - old_nb);
+        enqueue_decode(cur);
+        enqueue_front(cur, (uint64_t)(int)nb_fast_ - old_nb);
       }
-      // Queue up to kQueueDepth draws behind this one, each gated on the one
-      // before it.  Their inputs hold unless a draw before them triggers a
-      // refinement (the device then keeps the gate closed and every later queued
-      // draw returns at once); they cannot change the class or overshoot its
-      // target: the shots of cur and of every queued draw sum below maximal.
-      // With the draw head predicting a refinement, nothing more is queued ahead
-      // (the gates would stay closed); the device still decides, as always.
+      // The next draw is queued behind this one (kQueueDepth = 1), gated on it:
+      // its decode runs in this draw's sweep-and-commit launch, then come its
+      // filter and partner lists; its own sweep-and-commit launch comes with the
+      // decode of the draw after it.  Its inputs hold unless this draw triggers a
+      // refinement (the device then keeps the gate closed and the queued draw's
+      // kernels undo its decode or return at once); it cannot change the class or
+      // overshoot its target: the shots of cur and of the queued draw sum below
+      // maximal.  With the draw head predicting a refinement, nothing is queued
+      // (the gate would stay closed); the device still decides, as always.
       if (predicted_refine) lst_.speculation_skipped++;
-      uint64_t queued_shots = nshots;
-      for (int k = 0; k < live; k++) queued_shots += (uint64_t)ahead[k].nshots;
-      while (live < kQueueDepth && !predicted_refine && (max_draws < 0 || st->draws + 2 + live <= max_draws)) {
-        if ((int)ahead.size() == live) {
-          const uint64_t wn = live == 0 ? cur.w_end() : ahead[live - 1].w_end();
+      Prep* nx = nullptr;
+      if (live < kQueueDepth && !predicted_refine && (max_draws < 0 || st->draws + 2 <= max_draws)) {
+        if (ahead.empty()) {
+          const uint64_t wn = cur.w_end();
           ahead.push_back(host_timed("host_prepare_next", [&] { return prep_get(wn, nvox_); }));
         }
-        Prep& n = ahead[live];
-        if (!(n.nshots <= 48000 && (size_t)n.nshots <= cand_cap_ && queued_shots + (uint64_t)n.nshots < maximal))
-          break;  // it runs as a forced draw later
-        enqueue_first(n, false, 0);  // the miss count restarts after each draw
-        queued_shots += (uint64_t)n.nshots;
-        live++;
+        Prep& n = ahead.front();
+        if (n.nshots <= 48000 && (size_t)n.nshots <= cand_cap_ && nshots + (uint64_t)n.nshots < maximal) nx = &n;
       }
+      enqueue_settle(cur, cur.seq, cur.id, nx);
+      if (nx != nullptr) {
+        enqueue_front(*nx, 0);  // the miss count restarts after each draw
+        live = 1;
+      }
+      if (prof_)
+        prof_add("host_enqueue_draw", std::chrono::duration<double>(std::chrono::steady_clock::now() - tq).count(), 0);
       nb_new = (uint64_t)finish_draw(cur, rounds, &spec_ok, gate_open, st);
       if (!spec_ok) live = 0;  // their gates stayed closed: they are queued again, forced
     }

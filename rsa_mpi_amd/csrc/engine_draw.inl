@@ -274,7 +274,7 @@ void Engine<D>::wait_pub(int seq) {
 // one host round trip (the round's counts reach pinned host memory from the
 // publishing workgroup of k_sweep_commit; the tag update and commit run while
 // the host moves on).
-// enqueue_first queues a draw's first round (gated unless forced);
+// enqueue_decode / enqueue_front / enqueue_settle queue a draw's first round;
 // finish_draw waits for its rounds and queues the further ones.
 template <int D>
 void Engine<D>::init_pub() {
@@ -293,51 +293,150 @@ void Engine<D>::enqueue_round(const Prep& pr, int seq, int gate_id) {
   const int nshots = pr.nshots;
   using LG = LaneGroup<D>;
   const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
+  const CandSet c = cset(pr.id);
   timed("k_partner_lists", (double)nshots * (8.0 * (D + 1) + 10.0), [&] {
-    hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, cpos_.p, crad_.p, cprio_.p,
-                       ccell_.p, ctag_.p, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, slab_.p, pool_.p,
-                       (int)pool_.cap, dctl_.p, seq, gate_id, urec_.p, cellcc_.p, cslot_.p);
+    hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, c.cpos, c.crad, c.cprio,
+                       c.ccell, c.ctag, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, slab_.p, pool_.p,
+                       (int)pool_.cap, dctl_.p, seq, gate_id, urec_.p, c.cellcc, c.cslot);
   });
-  // the owners' workgroups and one more that publishes the round
-  const unsigned nsc = (unsigned)((nshots + kOwnersPerBlock - 1) / kOwnersPerBlock + 1);
-  timed("k_sweep_commit", (double)nshots * (8.0 * (D + 1) + 16.0), [&] {
-    hipLaunchKernelGGL(k_sweep_commit<D>, dim3(nsc), dim3(kSweepThreads), (size_t)nshots, st_, cs(), nshots,
-                       cpos_.p, crad_.p, cphase_.p, ccell_.p, owner_.p, ctag_.p, lab_.p, slab_.p, pool_.p, urec_.p,
-                       seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, seq, gate_id, pr.id, aimed_, pr.miss_base,
-                       hpub_dev_, cellcc_.p, cslot_.p, geo_.ncells);
-  });
-  HIP_OK(hipGetLastError());
+  enqueue_settle(pr, seq, gate_id, nullptr);
 }
 
+// The next draw's decode inputs (its candidate buffers: the set of its id).
 template <int D>
-void Engine<D>::enqueue_first(Prep& pr, bool forced, uint64_t miss_base) {
-  const auto tq = std::chrono::steady_clock::now();
-  const int nshots = pr.nshots;
-  const uint64_t w0 = pr.w_shots;
+DecodeArgs<D> Engine<D>::decode_args(const Prep& pr, int forced) {
+  const CandSet c = cset(pr.id);
+  DecodeArgs<D> a{};
+  a.ring = ring_.p;
+  a.mask = kRing - 1;
+  a.w0 = pr.w_shots;
+  a.nshots = pr.nshots;
+  a.N = (uint32_t)nvox_;
+  a.vpos = hvpos_dev_[pr.buf];
+  a.hrad = pr.law ? hrad_dev_[pr.buf] : nullptr;
+  a.rl = pr.rej;
+  a.end_rel = pr.end_rel;
+  a.vx = vview();
+  a.cls = class_dev();
+  a.g = geo_;
+  a.cs = cs();
+  a.cpos = c.cpos;
+  a.crad = c.crad;
+  a.cprio = c.cprio;
+  a.cphase = c.cphase;
+  a.ccell = c.ccell;
+  a.ctag = c.ctag;
+  a.owner = c.owner;
+  a.cellcc = c.cellcc;
+  a.cslot = c.cslot;
+  a.ovhead = c.ovhead;
+  a.next = c.next;
+  a.ctl = dctl_.p;
+  a.draw_id = pr.id;
+  a.forced = forced;
+  a.list_thr = list_thr_;
+  return a;
+}
+
+// A draw's decode on its own, forced (its launch opens the draw's gate): the
+// first draw after the queue of draws broke off.  k_decode_filter reads the shot
+// words from the device ring and the chain offsets from pinned host memory
+// (written before this launch, untouched until the draw's results have been
+// published).
+template <int D>
+void Engine<D>::enqueue_decode(Prep& pr) {
   pr.id = ++draw_id_;
+  host_timed("host_dev_words", [&] { dev_words(pr.w_shots, pr.w_shots + (uint64_t)pr.end_rel + (uint64_t)pr.nshots); });
+  using LG = LaneGroup<D>;
+  const unsigned nbl = nblocks((int64_t)pr.nshots * LG::LPC);
+  const DecodeArgs<D> a = decode_args(pr, 1);
+  timed("k_decode_filter", (double)pr.nshots * (4.0 * (2 * D + 2) + 4.0 * D + 8.0 * (D + 1) + 13.0), [&] {
+    auto kern = a.vx.wide ? k_decode_filter<D, true, false> : k_decode_filter<D, false, false>;
+    hipLaunchKernelGGL(kern, dim3(nbl), dim3(WG), 0, st_, a.ring, a.mask, a.w0, a.nshots, a.N, a.vpos, a.hrad, a.rl,
+                       a.end_rel, a.vx, a.cls, a.g, a.cs, a.cpos, a.crad, a.cprio, a.cphase, a.ccell, a.ctag, a.owner,
+                       a.cellcc, a.cslot, a.ovhead, a.next, a.ctl, a.draw_id, a.forced, a.list_thr);
+  });
+}
+
+// A decoded draw's first round up to its sweep: the validated filter (or, when
+// its gate stays closed, the undoing of its decode's arrivals) and the partner
+// lists, which also build the cell lists from the arrival slots (k_lists runs
+// only when a cell got more than CSLOT candidates).
+template <int D>
+void Engine<D>::enqueue_front(Prep& pr, uint64_t miss_base) {
   pr.seq = ++seq_;
   pr.miss_base = miss_base;
-  // k_decode_filter reads the shot words from the device ring and the chain
-  // offsets from pinned host memory (written before this launch, untouched
-  // until the draw's results have been published).
-  host_timed("host_dev_words", [&] { dev_words(w0, w0 + (uint64_t)pr.end_rel + (uint64_t)nshots); });
-  const uint32_t N = (uint32_t)nvox_;
-  const ClassDev cls = class_dev();
+  const int nshots = pr.nshots;
   using LG = LaneGroup<D>;
   const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
-  timed("k_decode_filter", (double)nshots * (4.0 * (2 * D + 2) + 4.0 * D + 8.0 * (D + 1) + 13.0), [&] {
-    const VoxView<D> vv = vview();
-    auto kern = vv.wide ? k_decode_filter<D, true> : k_decode_filter<D, false>;
-    hipLaunchKernelGGL(kern, dim3(nbl), dim3(WG), 0, st_, ring_.p, kRing - 1, w0, nshots, N, hvpos_dev_[pr.buf],
-                       pr.law ? hrad_dev_[pr.buf] : nullptr, pr.rej, pr.end_rel, vv, cls, geo_, cs(), cpos_.p,
-                       crad_.p, cprio_.p, cphase_.p, ccell_.p, ctag_.p, owner_.p, cellcc_.p, cslot_.p, ovhead_.p,
-                       next_.p, dctl_.p, pr.id, forced ? 1 : 0, list_thr_);
+  const CandSet c = cset(pr.id);
+  timed("k_filter", (double)nshots * (8.0 * (D + 1) + 5.0), [&] {
+    hipLaunchKernelGGL(k_filter<D>, dim3(nbl), dim3(WG), 0, st_, geo_, cs(), nshots, c.cpos, c.crad, c.ccell, c.ctag,
+                       c.cellcc, c.ovhead, dctl_.p, pr.id);
   });
-  // the first round's partner and update kernels build the cell lists themselves
-  // (k_lists runs only when a cell got more than CSLOT candidates)
-  enqueue_round(pr, pr.seq, pr.id);
-  if (prof_)
-    prof_add("host_enqueue_draw", std::chrono::duration<double>(std::chrono::steady_clock::now() - tq).count(), 0);
+  timed("k_partner_lists", (double)nshots * (8.0 * (D + 1) + 10.0), [&] {
+    hipLaunchKernelGGL(k_partner_lists<D>, dim3(nbl), dim3(WG), 0, st_, geo_, nshots, c.cpos, c.crad, c.cprio,
+                       c.ccell, c.ctag, lab_.p, cell2t_.p, seg_.p, len_.p, ord_.p, slab_.p, pool_.p,
+                       (int)pool_.cap, dctl_.p, pr.seq, pr.id, urec_.p, c.cellcc, c.cslot);
+  });
+}
+
+// A round's sweep, tag update, commit and publication; with `next`, the next
+// draw's decode in the same launch (its id assigned here, its words made
+// readable first).  The launch has the owners' workgroups and one more that
+// publishes the round, then the decode's.
+template <int D>
+void Engine<D>::enqueue_settle(const Prep& pr, int seq, int gate_id, Prep* next) {
+  const int nshots = pr.nshots;
+  const unsigned nsc = (unsigned)((nshots + kOwnersPerBlock - 1) / kOwnersPerBlock + 1);
+  const CandSet c = cset(pr.id);
+  SweepArgs<D> sa{};
+  sa.cs = cs();
+  sa.nshots = nshots;
+  sa.cpos = c.cpos;
+  sa.crad = c.crad;
+  sa.cphase = c.cphase;
+  sa.ccell = c.ccell;
+  sa.owner = c.owner;
+  sa.ctag = c.ctag;
+  sa.lab = lab_.p;
+  sa.slab = slab_.p;
+  sa.pool = pool_.p;
+  sa.urec = urec_.p;
+  sa.seg = seg_.p;
+  sa.len = len_.p;
+  sa.ord = ord_.p;
+  sa.cell2t = cell2t_.p;
+  sa.ctl = dctl_.p;
+  sa.seq = seq;
+  sa.gate_id = gate_id;
+  sa.draw_id = pr.id;
+  sa.aimed = aimed_;
+  sa.miss_base = pr.miss_base;
+  sa.host = hpub_dev_;
+  sa.cellcc = c.cellcc;
+  sa.cslot = c.cslot;
+  sa.ncells = geo_.ncells;
+  if (next == nullptr) {
+    timed("k_sweep_commit", (double)nshots * (8.0 * (D + 1) + 16.0), [&] {
+      hipLaunchKernelGGL(k_sweep_commit<D>, dim3(nsc), dim3(kSweepThreads), (size_t)nshots, st_, sa.cs, nshots,
+                         sa.cpos, sa.crad, sa.cphase, sa.ccell, sa.owner, sa.ctag, sa.lab, sa.slab, sa.pool, sa.urec,
+                         sa.seg, sa.len, sa.ord, sa.cell2t, sa.ctl, seq, gate_id, pr.id, sa.aimed, sa.miss_base,
+                         sa.host, sa.cellcc, sa.cslot, sa.ncells);
+    });
+  } else {
+    next->id = ++draw_id_;  // the publisher of this round opens its gate (id + 1) or not
+    host_timed("host_dev_words",
+               [&] { dev_words(next->w_shots, next->w_shots + (uint64_t)next->end_rel + (uint64_t)next->nshots); });
+    using LG = LaneGroup<D>;
+    const unsigned ndec = nblocks((int64_t)next->nshots * LG::LPC, kSweepThreads);
+    const DecodeArgs<D> da = decode_args(*next, 2);
+    timed("k_sweep_commit_decode", (double)nshots * (8.0 * (D + 1) + 16.0), [&] {
+      auto kern = da.vx.wide ? k_sweep_commit_decode<D, true> : k_sweep_commit_decode<D, false>;
+      hipLaunchKernelGGL(kern, dim3(nsc + ndec), dim3(kSweepThreads), (size_t)nshots, st_, sa, (int)nsc, da);
+    });
+  }
+  HIP_OK(hipGetLastError());
 }
 
 // Waits for the rounds of draw pr (its first round queued), queues its further
@@ -350,6 +449,7 @@ int64_t Engine<D>::finish_draw(const Prep& pr, int64_t& rounds, bool* next_live,
   using LG = LaneGroup<D>;
   const unsigned nbl = nblocks((int64_t)nshots * LG::LPC);
   const unsigned nbs = nblocks(nshots);
+  const CandSet c = cset(pr.id);
   int seq = pr.seq;
   int64_t local_new = 0;
   int r = 0;
@@ -358,8 +458,8 @@ int64_t Engine<D>::finish_draw(const Prep& pr, int64_t& rounds, bool* next_live,
   bool lists_kept = false;
   auto build_lists = [&]() {  // k_lists, forced: once per draw, before the cell counts are reset
     timed("k_lists", (double)nshots * 24.0, [&] {
-      hipLaunchKernelGGL(k_lists, dim3(nbs), dim3(WG), 0, st_, nshots, ccell_.p, owner_.p, ctag_.p, cellcc_.p,
-                         cslot_.p, ovhead_.p, next_.p, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, -1, geo_.ncells);
+      hipLaunchKernelGGL(k_lists, dim3(nbs), dim3(WG), 0, st_, nshots, c.ccell, c.owner, c.ctag, c.cellcc, c.cslot,
+                         c.ovhead, c.next, seg_.p, len_.p, ord_.p, cell2t_.p, dctl_.p, -1, geo_.ncells);
     });
     lists_kept = true;
   };
@@ -372,7 +472,7 @@ int64_t Engine<D>::finish_draw(const Prep& pr, int64_t& rounds, bool* next_live,
     st->image_cell_mismatch += h.img_mismatch;
     if (h.need_lists) {  // a cell got more than CSLOT candidates: build the lists, redo the first round
       *next_live = false;
-      HIP_OK(hipMemsetAsync(&dctl_.p->long_list, 0, sizeof(int), st_));
+      HIP_OK(hipMemsetAsync(&dctl_.p->long_list[pr.id & 1], 0, sizeof(int), st_));
       build_lists();
       if (prof_) prof_add("host_list_redo", 0, 0);
       seq = ++seq_;
@@ -399,10 +499,10 @@ int64_t Engine<D>::finish_draw(const Prep& pr, int64_t& rounds, bool* next_live,
     lists_kept = true;  // kept by the round's update kernel
     *next_live = false;
     timed("k_refilter", (double)nshots * (8.0 * (D + 1) + 5.0), [&] {
-      hipLaunchKernelGGL(k_refilter<D>, dim3(nbl), dim3(WG), 0, st_, geo_, cs(), nshots, cpos_.p, crad_.p, ccell_.p,
-                         ctag_.p, cell2t_.p, seg_.p, len_.p, ord_.p);
+      hipLaunchKernelGGL(k_refilter<D>, dim3(nbl), dim3(WG), 0, st_, geo_, cs(), nshots, c.cpos, c.crad, c.ccell,
+                         c.ctag, cell2t_.p, seg_.p, len_.p, ord_.p);
     });
-    hipLaunchKernelGGL(k_remove_skips, dim3(nbs), dim3(WG), 0, st_, nshots, owner_.p, seg_.p, len_.p, ord_.p, ctag_.p);
+    hipLaunchKernelGGL(k_remove_skips, dim3(nbs), dim3(WG), 0, st_, nshots, c.owner, seg_.p, len_.p, ord_.p, c.ctag);
     seq = ++seq_;
     enqueue_round(pr, seq, -1);
   }

@@ -150,11 +150,11 @@ __global__ void __launch_bounds__(WG) kb_decode_filter(const DrawDesc<D>* __rest
   const int r = blockIdx.x / bpr, bid = blockIdx.x % bpr;
   const DrawDesc<D>& a = tab[r];
   if (a.vx.wide)
-    k_decode_filter_body<D, true>(a.ring, a.mask, a.w0, a.nshots, a.N, a.vpos, a.hrad, a.rl, a.end_rel, a.vx, a.cls,
+    k_decode_filter_body<D, true, true>(a.ring, a.mask, a.w0, a.nshots, a.N, a.vpos, a.hrad, a.rl, a.end_rel, a.vx, a.cls,
                                   a.g, a.cs, a.cpos, a.crad, a.cprio, a.cphase, a.ccell, a.ctag, a.owner, a.cellcc,
                                   a.cslot, a.ovhead, a.next, a.ctl, a.draw_id, a.forced, a.list_thr, bid);
   else
-    k_decode_filter_body<D, false>(a.ring, a.mask, a.w0, a.nshots, a.N, a.vpos, a.hrad, a.rl, a.end_rel, a.vx, a.cls,
+    k_decode_filter_body<D, false, true>(a.ring, a.mask, a.w0, a.nshots, a.N, a.vpos, a.hrad, a.rl, a.end_rel, a.vx, a.cls,
                                    a.g, a.cs, a.cpos, a.crad, a.cprio, a.cphase, a.ccell, a.ctag, a.owner, a.cellcc,
                                    a.cslot, a.ovhead, a.next, a.ctl, a.draw_id, a.forced, a.list_thr, bid);
 }
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(WG) kb_lists(const DrawDesc<D>* __restrict__ t
   const int r = blockIdx.x / bpr, bid = blockIdx.x % bpr;
   const DrawDesc<D>& a = tab[r];
   if (!a.lists) return;
-  if (bid == 0 && threadIdx.x == 0) a.ctl->long_list = 0;  // no kernel of the redone round reads it
+  if (bid == 0 && threadIdx.x == 0) a.ctl->long_list[a.draw_id & 1] = 0;  // no kernel of the redone round reads it
   k_lists_body(a.nshots, a.ccell, a.owner, a.ctag, a.cellcc, a.cslot, a.ovhead, a.next, a.seg, a.len, a.ord,
                a.cell2t, a.ctl, -1, a.ncells, bid);
 }

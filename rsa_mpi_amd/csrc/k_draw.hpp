@@ -636,7 +636,7 @@ struct RejList {
 
 // Control block of the fast draw path (device side).
 struct DrawCtl {
-  int dropped;
+  int dropped[2];  // per parity of the draw id, like the other words a draw's decode writes
   int img_mismatch;
   int seg_cursor;
   int pool_cursor;
@@ -652,8 +652,8 @@ struct DrawCtl {
   // once.
   int gate[2];
   int nunres[2];  // candidates k_partner_lists left unresolved (one URec each), per parity of seq
-  int acc_draw;   // spheres accepted so far in the current draw
-  int long_list;  // a cell got more than CSLOT candidates this draw (set by k_decode_filter)
+  int acc_draw[2];   // spheres accepted so far in the draw, per parity of its id
+  int long_list[2];  // a cell got more than CSLOT candidates in the draw (set by its decode), per parity
 };
 // One unresolved candidate as k_sweep_commit reads it, written by
 // k_partner_lists at the candidate's place in the unresolved list: index,
@@ -684,7 +684,56 @@ constexpr int kPubSlots = 8;
 // (operator_cells_and_data_storage.ixx:54-76: at this point only placed spheres
 // and their images are validated); each candidate takes an arrival slot in its
 // cell (shot order is restored in k_lists).
-template <int D, bool WIDE>
+// The validated filter of one lane of a candidate's lane group (p, r in cell
+// `cell`): whether a placed sphere of the lane's stencil cells collides with it
+// (round-0 remove_collisions_with_validated_spheres,
+// operator_cells_and_data_storage.ixx:54-76).  With take, the candidate's
+// arrival slot is taken (into a) after the walk's first loads (a cell's count,
+// then its first slots) and waited for by the caller's first use: loads and
+// atomics complete in issue order for the wave's counter, so an atomic issued
+// before them would hold back every wait of the walk.
+template <int D>
+__device__ __forceinline__ bool filter_walk(const Geo<D>& g, const CellStore<D>& cs, int64_t cell, int sub,
+                                            const int* pk, const double* p, double r, bool take,
+                                            int* __restrict__ cellcc, int& a) {
+  using LG = LaneGroup<D>;
+  bool hit = false;
+  int cc[D];
+  cell_coords<D>(g, cell, cc);
+#pragma unroll
+  for (int jj = 0; jj < LG::CPL; jj++) {
+    const int s = sub * LG::CPL + jj;
+    Nb<D> nb;
+    const bool use = s < LG::NST && !hit && neighbour_at<D>(g, cc, pk[jj], nb);
+    // slots are read only for occupied cells (most neighbour cells of a late
+    // draw are empty); the first kSlotsAtOnce of them are requested together
+    const int nv = use ? (int)cs.cnt[nb.src] : 0;
+    const Slot<D>* sl = cs.sph + (size_t)(use ? nb.src : 0) * cs.cap;
+    Slot<D> q4[kSlotsAtOnce];
+#pragma unroll
+    for (int k = 0; k < kSlotsAtOnce; k++)
+      if (k < nv) q4[k] = sl[k];
+    if (jj == 0 && take) a = atomicAdd(&cellcc[cell], 1);
+    auto collides = [&](const Slot<D>& q) {
+      double qp[D];
+      if (nb.ghost) {
+        if (!image_of<D>(g, nb, q.c, qp)) return false;
+      } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) qp[d] = q.c[d];
+      }
+      return collide<D>(r, p, q.r, qp);
+    };
+#pragma unroll
+    for (int k = 0; k < kSlotsAtOnce; k++)
+      if (k < nv && !hit && collides(q4[k])) hit = true;
+    for (int k = kSlotsAtOnce; k < nv && !hit; k++)
+      if (collides(sl[k])) hit = true;
+  }
+  return hit;
+}
+
+template <int D, bool WIDE, bool FILTER>
 __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
                                                       int nshots, uint32_t N, const uint32_t* __restrict__ vpos,
                                                       const double* __restrict__ hrad, RejList rl,
@@ -770,53 +819,27 @@ __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict_
       ok = !coords_ghost<D>(g, cc);
     }
   }
-  // The arrival slot does not depend on the filter's outcome.  Its atomic is
-  // issued after the walk's first loads (a cell's count, then its first slots)
-  // and waited for at the end: loads and atomics complete in issue order for the
-  // wave's counter, so an atomic issued before them would hold back every wait
-  // of the walk.  With a closed gate nothing is written (the test below returns).
+  // The arrival slot does not depend on the filter's outcome (filter_walk takes
+  // it after its first loads; FILTER false: no walk, k_filter runs it later).
+  // With a closed gate nothing is written (the test below returns).  forced: 0,
+  // gated on gate[draw_id & 1]; 1, forced (the launch opens the gate); 2, run
+  // speculatively inside the previous draw's k_sweep_commit_decode (k_filter
+  // checks the gate and undoes the arrivals of a draw not taken).
   const bool take = valid && ok && sub == 0 && gate == draw_id;
   int a = -1;
   bool hit = false;
   if (valid && ok) {
-    int cc[D];
-    cell_coords<D>(g, cell, cc);
-#pragma unroll
-    for (int jj = 0; jj < LG::CPL; jj++) {
-      const int s = sub * LG::CPL + jj;
-      Nb<D> nb;
-      const bool use = s < LG::NST && !hit && neighbour_at<D>(g, cc, pk[jj], nb);
-      // slots are read only for occupied cells (most neighbour cells of a late
-      // draw are empty); the first kSlotsAtOnce of them are requested together
-      const int nv = use ? (int)cs.cnt[nb.src] : 0;
-      const Slot<D>* sl = cs.sph + (size_t)(use ? nb.src : 0) * cs.cap;
-      Slot<D> q4[kSlotsAtOnce];
-#pragma unroll
-      for (int k = 0; k < kSlotsAtOnce; k++)
-        if (k < nv) q4[k] = sl[k];
-      if (jj == 0 && take) a = atomicAdd(&cellcc[cell], 1);
-      auto collides = [&](const Slot<D>& q) {
-        double qp[D];
-        if (nb.ghost) {
-          if (!image_of<D>(g, nb, q.c, qp)) return false;
-        } else {
-#pragma unroll
-          for (int d = 0; d < D; d++) qp[d] = q.c[d];
-        }
-        return collide<D>(r, p, q.r, qp);
-      };
-#pragma unroll
-      for (int k = 0; k < kSlotsAtOnce; k++)
-        if (k < nv && !hit && collides(q4[k])) hit = true;
-      for (int k = kSlotsAtOnce; k < nv && !hit; k++)
-        if (collides(sl[k])) hit = true;
+    if constexpr (FILTER) {
+      hit = filter_walk<D>(g, cs, cell, sub, pk, p, r, take, cellcc, a);
+    } else if (take) {
+      a = atomicAdd(&cellcc[cell], 1);
     }
   }
   const bool anyhit = (__ballot(hit) & LG::mask(lane)) != 0;
   if (gate != draw_id) return;  // whole grid
   if (gtid == 0) {
-    ctl->acc_draw = 0;
-    if (forced) ctl->gate[draw_id & 1] = draw_id;
+    ctl->acc_draw[draw_id & 1] = 0;
+    if (forced == 1) ctl->gate[draw_id & 1] = draw_id;
   }
   if (valid && sub == 0) {
 #pragma unroll
@@ -828,23 +851,23 @@ __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict_
       ccell[i] = -1;
       ctag[i] = T_DROPPED;
       owner[i] = 0;
-      atomicAdd(&ctl->dropped, 1);
+      atomicAdd(&ctl->dropped[draw_id & 1], 1);
     } else {
       ccell[i] = (int)cell;
-      ctag[i] = anyhit ? T_SKIP : T_UNDECIDED;
+      ctag[i] = FILTER && anyhit ? T_SKIP : T_UNDECIDED;
       if (a < CSLOT)
         cslot[(size_t)cell * CSLOT + a] = i;
       else
         next[i] = atomicExch(&ovhead[cell], i);
       // more than list_thr (<= CSLOT) candidates in a cell: the first round's
       // kernels give up and the host builds the lists (k_lists) and redoes it
-      if (a == list_thr) atomicOr(&ctl->long_list, 1);
+      if (a == list_thr) atomicOr(&ctl->long_list[draw_id & 1], 1);
       owner[i] = a == 0;
     }
   }
 }
 
-template <int D, bool WIDE>
+template <int D, bool WIDE, bool FILTER>
 __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict__ ring, uint64_t mask, uint64_t w0,
                                                       int nshots, uint32_t N, const uint32_t* __restrict__ vpos,
                                                       const double* __restrict__ hrad, RejList rl,
@@ -857,7 +880,58 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
                                                       int* __restrict__ ovhead, int* __restrict__ next,
                                                       DrawCtl* __restrict__ ctl, int draw_id, int forced,
                                                       int list_thr) {
-  k_decode_filter_body<D, WIDE>(ring, mask, w0, nshots, N, vpos, hrad, rl, end_rel, vx, cls, g, cs, cpos, crad, cprio, cphase, ccell, ctag, owner, cellcc, cslot, ovhead, next, ctl, draw_id, forced, list_thr, (int)blockIdx.x);
+  k_decode_filter_body<D, WIDE, FILTER>(ring, mask, w0, nshots, N, vpos, hrad, rl, end_rel, vx, cls, g, cs, cpos, crad, cprio, cphase, ccell, ctag, owner, cellcc, cslot, ovhead, next, ctl, draw_id, forced, list_thr, (int)blockIdx.x);
+}
+
+// A first round's validated filter after a decode without it (k_decode_filter
+// with FILTER false): one lane per stencil cell of each candidate, as in
+// k_decode_filter, reading the candidate back.  A hit makes it Skip.  When the
+// draw's gate is closed (the decode ran speculatively and the draw before it
+// ended otherwise), the decode's arrivals are undone instead: every cell a
+// candidate arrived in gets its count and overflow list reset, and the draw's
+// per-parity words are cleared, so the buffers serve the draw after next.
+template <int D>
+__global__ void __launch_bounds__(WG) k_filter(Geo<D> g, CellStore<D> cs, int nshots, const double* __restrict__ cpos,
+                                               const double* __restrict__ crad, const int* __restrict__ ccell,
+                                               uint8_t* __restrict__ ctag, int* __restrict__ cellcc,
+                                               int* __restrict__ ovhead, DrawCtl* __restrict__ ctl, int draw_id) {
+  using LG = LaneGroup<D>;
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = gtid / LG::LPC;
+  const int sub = gtid % LG::LPC;
+  const bool valid = i < nshots;
+  const int ic = valid ? i : 0;
+  // every load issued before any test (indices clamped: in bounds whatever they hold)
+  const int gate = load_late(&ctl->gate[draw_id & 1]);
+  int pk[LG::CPL];
+  lane_stencil<D, LG::CPL, LG::NST>(g, sub, pk);
+  const int c = load_now(ccell + ic);
+  const uint8_t t = load_now(ctag + ic);
+  double p[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) p[d] = load_now(cpos + (size_t)d * nshots + ic);
+  const double r = load_now(crad + ic);
+  const bool live = gate == draw_id;
+  const bool in_cell = valid && c >= 0 && c < g.ncells;
+  bool hit = false;
+  if (live && in_cell && t == T_UNDECIDED) {
+    int unused = -1;
+    hit = filter_walk<D>(g, cs, c, sub, pk, p, r, false, nullptr, unused);
+  }
+  const bool anyhit = (__ballot(hit) & LG::mask(lane)) != 0;
+  if (live) {
+    if (in_cell && sub == 0 && anyhit) ctag[i] = T_SKIP;
+    return;
+  }
+  if (in_cell && sub == 0) {
+    cellcc[c] = 0;
+    ovhead[c] = -1;
+  }
+  if (gtid == 0) {
+    ctl->long_list[draw_id & 1] = 0;
+    ctl->dropped[draw_id & 1] = 0;
+  }
 }
 
 // The list of a cell of n <= CSLOT candidates as k_lists builds it: the arrival
@@ -1238,7 +1312,7 @@ __device__ __forceinline__ void k_partner_lists_body(Geo<D> g, int nshots, const
   // slots; the later ones read the lists k_sweep_commit kept.
   const bool first = gate_id >= 0;
   // tested before the first store: read whatever `first` says, so that the loads overlap
-  const int gate_ld = load_late(&ctl->gate[gate_id & 1]), long_ld = load_late(&ctl->long_list);
+  const int gate_ld = load_late(&ctl->gate[gate_id & 1]), long_ld = load_late(&ctl->long_list[gate_id & 1]);
   const int gate = first ? gate_ld : gate_id;
   const int long_list = first ? long_ld : 0;
   const int gtid = bid * blockDim.x + threadIdx.x;
@@ -1267,7 +1341,12 @@ __device__ __forceinline__ void k_partner_lists_body(Geo<D> g, int nshots, const
     n = lane_partners<D>(g, cc, sub, nshots, p, r, pi, cpos, crad, cprio, ctag, cell2t, seg, len, ord, pk,
                          first ? cellcc : nullptr, cslot, i, nullptr, fg, &mism, &keep);
   }
-  if (gate != gate_id) return;  // whole grid
+  if (gate != gate_id) {  // whole grid: the round is not taken
+    // its sweep will not run either, so the unresolved count of the next round
+    // (the parity of the round before this one) is cleared here
+    if (gtid == 0) ctl->nunres[(seq + 1) & 1] = 0;
+    return;
+  }
   if (long_list) return;  // a cell of more than CSLOT candidates: the host builds the lists and redoes the round
   if (!active && sub == 0 && i < nshots) lab[i] = T_DROPPED;
   if (mism) atomicAdd(&ctl->img_mismatch, mism);
@@ -1396,7 +1475,7 @@ __device__ __forceinline__ void k_sweep_commit_body(
   // read under a test or a || is waited for at once); stores come only once the
   // gate is known open.
   const int gate_ld = load_late(&ctl->gate[gate_id & 1]), povf_ld = load_late(&ctl->pool_overflow);
-  const int long_ld = load_late(&ctl->long_list), nu_ld = load_late(&ctl->nunres[seq & 1]);
+  const int long_ld = load_late(&ctl->long_list[gate_id & 1]), nu_ld = load_late(&ctl->nunres[seq & 1]);
   const int gate = first ? gate_ld : gate_id;
   const bool ovf = povf_ld != 0;
   // a first round that met a cell of more than CSLOT candidates did nothing: the
@@ -1407,11 +1486,11 @@ __device__ __forceinline__ void k_sweep_commit_body(
   // k_partner_lists, so they are read now, with the loads above
   int dropped_ld = 0, mism_ld = 0, need_ld = 0, cell_ld = 0, accd_ld = 0;
   if (pub) {
-    dropped_ld = load_late(&ctl->dropped);
+    dropped_ld = load_late(&ctl->dropped[draw_id & 1]);
     mism_ld = load_late(&ctl->img_mismatch);
     need_ld = load_late(&ctl->need_pool);
     cell_ld = load_late(&ctl->cell_overflow);
-    accd_ld = load_late(&ctl->acc_draw);
+    accd_ld = load_late(&ctl->acc_draw[draw_id & 1]);
   }
   // The owner's inputs, read without a branch (index clamped to candidate 0):
   // its owner flag and cell, and its list -- in a draw's first round the cell's
@@ -1623,7 +1702,7 @@ __device__ __forceinline__ void k_sweep_commit_body(
   if (pub) {
     if (tid != 0) return;
     // counters whose values go into the publication below, cleared for the next round
-    ctl->dropped = 0;
+    ctl->dropped[draw_id & 1] = 0;
     ctl->img_mismatch = 0;
     ctl->seg_cursor = 0;
     ctl->nunres[(seq + 1) & 1] = 0;
@@ -1635,7 +1714,7 @@ __device__ __forceinline__ void k_sweep_commit_body(
     // difference placed before the draw (non-zero on a proceed's first draw).
     int open = 0;
     const int acc_draw = redo ? accd_ld : accd_ld + acc;
-    ctl->acc_draw = acc_draw;
+    ctl->acc_draw[draw_id & 1] = acc_draw;
     if (!redo && !stuck && con == 0) {
       const uint64_t miss = (uint64_t)nshots - (uint64_t)acc_draw - miss_base;
       const double rate = (double)miss / (0.000001 + (double)(uint64_t)nshots);
@@ -1816,6 +1895,82 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_commit(
   k_sweep_commit_body<D>(cs, nshots, cpos, crad, cphase, ccell, owner, ctag, lab, slab, pool, urec, seg, len, ord,
                          cell2t, ctl, seq, gate_id, draw_id, aimed, miss_base, host, cellcc, cslot, ncells,
                          (int)blockIdx.x);
+}
+
+// A draw's sweep and commit together with the next draw's decode, in one
+// launch: the next draw's shots are decoded (without the filter, and whatever
+// this draw decides: k_filter checks the gate) while this one's owners commit.
+// The two use different buffers (per parity of the draw id) and different
+// control words; workgroups 0 .. nsc - 1 run the sweep and commit, the others the
+// decode, kSweepThreads threads each.
+template <int D>
+struct SweepArgs {
+  CellStore<D> cs;
+  int nshots;
+  const double* cpos;
+  const double* crad;
+  const uint32_t* cphase;
+  const int* ccell;
+  const uint8_t* owner;
+  uint8_t* ctag;
+  const uint8_t* lab;
+  const int* slab;
+  const int* pool;
+  const URec* urec;
+  int* seg;
+  int* len;
+  int* ord;
+  int* cell2t;
+  DrawCtl* ctl;
+  int seq, gate_id, draw_id;
+  double aimed;
+  uint64_t miss_base;
+  DrawPub* host;
+  int* cellcc;
+  const int* cslot;
+  int ncells;
+};
+template <int D>
+struct DecodeArgs {
+  const uint32_t* ring;
+  uint64_t mask, w0;
+  int nshots;
+  uint32_t N;
+  const uint32_t* vpos;
+  const double* hrad;
+  RejList rl;
+  int end_rel;
+  VoxView<D> vx;
+  ClassDev cls;
+  Geo<D> g;
+  CellStore<D> cs;
+  double* cpos;
+  double* crad;
+  uint32_t* cprio;
+  uint32_t* cphase;
+  int* ccell;
+  uint8_t* ctag;
+  uint8_t* owner;
+  int* cellcc;
+  int* cslot;
+  int* ovhead;
+  int* next;
+  DrawCtl* ctl;
+  int draw_id, forced, list_thr;
+};
+template <int D, bool WIDE>
+__global__ void __launch_bounds__(kSweepThreads) k_sweep_commit_decode(SweepArgs<D> s, int nsc, DecodeArgs<D> d) {
+  const int b = (int)blockIdx.x;
+  if (b < nsc) {
+    k_sweep_commit_body<D>(s.cs, s.nshots, s.cpos, s.crad, s.cphase, s.ccell, s.owner, s.ctag, s.lab, s.slab, s.pool,
+                           s.urec, s.seg, s.len, s.ord, s.cell2t, s.ctl, s.seq, s.gate_id, s.draw_id, s.aimed,
+                           s.miss_base, s.host, s.cellcc, s.cslot, s.ncells, b);
+  } else {
+    k_decode_filter_body<D, WIDE, false>(d.ring, d.mask, d.w0, d.nshots, d.N, d.vpos, d.hrad, d.rl, d.end_rel, d.vx,
+                                         d.cls, d.g, d.cs, d.cpos, d.crad, d.cprio, d.cphase, d.ccell, d.ctag,
+                                         d.owner, d.cellcc, d.cslot, d.ovhead, d.next, d.ctl, d.draw_id, d.forced,
+                                         d.list_thr, b - nsc);
+  }
 }
 
 }  // namespace rsa_amd
