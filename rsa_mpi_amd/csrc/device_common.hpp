@@ -86,9 +86,12 @@ __device__ __host__ __forceinline__ int stencil_off(int packed, int d) {
 template <int D>
 struct CellStore {
   uint8_t* cnt;     // validated spheres per cell
-  Slot<D>* sph;     // cell * cap + k
+  Slot<D>* sph;     // cell * cap + k (slots past the count hold NaN until written)
   uint32_t* phase;  // cell * cap + k
   int cap;
+  // per cell, the last draw that appended to it and the count before it:
+  // ((draw id mod 2^23) << 8) | count; -1 before any (null: not kept)
+  int* mod = nullptr;
 };
 
 // compute_cell_idx (rsa_grid_traversal.ixx:373-382)

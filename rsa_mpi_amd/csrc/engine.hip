@@ -527,6 +527,7 @@ class Engine final : public IEngine {
   // cells
   int cap_ = 8;
   DevBuf<uint8_t> cells_cnt_;
+  DevBuf<int> cells_mod_;  // CellStore::mod
   DevBuf<Slot<D>> cells_sph_;
   DevBuf<uint32_t> cells_ph_;
   DevBuf<int> cell2t_, cellcc_;
@@ -637,7 +638,7 @@ class Engine final : public IEngine {
   uint64_t dev_hi_ = 0;  // words below dev_hi_ are copied or in flight
   std::mutex dw_m_;
 
-  CellStore<D> cs() { return CellStore<D>{cells_cnt_.p, cells_sph_.p, cells_ph_.p, cap_}; }
+  CellStore<D> cs() { return CellStore<D>{cells_cnt_.p, cells_sph_.p, cells_ph_.p, cap_, cells_mod_.p}; }
 
   // ---------------------------------------------------------------- kernel profile
   struct KProf {
@@ -849,6 +850,7 @@ class Engine final : public IEngine {
     DevBuf<uint32_t> ph;
     sph.reserve((size_t)geo_.ncells * cap);
     ph.reserve((size_t)geo_.ncells * cap);
+    fill_nan(sph.p, (int64_t)geo_.ncells * cap);  // the slots past the copied ones
     // copied by a kernel on the engine's stream (no copy-engine writes that later kernels read)
     const int64_t total = (int64_t)geo_.ncells * cap_;
     hipLaunchKernelGGL(k_grow_cells<D>, dim3(nblocks(total)), dim3(WG), 0, st_, total, cap_, cap, cells_sph_.p,
@@ -862,12 +864,22 @@ class Engine final : public IEngine {
   }
   double store_rmin_ = 0;  // smallest radius any class so far may place
 
+  // sphere slots set to NaN (CellStore::sph past each count)
+  void fill_nan(Slot<D>* sph, int64_t slots) {
+    static_assert(sizeof(Slot<D>) % sizeof(double) == 0, "slots are whole doubles");
+    const int64_t n = slots * (int64_t)(sizeof(Slot<D>) / sizeof(double));
+    hipLaunchKernelGGL(k_fill_nan, dim3(nblocks(n)), dim3(WG), 0, st_, reinterpret_cast<double*>(sph), n);
+    HIP_OK(hipGetLastError());
+  }
   void alloc_cells() {
     store_rmin_ = gmin_;
     cap_ = cell_cap(gmin_, gmax_);
     cells_cnt_.reserve(geo_.ncells);
     cells_sph_.reserve((size_t)geo_.ncells * cap_);
     cells_ph_.reserve((size_t)geo_.ncells * cap_);
+    fill_nan(cells_sph_.p, (int64_t)geo_.ncells * cap_);
+    cells_mod_.reserve(geo_.ncells);
+    HIP_OK(hipMemsetAsync(cells_mod_.p, 0xff, (size_t)geo_.ncells * sizeof(int), st_));
     cell2t_.reserve(geo_.ncells);
     cellcc_.reserve(geo_.ncells);
     ovhead_.reserve(geo_.ncells);

@@ -351,7 +351,7 @@ void Engine<D>::enqueue_decode(Prep& pr) {
   const unsigned nbl = nblocks((int64_t)pr.nshots * LG::LPC);
   const DecodeArgs<D> a = decode_args(pr, 1);
   timed("k_decode_filter", (double)pr.nshots * (4.0 * (2 * D + 2) + 4.0 * D + 8.0 * (D + 1) + 13.0), [&] {
-    auto kern = a.vx.wide ? k_decode_filter<D, true, false> : k_decode_filter<D, false, false>;
+    auto kern = a.vx.wide ? k_decode_filter<D, true, true> : k_decode_filter<D, false, true>;
     hipLaunchKernelGGL(kern, dim3(nbl), dim3(WG), 0, st_, a.ring, a.mask, a.w0, a.nshots, a.N, a.vpos, a.hrad, a.rl,
                        a.end_rel, a.vx, a.cls, a.g, a.cs, a.cpos, a.crad, a.cprio, a.cphase, a.ccell, a.ctag, a.owner,
                        a.cellcc, a.cslot, a.ovhead, a.next, a.ctl, a.draw_id, a.forced, a.list_thr);

@@ -820,10 +820,11 @@ __device__ __forceinline__ void k_decode_filter_body(const uint32_t* __restrict_
     }
   }
   // The arrival slot does not depend on the filter's outcome (filter_walk takes
-  // it after its first loads; FILTER false: no walk, k_filter runs it later).
-  // With a closed gate nothing is written (the test below returns).  forced: 0,
-  // gated on gate[draw_id & 1]; 1, forced (the launch opens the gate); 2, run
-  // speculatively inside the previous draw's k_sweep_commit_decode (k_filter
+  // it after its first loads; FILTER false: no walk).  With a closed gate
+  // nothing is written (the test below returns).  forced: 0, gated on
+  // gate[draw_id & 1]; 1, forced (the launch opens the gate); 2, run
+  // speculatively inside the previous draw's k_sweep_commit_decode (its walk
+  // may miss spheres that draw is placing, which k_filter tests; k_filter also
   // checks the gate and undoes the arrivals of a draw not taken).
   const bool take = valid && ok && sub == 0 && gate == draw_id;
   int a = -1;
@@ -883,13 +884,19 @@ __global__ void __launch_bounds__(WG) k_decode_filter(const uint32_t* __restrict
   k_decode_filter_body<D, WIDE, FILTER>(ring, mask, w0, nshots, N, vpos, hrad, rl, end_rel, vx, cls, g, cs, cpos, crad, cprio, cphase, ccell, ctag, owner, cellcc, cslot, ovhead, next, ctl, draw_id, forced, list_thr, (int)blockIdx.x);
 }
 
-// A first round's validated filter after a decode without it (k_decode_filter
-// with FILTER false): one lane per stencil cell of each candidate, as in
-// k_decode_filter, reading the candidate back.  A hit makes it Skip.  When the
-// draw's gate is closed (the decode ran speculatively and the draw before it
-// ended otherwise), the decode's arrivals are undone instead: every cell a
-// candidate arrived in gets its count and overflow list reset, and the draw's
-// per-parity words are cleared, so the buffers serve the draw after next.
+// A first round's validated filter, completed: the decode ran its walk while
+// the draw before it committed (k_sweep_commit_decode), so it saw every sphere
+// placed before that draw and may have missed the ones that draw placed (a
+// count read before the commit, or slots not yet written: they hold NaN, which
+// collides with nothing).  Here each candidate not yet Skip is tested against
+// the spheres the draw before it appended to its stencil cells (the cells whose
+// tag names that draw, slots from the count before it); a hit makes it Skip.
+// Testing more spheres than missed is harmless: they are all placed.  One lane
+// per stencil cell of each candidate, as in k_decode_filter.  When the draw's
+// gate is closed (the decode ran speculatively and the draw before it ended
+// otherwise), the decode's arrivals are undone instead: every cell a candidate
+// arrived in gets its count and overflow list reset, and the draw's per-parity
+// words are cleared, so the buffers serve the draw after next.
 template <int D>
 __global__ void __launch_bounds__(WG) k_filter(Geo<D> g, CellStore<D> cs, int nshots, const double* __restrict__ cpos,
                                                const double* __restrict__ crad, const int* __restrict__ ccell,
@@ -914,10 +921,32 @@ __global__ void __launch_bounds__(WG) k_filter(Geo<D> g, CellStore<D> cs, int ns
   const double r = load_now(crad + ic);
   const bool live = gate == draw_id;
   const bool in_cell = valid && c >= 0 && c < g.ncells;
+  const int prev = (draw_id - 1) & 0x7fffff;  // the draw whose commit ran beside the decode
   bool hit = false;
   if (live && in_cell && t == T_UNDECIDED) {
-    int unused = -1;
-    hit = filter_walk<D>(g, cs, c, sub, pk, p, r, false, nullptr, unused);
+    int cc[D];
+    cell_coords<D>(g, c, cc);
+#pragma unroll
+    for (int jj = 0; jj < LG::CPL; jj++) {
+      const int s = sub * LG::CPL + jj;
+      Nb<D> nb;
+      if (s >= LG::NST || hit || !neighbour_at<D>(g, cc, pk[jj], nb)) continue;
+      const int m = load_now(cs.mod + nb.src);
+      const int nv = cs.cnt[nb.src];
+      if ((m >> 8) != prev) continue;
+      const Slot<D>* sl = cs.sph + (size_t)nb.src * cs.cap;
+      for (int k = m & 0xff; k < nv && !hit; k++) {
+        const Slot<D> q = sl[k];
+        double qp[D];
+        if (nb.ghost) {
+          if (!image_of<D>(g, nb, q.c, qp)) continue;
+        } else {
+#pragma unroll
+          for (int d = 0; d < D; d++) qp[d] = q.c[d];
+        }
+        if (collide<D>(r, p, q.r, qp)) hit = true;
+      }
+    }
   }
   const bool anyhit = (__ballot(hit) & LG::mask(lane)) != 0;
   if (live) {
@@ -1843,6 +1872,7 @@ __device__ __forceinline__ void k_sweep_commit_body(
       }
     }
     cs.cnt[c] = (uint8_t)k0;
+    if (cs.mod) cs.mod[c] = ((draw_id & 0x7fffff) << 8) | cnt0;
     cell2t[c] = -1;
     return;
   }
@@ -1880,6 +1910,7 @@ __device__ __forceinline__ void k_sweep_commit_body(
     k0++;
   }
   cs.cnt[c] = (uint8_t)k0;
+  if (cs.mod) cs.mod[c] = ((draw_id & 0x7fffff) << 8) | cnt0;
   cell2t[c] = -1;
 }
 
@@ -1898,8 +1929,9 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_commit(
 }
 
 // A draw's sweep and commit together with the next draw's decode, in one
-// launch: the next draw's shots are decoded (without the filter, and whatever
-// this draw decides: k_filter checks the gate) while this one's owners commit.
+// launch: the next draw's shots are decoded (whatever this draw decides:
+// k_filter checks the gate, and tests the spheres this draw places, which the
+// decode's filter may miss) while this one's owners commit.
 // The two use different buffers (per parity of the draw id) and different
 // control words; workgroups 0 .. nsc - 1 run the sweep and commit, the others the
 // decode, kSweepThreads threads each.
@@ -1966,7 +1998,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_commit_decode(SweepArgs
                            s.urec, s.seg, s.len, s.ord, s.cell2t, s.ctl, s.seq, s.gate_id, s.draw_id, s.aimed,
                            s.miss_base, s.host, s.cellcc, s.cslot, s.ncells, b);
   } else {
-    k_decode_filter_body<D, WIDE, false>(d.ring, d.mask, d.w0, d.nshots, d.N, d.vpos, d.hrad, d.rl, d.end_rel, d.vx,
+    k_decode_filter_body<D, WIDE, true>(d.ring, d.mask, d.w0, d.nshots, d.N, d.vpos, d.hrad, d.rl, d.end_rel, d.vx,
                                          d.cls, d.g, d.cs, d.cpos, d.crad, d.cprio, d.cphase, d.ccell, d.ctag,
                                          d.owner, d.cellcc, d.cslot, d.ovhead, d.next, d.ctl, d.draw_id, d.forced,
                                          d.list_thr, b - nsc);

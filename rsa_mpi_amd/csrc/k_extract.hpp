@@ -102,6 +102,14 @@ __global__ void k_grow_cells(int64_t total, int cap0, int cap1, const Slot<D>* _
   dstph[c * cap1 + k] = srcph[t];
 }
 
+// n doubles set to a quiet NaN: sphere slots not yet written (a NaN centre or
+// radius collides with nothing, so a decode that reads a slot before the commit
+// writing it has finished misses that sphere and k_filter tests it)
+__global__ void k_fill_nan(double* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = __longlong_as_double(0x7ff8000000000000ll);
+}
+
 __global__ void k_set_table(int* __restrict__ dst, StencilTable t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < t.n) dst[i] = t.v[i];
