@@ -280,19 +280,22 @@ __device__ __forceinline__ void k_cover_body(Geo<D> g, CellStore<D> cs, VoxView<
           // (slack >= 0), and the coverage pre-check s2 <= (R^2 - h2) + slack with
           // h2 >= 0 (D <= 3), rounding being monotone.  A sphere with s2 <= U is listed
           // and gets the exact gates in test().
-          auto consider = [&](const Slot<D>& q, uint32_t idx) {
+          // The entry is written at position nl whether or not it is listed (nl < KLIST
+          // there: a full list is flushed at once), so that listing is a count.
+          const Slot<D>* const qe = csph + b;
+          uint32_t idx = (uint32_t)a;
+          for (const Slot<D>* qp = csph + a; qp < qe; qp++, idx++) {
+            const Slot<D> q = *qp;
             double s2 = 0.;
 #pragma unroll
             for (int d = 0; d < D; d++) {
               const double t = q.c[d] - cb[d];
               s2 += t * t;
             }
-            if (s2 <= q.r) {
-              if (nl == KLIST) flush();
-              s_list[nl++][threadIdx.x] = idx;
-            }
-          };
-          for (int k = a; k < b; k++) consider(csph[k], (uint32_t)k);
+            s_list[nl][threadIdx.x] = idx;
+            nl += s2 <= q.r ? 1 : 0;
+            if (nl == KLIST) flush();
+          }
           if (s_last[rw]) flush();  // end of a distance shell: test what was listed
         }
         flush();
