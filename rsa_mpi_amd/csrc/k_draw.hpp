@@ -970,6 +970,15 @@ __global__ void __launch_bounds__(WG) k_filter(Geo<D> g, CellStore<D> cs, int ns
 // candidate index (the tag loads of unused entries read it, without a branch).
 // The first n arrival slots in shot order (entries n.. become INT_MAX):
 // 19-comparator network for 8 keys (Batcher odd-even merge).
+// 63-comparator network for 16 keys (Batcher odd-even merge; it sorts every
+// 0-1 input, hence every input)
+constexpr int kSortNet16[63][2] = {
+    {0, 1},  {2, 3},   {0, 2},   {1, 3},   {1, 2},  {4, 5},   {6, 7},  {4, 6},   {5, 7},   {5, 6},   {0, 4},
+    {2, 6},  {2, 4},   {1, 5},   {3, 7},   {3, 5},  {1, 2},   {3, 4},  {5, 6},   {8, 9},   {10, 11}, {8, 10},
+    {9, 11}, {9, 10},  {12, 13}, {14, 15}, {12, 14}, {13, 15}, {13, 14}, {8, 12}, {10, 14}, {10, 12}, {9, 13},
+    {11, 15}, {11, 13}, {9, 10}, {11, 12}, {13, 14}, {0, 8},  {4, 12}, {4, 8},   {2, 10},  {6, 14},  {6, 10},
+    {2, 4},  {6, 8},   {10, 12}, {1, 9},   {5, 13}, {5, 9},   {3, 11}, {7, 15},  {7, 11},  {3, 5},   {7, 9},
+    {11, 13}, {1, 2},  {3, 4},   {5, 6},   {7, 8},  {9, 10},  {11, 12}, {13, 14}};
 constexpr int kSortNet[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
                                  {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
 __device__ __forceinline__ void sort_slots(int n, int (&v)[CSLOT]) {
@@ -1062,6 +1071,43 @@ __device__ __forceinline__ void k_lists_body(int nshots, const int* __restrict__
 #pragma unroll
     for (int k = 0; k < CSLOT; k++)
       if (k < m) a[k] = v[k];
+  } else if (n <= 2 * CSLOT) {
+    // in registers too (a long list of a clustered late draw): the fixed slots,
+    // then the overflow list (the count says how long it is), a 63-comparator
+    // network (Batcher odd-even merge for 16 keys), the tags together, and the
+    // round-0 Skips removed from the back with swap-with-last
+    constexpr int L2S = 2 * CSLOT;
+    int v[L2S] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+#pragma unroll
+    for (int q = CSLOT; q < L2S; q++) v[q] = INT_MAX;
+    int j = ovhead[c];
+    for (int k = CSLOT; k < n && j != -1; k++) {
+#pragma unroll
+      for (int q = CSLOT; q < L2S; q++) v[q] = k == q ? j : v[q];
+      j = next[j];
+    }
+#pragma unroll
+    for (int e = 0; e < 63; e++) {
+      const int x = v[kSortNet16[e][0]], y = v[kSortNet16[e][1]];
+      v[kSortNet16[e][0]] = min(x, y);
+      v[kSortNet16[e][1]] = max(x, y);
+    }
+    uint8_t tg[L2S];
+#pragma unroll
+    for (int q = 0; q < L2S; q++) tg[q] = ctag[(q < n && (unsigned)v[q] < (unsigned)nshots) ? v[q] : i];
+#pragma unroll
+    for (int q = L2S - 1; q >= 0; q--) {
+      if (q < n && tg[q] == T_SKIP) {
+        int x = v[q];
+#pragma unroll
+        for (int k = q + 1; k < L2S; k++) x = (m - 1 == k) ? v[k] : x;
+        v[q] = x;
+        m--;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < L2S; q++)
+      if (q < m) a[q] = v[q];
   } else {
     for (int k = 0; k < CSLOT; k++) a[k] = cslot[(size_t)c * CSLOT + k];
     int k = CSLOT;
