@@ -293,3 +293,37 @@ def test_listing_threshold_contains_both_gates():
         passes += gate
         assert not gate or s2 <= U, (s2, U, R2, h2, sl)
     assert passes > 20000
+
+
+def _network(name):
+    """A comparator table of the draw kernels, read from their source."""
+    src = open(os.path.join(ROOT, "rsa_mpi_amd", "csrc", "k_draw.hpp")).read()
+    m = re.search(r"constexpr int %s\[(\d+)\]\[2\] = \{(.*?)\};" % name, src, re.S)
+    assert m, name
+    pairs = [(int(a), int(b)) for a, b in re.findall(r"\{\s*(\d+),\s*(\d+)\s*\}", m.group(2))]
+    assert len(pairs) == int(m.group(1))
+    return pairs
+
+
+def _sorts_all_01(net, keys, skip=-1):
+    v = ((np.arange(1 << keys)[:, None] >> np.arange(keys)[None, :]) & 1).astype(np.uint8)
+    for k, (a, b) in enumerate(net):
+        if k == skip:
+            continue
+        lo, hi = np.minimum(v[:, a], v[:, b]), np.maximum(v[:, a], v[:, b])
+        v[:, a], v[:, b] = lo, hi
+    return bool((np.diff(v.astype(np.int8), axis=1) >= 0).all())
+
+
+@pytest.mark.parametrize("name,keys", [("kSortNet", 8), ("kSortNet16", 16)])
+def test_sorting_networks_sort_every_input(name, keys):
+    """The cell lists are put in shot order by fixed comparator networks (k_lists,
+    the partner walk, the sweep's commit).  By the 0-1 principle a comparator
+    network sorts every input iff it sorts every 0-1 input: all 2^keys of them
+    are run through it.  Every comparator of both networks is needed, so a table
+    with any one comparator missing fails the same check."""
+    net = _network(name)
+    assert all(0 <= a < b < keys for a, b in net)
+    assert _sorts_all_01(net, keys)
+    for drop in range(len(net)):
+        assert not _sorts_all_01(net, keys, drop), (name, drop)
