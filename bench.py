@@ -40,7 +40,7 @@ def source_sha16():
     h = hashlib.sha256()
     d = os.path.join(ROOT, "rsa_mpi_amd", "csrc")
     for f in sorted(os.listdir(d)):
-        if f.endswith((".hip", ".hpp", ".cpp", ".h")):
+        if f.endswith((".hip", ".hpp", ".cpp", ".h", ".inl")):
             h.update(f.encode())
             h.update(open(os.path.join(d, f), "rb").read())
     return h.hexdigest()[:16]
